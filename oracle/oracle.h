@@ -1,0 +1,123 @@
+/*
+ * oracle.h - CPU restatement of the 1d-mctdhb configurational hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the GPU library is
+ * compared against; only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  It is never linked into, or
+ * called from, the product path (1d-mctdhb_amd/).
+ *
+ * Every function restates the algorithm of the reference file:line cited in
+ * its comment, in the same loop and summation order, in FP64 complex.
+ * Complex arrays are interleaved (re, im) doubles, i.e. C99 double complex.
+ * Matrices are contiguous row-major (the reference uses row-pointer Cmatrix;
+ * element [k][l] is stored at k*M + l here).
+ */
+#ifndef MCTDHB_ORACLE_H
+#define MCTDHB_ORACLE_H
+
+#include <complex.h>
+#include <stdint.h>
+
+typedef double complex ocplx;
+
+/* ---------------------------------------------------------------- space */
+
+typedef struct
+{
+    uint16_t  npar, norb;
+    uint32_t  dim;
+    uint16_t* hash_table; /* dim x norb, row-major */
+    uint32_t* subsdim;    /* (npar+1) x (norb+1) */
+    uint32_t* map;        /* norb*norb*dim */
+    uint32_t* mapot;      /* n_ot * norb^2 */
+    uint32_t* maptt;      /* n_tt * norb^2 */
+    uint32_t* strideot;   /* dim */
+    uint32_t* stridett;   /* dim */
+    uint64_t  mapot_len, maptt_len;
+} orc_space;
+
+int      orc_space_dimension(uint16_t npar, uint16_t norb, uint32_t* dim);
+void     orc_subspaces_dim(uint16_t npar, uint16_t norb, uint32_t* subsdim);
+void     orc_set_config(uint32_t hash_i, uint16_t npar, uint16_t norb, uint16_t* conf);
+uint32_t orc_hash_index(uint16_t npar, uint16_t norb, const uint32_t* subsdim, const uint16_t* conf);
+void     orc_hash_table(uint16_t npar, uint16_t norb, uint16_t* ht);
+void     orc_single_jump_map(uint16_t npar, uint16_t norb, const uint32_t* subsdim,
+                             const uint16_t* ht, uint32_t* map);
+uint64_t orc_count_diffjump(uint32_t dim, uint16_t norb, const uint16_t* ht);
+uint64_t orc_count_equaljump(uint32_t dim, uint16_t norb, const uint16_t* ht);
+void     orc_double_diffjump_map(uint16_t npar, uint16_t norb, const uint32_t* subsdim,
+                                 const uint16_t* ht, uint32_t* strides, uint32_t* map);
+void     orc_double_equaljump_map(uint16_t npar, uint16_t norb, const uint32_t* subsdim,
+                                  const uint16_t* ht, uint32_t* strides, uint32_t* map);
+orc_space* orc_space_create(uint16_t npar, uint16_t norb);
+void       orc_space_destroy(orc_space* S);
+
+/* ------------------------------------------------------ configurational */
+
+void orc_apply_hamiltonian(const orc_space* S, const ocplx* coef, const ocplx* hob,
+                           const ocplx* hint, ocplx* hc);
+void orc_onebody_dm(const orc_space* S, const ocplx* coef, ocplx* rho);
+void orc_twobody_dm(const orc_space* S, const ocplx* coef, ocplx* rho);
+void orc_total_energy(int norb, const ocplx* rho1, const ocplx* rho2, const ocplx* hob,
+                      const ocplx* hint, ocplx* energy);
+
+/* ------------------------------------------------------ lanczos and SIL */
+
+ocplx  orc_carr_dot(uint32_t n, const ocplx* a, const ocplx* b);
+double orc_carr_mod(uint32_t n, const ocplx* a);
+int    orc_tridiag_eig(int n, double* d, double* e, double* z);
+int    orc_lanczos(const orc_space* S, const ocplx* hob, const ocplx* hint, int lm,
+                   ocplx* diag, ocplx* offd, ocplx* lvec);
+int    orc_sil_step(const orc_space* S, const ocplx* hob, const ocplx* hint, int lm,
+                    double prop_dt_re, double prop_dt_im, int imagtime, ocplx* coef,
+                    ocplx* lvec_work);
+double orc_lowest_state_lanczos(const orc_space* S, const ocplx* hob, const ocplx* hint,
+                                int niter, ocplx* coef);
+
+/* -------------------------------------------------------------- orbital */
+
+typedef struct
+{
+    int      norb, grid_size, bounds; /* bounds: 1000 zero, 1001 periodic */
+    double   xi, xf, dx, tstep, g, d2coef;
+    double   d1coef_re, d1coef_im;
+    double   prop_dt_re, prop_dt_im;   /* imag time: (0,-dt); real time: (dt,0) */
+    double   time_fac_re, time_fac_im; /* imag time: (0,-1);  real time: (1,0)  */
+    const double* pot;                 /* grid_size */
+} orc_orbeq;
+
+ocplx orc_csimps(uint32_t n, double h, const ocplx* f);
+double orc_rsimps(uint32_t n, double h, const double* f);
+void  orc_dxfd(uint32_t n, double dx, const ocplx* f, ocplx* dfdx);
+ocplx orc_scalar_product(uint32_t n, double h, const ocplx* fstar, const ocplx* f);
+void  orc_renormalize(uint32_t n, double dx, double new_norm, ocplx* f);
+void  orc_orthonormalize(uint32_t npts, double dx, uint32_t nfun, ocplx* F);
+void  orc_orbital_hob(const orc_orbeq* eq, const ocplx* orb, ocplx* hob);
+void  orc_orbital_hint(const orc_orbeq* eq, const ocplx* orb, ocplx* hint);
+void  orc_cn_tridiagonal(const orc_orbeq* eq, ocplx* upper, ocplx* lower, ocplx* mid);
+void  orc_cn_rhs(const orc_orbeq* eq, const ocplx* f, ocplx* out);
+void  orc_solve_tridiag(uint32_t n, const ocplx* upper, const ocplx* lower, const ocplx* mid,
+                        const ocplx* rhs, ocplx* ans);
+void  orc_solve_cyclic_tridiag_sm(uint32_t n, const ocplx* upper, const ocplx* lower,
+                                  ocplx* mid, const ocplx* rhs, ocplx* ans);
+void  orc_advance_linear_cn(const orc_orbeq* eq, ocplx* orb);
+void  orc_hder_fftexp(const orc_orbeq* eq, ocplx* hder_exp);
+void  orc_dft(int n, int sign, double scale, const ocplx* in, ocplx* out);
+void  orc_advance_linear_fft(const orc_orbeq* eq, ocplx* orb);
+ocplx orc_orb_full_nonlinear(int k, int n, double g, int norb, int grid_size, const ocplx* orb,
+                             const ocplx* hob, const ocplx* hint, const ocplx* rho2,
+                             const ocplx* rinv);
+void  orc_nonlinear_rhs(const orc_orbeq* eq, const ocplx* orb, const ocplx* hob,
+                        const ocplx* hint, const ocplx* rho2, const ocplx* rinv, ocplx* der);
+int   orc_herm_eig(int n, const ocplx* A, double* eigvals, ocplx* eigvec);
+int   orc_herm_inverse(int n, const ocplx* A, ocplx* Ainv);
+void  orc_regularization(int n, double x, ocplx* A);
+void  orc_sync_density(const orc_space* S, const ocplx* coef, ocplx* rho1, ocplx* rho2,
+                       ocplx* rinv, double reg_factor);
+void  orc_splitstep_orb(const orc_orbeq* eq, int imagtime, int use_fft, ocplx* orb,
+                        ocplx* hob, ocplx* hint, const ocplx* rho2, const ocplx* rinv);
+void  orc_propagate_step(const orc_space* S, orc_orbeq* eq, int imagtime, int use_fft,
+                         int lanczos_iter, ocplx* orb, ocplx* coef, ocplx* hob, ocplx* hint,
+                         ocplx* rho1, ocplx* rho2, ocplx* rinv, ocplx* lvec_work);
+
+#endif
