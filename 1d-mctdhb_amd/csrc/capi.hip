@@ -1,0 +1,679 @@
+// capi.hip - the C ABI of libmctdhb_hip.so (include/mctdhb_hip.h).
+//
+// Layer 1 keeps the reference's names, signatures and fail-stop error model
+// (src/configurational/*.c, src/linalg/multiconfig_lanczos.c,
+// src/integrator/coefficients_integration.c, src/assistant/types_definition.c)
+// on host arrays, moving the work to the GPU.  Layer 2 (mctdhb_*) is the
+// device-resident, status-returning API the drop-in layer is built on.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "density.hpp"
+#include "hamiltonian.hpp"
+#include "mctdhb_hip.h"
+#include "sil_api.hpp"
+
+namespace mctdhb {
+
+static thread_local std::string g_error;
+void set_error(const std::string& msg) { g_error = msg; }
+const char* last_error() { return g_error.c_str(); }
+
+}  // namespace mctdhb
+
+using namespace mctdhb;
+
+struct mctdhb_space {
+    Space* S = nullptr;
+    void* dm_work = nullptr;
+    // scratch of the reference-named host entry points
+    double2* d_coef = nullptr;
+    double2* d_out = nullptr;
+    double2* d_hob = nullptr;
+    double2* d_hint = nullptr;
+    double2* d_rho1 = nullptr;
+    double2* d_rho2 = nullptr;
+    double2* d_lvec = nullptr;
+    mctdhb_sil* sil = nullptr;  // cached SIL workspace of the host entry points
+};
+
+struct mctdhb_sil {
+    Sil* L = nullptr;
+    const mctdhb_space* sp = nullptr;
+};
+
+// ------------------------------------------------------------------ layer 2
+
+extern "C" const char* mctdhb_last_error(void) { return last_error(); }
+
+extern "C" int mctdhb_device_count(int* count) {
+    MCTDHB_HIP_CHECK(hipGetDeviceCount(count));
+    return kOk;
+}
+
+extern "C" int mctdhb_space_create(uint16_t npar, uint16_t norb, int flags, mctdhb_space** out) {
+    if (!out) {
+        set_error("null output pointer");
+        return kInvalidArgument;
+    }
+    *out = nullptr;
+    Space* S = nullptr;
+    int rc = space_create(npar, norb, flags, &S);
+    if (rc) return rc;
+    mctdhb_space* sp = new mctdhb_space();
+    sp->S = S;
+    *out = sp;
+    return kOk;
+}
+
+static void free_scratch(mctdhb_space* sp) {
+    hipFree(sp->dm_work);
+    hipFree(sp->d_coef);
+    hipFree(sp->d_out);
+    hipFree(sp->d_hob);
+    hipFree(sp->d_hint);
+    hipFree(sp->d_rho1);
+    hipFree(sp->d_rho2);
+    hipFree(sp->d_lvec);
+    if (sp->sil) {
+        sil_destroy(sp->sil->L);
+        delete sp->sil;
+    }
+}
+
+extern "C" int mctdhb_space_destroy(mctdhb_space* sp) {
+    if (!sp) return kOk;
+    hipDeviceSynchronize();
+    free_scratch(sp);
+    space_destroy(sp->S);
+    delete sp;
+    return kOk;
+}
+
+extern "C" int mctdhb_space_info(const mctdhb_space* sp, uint32_t* dim, uint64_t* mapot_len, uint64_t* maptt_len,
+                                 int* has_tables) {
+    if (!sp) {
+        set_error("null space");
+        return kInvalidArgument;
+    }
+    if (dim) *dim = sp->S->dim;
+    if (mapot_len) *mapot_len = sp->S->mapot_len;
+    if (maptt_len) *maptt_len = sp->S->maptt_len;
+    if (has_tables) *has_tables = sp->S->tables ? 1 : 0;
+    return kOk;
+}
+
+extern "C" int mctdhb_space_tables(const mctdhb_space* sp, const uint16_t** ht, const uint32_t** subsdim,
+                                   const uint32_t** map, const uint32_t** mapot, const uint32_t** maptt,
+                                   const uint32_t** strideot, const uint32_t** stridett) {
+    if (!sp) {
+        set_error("null space");
+        return kInvalidArgument;
+    }
+    const Space* S = sp->S;
+    if (ht) *ht = S->d_ht;
+    if (subsdim) *subsdim = S->d_subsdim;
+    if (map) *map = S->d_map;
+    if (mapot) *mapot = S->d_mapot;
+    if (maptt) *maptt = S->d_maptt;
+    if (strideot) *strideot = S->d_strideot;
+    if (stridett) *stridett = S->d_stridett;
+    return kOk;
+}
+
+extern "C" int mctdhb_apply_hamiltonian_dev(const mctdhb_space* sp, const void* coef, const void* hob,
+                                            const void* hint, void* hc, int prec, int index, void* stream) {
+    if (!sp || !coef || !hob || !hint || !hc) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    return hc_apply(sp->S, coef, (const double2*) hob, (const double2*) hint, hc, prec, index,
+                    (hipStream_t) stream);
+}
+
+extern "C" int mctdhb_density_dev(mctdhb_space* sp, const void* coef, void* rho1, void* rho2, int prec, int index,
+                                  void* stream) {
+    if (!sp || !coef || !rho1) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    if (prec != kDmFP64 && prec != kDmMixed) {
+        set_error("unknown precision mode");
+        return kInvalidArgument;
+    }
+    if (!sp->dm_work) MCTDHB_HIP_CHECK(hipMalloc(&sp->dm_work, dm_workspace_bytes(sp->S)));
+    return density_matrices(sp->S, coef, (double2*) rho1, (double2*) rho2, prec,
+                            prec == kDmMixed ? (int) kIndexOnTheFly : index, sp->dm_work, (hipStream_t) stream);
+}
+
+extern "C" int mctdhb_sil_create(const mctdhb_space* sp, int lanczos_iter, int prec, int index, mctdhb_sil** out) {
+    if (!sp || !out) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    *out = nullptr;
+    Sil* L = nullptr;
+    int rc = sil_create(sp->S, lanczos_iter, prec, index, &L);
+    if (rc) return rc;
+    mctdhb_sil* s = new mctdhb_sil();
+    s->L = L;
+    s->sp = sp;
+    *out = s;
+    return kOk;
+}
+
+extern "C" int mctdhb_sil_destroy(mctdhb_sil* s) {
+    if (!s) return kOk;
+    hipDeviceSynchronize();
+    sil_destroy(s->L);
+    delete s;
+    return kOk;
+}
+
+extern "C" size_t mctdhb_sil_device_bytes(const mctdhb_space* sp, int lanczos_iter, int prec) {
+    return sp ? sil_device_bytes(sp->S, lanczos_iter, prec) : 0;
+}
+
+extern "C" int mctdhb_sil_step_dev(mctdhb_sil* s, void* coef, const void* hob, const void* hint, double dt_re,
+                                   double dt_im, int imagtime, void* stream) {
+    if (!s || !coef || !hob || !hint) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    return sil_step(s->L, (double2*) coef, (const double2*) hob, (const double2*) hint, dt_re, dt_im, imagtime,
+                    (hipStream_t) stream);
+}
+
+extern "C" int mctdhb_lanczos_dev(mctdhb_sil* s, const void* coef, const void* hob, const void* hint, void* lvec,
+                                  void* stream) {
+    if (!s || !coef || !hob || !hint || !lvec) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    return sil_lanczos(s->L, (const double2*) coef, (const double2*) hob, (const double2*) hint, (double2*) lvec,
+                       (hipStream_t) stream);
+}
+
+extern "C" int mctdhb_sil_scalars(const mctdhb_sil* s, int* iter_done, double* diag, double* offd, void* stream) {
+    if (!s) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    SilState h;
+    int rc = sil_read_state(s->L, &h, (hipStream_t) stream);
+    if (rc) return rc;
+    if (h.eig_fail) {
+        set_error("tridiagonal eigensolver did not converge");
+        return kNumerical;
+    }
+    if (iter_done) *iter_done = h.iter_done;
+    for (int k = 0; k < s->L->lm; k++) {
+        if (diag) {
+            diag[2 * k] = h.alpha[k].x;
+            diag[2 * k + 1] = h.alpha[k].y;
+        }
+        if (offd) offd[k] = h.beta[k];
+    }
+    return kOk;
+}
+
+// ------------------------------------------------------------------ layer 1
+
+[[noreturn]] static void fail_stop(const char* what) {
+    printf("\n\nERROR: %s: %s\n\n", what, last_error());
+    exit(EXIT_FAILURE);
+}
+
+static void* xmalloc(size_t bytes) {
+    void* p = malloc(bytes ? bytes : 1);
+    if (!p) {
+        printf("\n\nMEMORY ERROR : malloc failed to allocate %zu bytes\n\n", bytes);
+        exit(EXIT_FAILURE);
+    }
+    return p;
+}
+
+extern "C" void assert_space_parameters(uint16_t npar, uint16_t norb) {
+    if (npar == 0 || norb == 0) {
+        printf("\n\nInvalid number of particles %u or orbitals %u\n\n", npar, norb);
+        exit(EXIT_FAILURE);
+    }
+    if (norb > MAX_ORBITALS) {
+        printf("\nERROR: Exceeded max number of orbitals %d\n", MAX_ORBITALS);
+        exit(EXIT_FAILURE);
+    }
+    if (npar > MAX_PARTICLES) {
+        printf("\nERROR: Exceeded max number of particles %d\n", MAX_PARTICLES);
+        exit(EXIT_FAILURE);
+    }
+}
+
+extern "C" uint32_t space_dimension(uint16_t npar, uint16_t norb) {
+    uint32_t d = 0;
+    if (space_dimension_exact(npar, norb, &d) != kOk) {
+        printf("\n\nINTEGER SIZE ERROR : overflow occurred representing the space dimension as 32-bit "
+               "integers for %u particles and %u orbitals\n\n",
+               npar, norb);
+        exit(EXIT_FAILURE);
+    }
+    return d;
+}
+
+extern "C" uint32_t* get_subspaces_dim(uint16_t npar, uint16_t norb) {
+    uint32_t* s = (uint32_t*) xmalloc(sizeof(uint32_t) * (norb + 1) * (npar + 1));
+    for (uint32_t n = 0; n <= npar; n++) {
+        s[n] = 0;
+        s[n + (npar + 1)] = 1;
+        for (uint32_t m = 2; m <= norb; m++) s[n + (npar + 1) * m] = (uint32_t) binom_u64(n + m - 1, m - 1);
+    }
+    return s;
+}
+
+extern "C" void set_config(uint32_t hash_i, uint16_t npar, uint16_t norb, uint16_t* conf) {
+    unrank_host(hash_i, npar, norb, conf);
+}
+
+extern "C" uint32_t hash_index(uint16_t npar, uint16_t norb, uint32_t* subspaces_dim, uint16_t* conf) {
+    (void) subspaces_dim;  // the closed-form rank needs no table
+    return rank_host(npar, norb, conf);
+}
+
+// Device space for a MultiConfiguration (built on first use if the caller
+// assembled the struct itself).
+static mctdhb_space* ensure_dev(MultiConfiguration mc) {
+    if (!mc->dev) {
+        mctdhb_space* sp = nullptr;
+        if (mctdhb_space_create(mc->npar, mc->norb, MCTDHB_SPACE_TABLES, &sp)) fail_stop("device space");
+        mc->dev = sp;
+    }
+    return (mctdhb_space*) mc->dev;
+}
+
+template <typename T> static T* d2h_array(const T* d, size_t n) {
+    T* h = (T*) xmalloc(sizeof(T) * n);
+    if (n && hipMemcpy(h, d, sizeof(T) * n, hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("device to host copy failed");
+        fail_stop("table copy");
+    }
+    return h;
+}
+
+static mctdhb_space* temp_space(uint16_t npar, uint16_t norb) {
+    assert_space_parameters(npar, norb);
+    mctdhb_space* sp = nullptr;
+    if (mctdhb_space_create(npar, norb, MCTDHB_SPACE_TABLES, &sp)) fail_stop("device space");
+    return sp;
+}
+
+extern "C" uint16_t* get_hash_table(uint16_t npar, uint16_t norb) {
+    mctdhb_space* sp = temp_space(npar, norb);
+    uint16_t* h = d2h_array(sp->S->d_ht, (size_t) sp->S->dim * norb);
+    mctdhb_space_destroy(sp);
+    return h;
+}
+
+extern "C" uint32_t* get_single_jump_map(uint16_t npar, uint16_t norb, uint32_t* subsdim, uint16_t* hash_table) {
+    (void) subsdim;
+    (void) hash_table;
+    mctdhb_space* sp = temp_space(npar, norb);
+    uint32_t* h = d2h_array(sp->S->d_map, (size_t) sp->S->dim * norb * norb);
+    mctdhb_space_destroy(sp);
+    return h;
+}
+
+extern "C" uint32_t* get_double_diffjump_map(uint16_t npar, uint16_t norb, uint32_t* subsdim, uint16_t* hash_table,
+                                             uint32_t* strides) {
+    (void) subsdim;
+    (void) hash_table;
+    mctdhb_space* sp = temp_space(npar, norb);
+    if (hipMemcpy(strides, sp->S->d_stridett, sizeof(uint32_t) * sp->S->dim, hipMemcpyDeviceToHost) != hipSuccess)
+        fail_stop("table copy");
+    uint32_t* h = d2h_array(sp->S->d_maptt, sp->S->maptt_len);
+    mctdhb_space_destroy(sp);
+    return h;
+}
+
+extern "C" uint32_t* get_double_equaljump_map(uint16_t npar, uint16_t norb, uint32_t* subsdim, uint16_t* hash_table,
+                                              uint32_t* strides) {
+    (void) subsdim;
+    (void) hash_table;
+    mctdhb_space* sp = temp_space(npar, norb);
+    if (hipMemcpy(strides, sp->S->d_strideot, sizeof(uint32_t) * sp->S->dim, hipMemcpyDeviceToHost) != hipSuccess)
+        fail_stop("table copy");
+    uint32_t* h = d2h_array(sp->S->d_mapot, sp->S->mapot_len);
+    mctdhb_space_destroy(sp);
+    return h;
+}
+
+extern "C" uint32_t jump1_index(uint16_t norb, uint32_t dim, uint32_t inp_index, uint32_t* map1jump,
+                                uint16_t from_orb, uint16_t to_orb) {
+    return map1jump[inp_index + from_orb * dim + to_orb * dim * norb];
+}
+
+extern "C" uint32_t jump2same_index(uint16_t norb, uint16_t* conf, uint32_t conf_stride, uint32_t* mapot,
+                                    uint16_t from_orb, uint16_t to_orb1, uint16_t to_orb2) {
+    uint32_t c = 0;
+    for (int h = 0; h < from_orb; h++) c += conf[h] > 1;
+    return mapot[conf_stride + c * norb * norb + to_orb1 + to_orb2 * norb];
+}
+
+extern "C" uint32_t jump2diff_index(uint16_t norb, uint16_t* conf, uint32_t conf_stride, uint32_t* maptt,
+                                    uint16_t from_orb1, uint16_t from_orb2, uint16_t to_orb1, uint16_t to_orb2) {
+    if (from_orb2 < from_orb1) {
+        uint16_t t = from_orb1;
+        from_orb1 = from_orb2;
+        from_orb2 = t;
+    }
+    uint32_t c = 0;
+    for (int h = 0; h < from_orb1; h++)
+        for (int g = h + 1; g < norb; g++) c += (conf[h] > 0 && conf[g] > 0);
+    for (int g = from_orb1 + 1; g < from_orb2; g++) c += conf[g] > 0;
+    return maptt[conf_stride + c * norb * norb + to_orb1 + to_orb2 * norb];
+}
+
+extern "C" MultiConfiguration get_multiconf_struct(uint16_t npar, uint16_t norb) {
+    assert_space_parameters(npar, norb);
+    mctdhb_space* sp = nullptr;
+    if (mctdhb_space_create(npar, norb, MCTDHB_SPACE_TABLES, &sp)) fail_stop("get_multiconf_struct");
+    const Space* S = sp->S;
+    MultiConfiguration mc = (MultiConfiguration) xmalloc(sizeof(_MultiConfiguration));
+    OperatorMappings om = (OperatorMappings) xmalloc(sizeof(_OperatorMappings));
+    mc->npar = npar;
+    mc->norb = norb;
+    mc->dim = S->dim;
+    mc->subspaces_dim = get_subspaces_dim(npar, norb);
+    mc->hash_table = d2h_array(S->d_ht, (size_t) S->dim * norb);
+    om->strideot = d2h_array(S->d_strideot, S->dim);
+    om->stridett = d2h_array(S->d_stridett, S->dim);
+    om->map = d2h_array(S->d_map, (size_t) S->dim * norb * norb);
+    om->maptt = d2h_array(S->d_maptt, S->maptt_len);
+    om->mapot = d2h_array(S->d_mapot, S->mapot_len);
+    mc->op_maps = om;
+    mc->dev = sp;
+    return mc;
+}
+
+extern "C" void destroy_multiconf_struct(MultiConfiguration mc) {
+    if (!mc) return;
+    free(mc->op_maps->map);
+    free(mc->op_maps->mapot);
+    free(mc->op_maps->maptt);
+    free(mc->op_maps->strideot);
+    free(mc->op_maps->stridett);
+    free(mc->op_maps);
+    free(mc->hash_table);
+    free(mc->subspaces_dim);
+    mctdhb_space_destroy((mctdhb_space*) mc->dev);
+    free(mc);
+}
+
+static Cmatrix alloc_cmatrix(int rows, int cols) {
+    Cmatrix m = (Cmatrix) xmalloc(sizeof(Carray) * rows);
+    for (int i = 0; i < rows; i++) m[i] = (Carray) xmalloc(sizeof(dcomplex) * cols);
+    return m;
+}
+
+static void free_cmatrix(int rows, Cmatrix m) {
+    if (!m) return;
+    for (int i = 0; i < rows; i++) free(m[i]);
+    free(m);
+}
+
+extern "C" ManyBodyState get_manybody_state(uint16_t npar, uint16_t norb, uint16_t grid_size) {
+    assert_space_parameters(npar, norb);
+    ManyBodyState s = (ManyBodyState) xmalloc(sizeof(_ManyBodyState));
+    s->npar = npar;
+    s->norb = norb;
+    s->grid_size = grid_size;
+    s->space_dim = space_dimension(npar, norb);
+    const size_t m4 = (size_t) norb * norb * norb * norb;
+    s->coef = (Carray) xmalloc(sizeof(dcomplex) * s->space_dim);
+    s->hint = (Carray) xmalloc(sizeof(dcomplex) * m4);
+    s->tb_denmat = (Carray) xmalloc(sizeof(dcomplex) * m4);
+    s->ob_denmat = alloc_cmatrix(norb, norb);
+    s->inv_ob_denmat = alloc_cmatrix(norb, norb);
+    s->hob = alloc_cmatrix(norb, norb);
+    s->orbitals = alloc_cmatrix(norb, grid_size);
+    return s;
+}
+
+extern "C" void destroy_manybody_sate(ManyBodyState s) {
+    if (!s) return;
+    free(s->coef);
+    free(s->hint);
+    free(s->tb_denmat);
+    free_cmatrix(s->norb, s->ob_denmat);
+    free_cmatrix(s->norb, s->inv_ob_denmat);
+    free_cmatrix(s->norb, s->hob);
+    free_cmatrix(s->norb, s->orbitals);
+    free(s);
+}
+
+extern "C" WorkspaceLanczos get_lanczos_workspace(uint16_t iter, uint32_t space_dim) {
+    WorkspaceLanczos w = (WorkspaceLanczos) xmalloc(sizeof(_WorkspaceLanczos));
+    w->iter = iter;
+    w->space_dim = space_dim;
+    w->coef_lspace = (Carray) xmalloc(sizeof(dcomplex) * iter);
+    w->decomp_diag = (Carray) xmalloc(sizeof(dcomplex) * iter);
+    w->decomp_offd = (Carray) xmalloc(sizeof(dcomplex) * iter);
+    w->lanczos_vectors = alloc_cmatrix(iter, space_dim);
+    w->lapack_diag = (Rarray) xmalloc(sizeof(double) * iter);
+    w->lapack_offd = (Rarray) xmalloc(sizeof(double) * iter);
+    w->lapack_eigvec = (Rarray) xmalloc(sizeof(double) * iter * iter);
+    w->transform = (Carray) xmalloc(sizeof(dcomplex) * iter);
+    w->hc = (Carray) xmalloc(sizeof(dcomplex) * space_dim);
+    w->reortho = (Carray) xmalloc(sizeof(dcomplex) * iter);
+    return w;
+}
+
+extern "C" void destroy_lanczos_workspace(WorkspaceLanczos w) {
+    if (!w) return;
+    free(w->coef_lspace);
+    free(w->decomp_diag);
+    free(w->decomp_offd);
+    free_cmatrix(w->iter, w->lanczos_vectors);
+    free(w->lapack_diag);
+    free(w->lapack_offd);
+    free(w->lapack_eigvec);
+    free(w->transform);
+    free(w->hc);
+    free(w->reortho);
+    free(w);
+}
+
+extern "C" CoefWorkspace get_coef_workspace(CoefIntegrator coef_integ_method, uint32_t space_dim, uint16_t lan_it) {
+    CoefWorkspace c = (CoefWorkspace) xmalloc(sizeof(_CoefWorkspace));
+    c->lan_work = nullptr;
+    c->extern_work = nullptr;
+    if (coef_integ_method == LANCZOS) c->lan_work = get_lanczos_workspace(lan_it, space_dim);
+    return c;
+}
+
+extern "C" void destroy_coef_workspace(CoefWorkspace c) {
+    if (!c) return;
+    destroy_lanczos_workspace(c->lan_work);
+    free(c);
+}
+
+// ---- host <-> device staging for the reference-named operators
+
+static void stage_alloc(double2** p, size_t n) {
+    if (!*p && hipMalloc((void**) p, sizeof(double2) * n) != hipSuccess) {
+        set_error("device allocation failed");
+        fail_stop("staging buffers");
+    }
+}
+
+static void h2d(void* d, const void* h, size_t bytes) {
+    if (hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("host to device copy failed");
+        fail_stop("staging");
+    }
+}
+
+static void d2h(void* h, const void* d, size_t bytes) {
+    if (hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("device to host copy failed");
+        fail_stop("staging");
+    }
+}
+
+static void stage_hob_hint(mctdhb_space* sp, Cmatrix hob, Carray hint) {
+    const int M = sp->S->norb;
+    stage_alloc(&sp->d_hob, (size_t) M * M);
+    stage_alloc(&sp->d_hint, (size_t) M * M * M * M);
+    std::vector<dcomplex> packed((size_t) M * M);
+    for (int k = 0; k < M; k++) memcpy(&packed[(size_t) k * M], hob[k], sizeof(dcomplex) * M);
+    h2d(sp->d_hob, packed.data(), sizeof(dcomplex) * M * M);
+    h2d(sp->d_hint, hint, sizeof(dcomplex) * M * M * M * M);
+}
+
+extern "C" void apply_hamiltonian(MultiConfiguration mc, Carray coef, Cmatrix hob, Carray hint, Carray hc) {
+    mctdhb_space* sp = ensure_dev(mc);
+    const size_t n = sp->S->dim;
+    stage_alloc(&sp->d_coef, n);
+    stage_alloc(&sp->d_out, n);
+    stage_hob_hint(sp, hob, hint);
+    h2d(sp->d_coef, coef, sizeof(dcomplex) * n);
+    if (mctdhb_apply_hamiltonian_dev(sp, sp->d_coef, sp->d_hob, sp->d_hint, sp->d_out, MCTDHB_FP64,
+                                     MCTDHB_INDEX_FLY, nullptr))
+        fail_stop("apply_hamiltonian");
+    d2h(hc, sp->d_out, sizeof(dcomplex) * n);
+}
+
+static void density_host(MultiConfiguration mc, Carray coef, bool two) {
+    mctdhb_space* sp = ensure_dev(mc);
+    const size_t n = sp->S->dim;
+    const int M = sp->S->norb;
+    stage_alloc(&sp->d_coef, n);
+    stage_alloc(&sp->d_rho1, (size_t) M * M);
+    if (two) stage_alloc(&sp->d_rho2, (size_t) M * M * M * M);
+    h2d(sp->d_coef, coef, sizeof(dcomplex) * n);
+    if (mctdhb_density_dev(sp, sp->d_coef, sp->d_rho1, two ? sp->d_rho2 : nullptr, MCTDHB_FP64,
+                           sp->S->tables ? MCTDHB_INDEX_TABLES : MCTDHB_INDEX_FLY, nullptr))
+        fail_stop("density matrices");
+}
+
+extern "C" void set_onebody_dm(MultiConfiguration mc, Carray coef, Cmatrix rho) {
+    density_host(mc, coef, false);
+    mctdhb_space* sp = (mctdhb_space*) mc->dev;
+    const int M = sp->S->norb;
+    std::vector<dcomplex> r((size_t) M * M);
+    d2h(r.data(), sp->d_rho1, sizeof(dcomplex) * M * M);
+    for (int k = 0; k < M; k++) memcpy(rho[k], &r[(size_t) k * M], sizeof(dcomplex) * M);
+}
+
+extern "C" void set_twobody_dm(MultiConfiguration mc, Carray coef, Carray rho) {
+    density_host(mc, coef, true);
+    mctdhb_space* sp = (mctdhb_space*) mc->dev;
+    const int M = sp->S->norb;
+    d2h(rho, sp->d_rho2, sizeof(dcomplex) * M * M * M * M);
+}
+
+static mctdhb_sil* host_sil(mctdhb_space* sp, int lm) {
+    if (sp->sil && sp->sil->L->lm != lm) {
+        mctdhb_sil_destroy(sp->sil);
+        sp->sil = nullptr;
+    }
+    if (!sp->sil) {
+        if (lm < MIN_LANCZOS_ITER || lm > MAX_LANCZOS_ITER) {
+            printf("\n\nERROR: Lanczos iterations must be between %d and %d but %d was requested\n\n",
+                   MIN_LANCZOS_ITER, MAX_LANCZOS_ITER, lm);
+            exit(EXIT_FAILURE);
+        }
+        if (mctdhb_sil_create(sp, lm, MCTDHB_FP64, MCTDHB_INDEX_FLY, &sp->sil)) fail_stop("Lanczos workspace");
+    }
+    return sp->sil;
+}
+
+extern "C" int lanczos(MultiConfiguration mc, Cmatrix hob, Carray hint, int lm, Carray diag, Carray offd,
+                       Cmatrix lvec) {
+    mctdhb_space* sp = ensure_dev(mc);
+    const size_t n = sp->S->dim;
+    mctdhb_sil* s = host_sil(sp, lm);
+    stage_alloc(&sp->d_coef, n);
+    stage_hob_hint(sp, hob, hint);
+    if (sp->d_lvec) {
+        hipFree(sp->d_lvec);
+        sp->d_lvec = nullptr;
+    }
+    stage_alloc(&sp->d_lvec, n * lm);
+    h2d(sp->d_coef, lvec[0], sizeof(dcomplex) * n);
+    if (mctdhb_lanczos_dev(s, sp->d_coef, sp->d_hob, sp->d_hint, sp->d_lvec, nullptr)) fail_stop("lanczos");
+    int done = 0;
+    std::vector<double> dg(2 * lm), od(lm);
+    if (mctdhb_sil_scalars(s, &done, dg.data(), od.data(), nullptr)) fail_stop("lanczos");
+    for (int k = 1; k < done; k++) d2h(lvec[k], sp->d_lvec + (size_t) k * n, sizeof(dcomplex) * n);
+    double* dd = (double*) diag;
+    double* oo = (double*) offd;
+    for (int k = 0; k < done; k++) {
+        dd[2 * k] = dg[2 * k];
+        dd[2 * k + 1] = dg[2 * k + 1];
+        if (k < lm - 1 || done < lm) {
+            oo[2 * k] = od[k];
+            oo[2 * k + 1] = 0;
+        }
+    }
+    return done;
+}
+
+extern "C" void propagate_coef_sil(MCTDHBDataStruct mctdhb, Carray cnext) {
+    MultiConfiguration mc = mctdhb->multiconfig_space;
+    mctdhb_space* sp = ensure_dev(mc);
+    const size_t n = sp->S->dim;
+    const int lm = mctdhb->coef_workspace->lan_work->iter;
+    mctdhb_sil* s = host_sil(sp, lm);
+    stage_alloc(&sp->d_coef, n);
+    stage_hob_hint(sp, mctdhb->state->hob, mctdhb->state->hint);
+    h2d(sp->d_coef, mctdhb->state->coef, sizeof(dcomplex) * n);
+    const double* dt = (const double*) &mctdhb->orb_eq->prop_dt;
+    // the renormalisation in imaginary time is applied below, to state->coef only
+    if (mctdhb_sil_step_dev(s, sp->d_coef, sp->d_hob, sp->d_hint, dt[0], dt[1], 0, nullptr))
+        fail_stop("propagate_coef_sil");
+    int done = 0;
+    if (mctdhb_sil_scalars(s, &done, nullptr, nullptr, nullptr)) {
+        printf("\n\nERROR IN DIAGONALIZATION\n\n");
+        exit(EXIT_FAILURE);
+    }
+    if (done < lm)
+        printf("\n\nWARNING : lanczos iterations exit before expected - %d of %d iterations\n\n", done, lm);
+    d2h(cnext, sp->d_coef, sizeof(dcomplex) * n);
+    memcpy(mctdhb->state->coef, cnext, sizeof(dcomplex) * n);
+    if (mctdhb->integ_type == IMAGTIME) {
+        double* c = (double*) mctdhb->state->coef;
+        double s2 = 0;
+        for (size_t i = 0; i < n; i++) s2 += c[2 * i] * c[2 * i] + c[2 * i + 1] * c[2 * i + 1];
+        const double r = 1.0 / sqrt(s2);
+        for (size_t i = 0; i < 2 * n; i++) c[i] *= r;
+    }
+}
+
+extern "C" dcomplex total_energy(ManyBodyState psi) {
+    const int M = psi->norb;
+    const double* h = nullptr;
+    double wr = 0, wi = 0, zr = 0, zi = 0;
+    const double* r2 = (const double*) psi->tb_denmat;
+    const double* g = (const double*) psi->hint;
+    for (int k = 0; k < M; k++)
+        for (int l = 0; l < M; l++) {
+            const double* r1 = (const double*) &psi->ob_denmat[k][l];
+            h = (const double*) &psi->hob[k][l];
+            wr += r1[0] * h[0] - r1[1] * h[1];
+            wi += r1[0] * h[1] + r1[1] * h[0];
+            for (int s = 0; s < M; s++)
+                for (int q = 0; q < M; q++) {
+                    const size_t i = k + l * M + s * M * M + (size_t) q * M * M * M;
+                    const size_t j = k + l * M + q * M * M + (size_t) s * M * M * M;
+                    zr += r2[2 * i] * g[2 * j] - r2[2 * i + 1] * g[2 * j + 1];
+                    zi += r2[2 * i] * g[2 * j + 1] + r2[2 * i + 1] * g[2 * j];
+                }
+        }
+    double out[2] = {wr + zr / 2, wi + zi / 2};
+    dcomplex e;
+    memcpy(&e, out, sizeof(e));
+    return e;
+}

@@ -1,0 +1,136 @@
+// common.hpp - device helpers shared by every kernel of libmctdhb_hip.
+//
+// Complex numbers are interleaved (re, im) pairs, double2 for FP64 and
+// float2 for the mixed-precision forms.  All kernels are written for gfx950
+// (wave64, 160 KB LDS per CU, 8 XCDs).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+namespace mctdhb {
+
+// ------------------------------------------------------------ error model
+// Every C-ABI entry point returns an int status (0 = ok) and leaves a
+// message in a thread-local buffer (mctdhb_last_error()).
+enum Status : int {
+    kOk = 0,
+    kInvalidArgument = 1,
+    kOverflow = 2,
+    kDeviceError = 3,
+    kOutOfMemory = 4,
+    kNotBuilt = 5,
+    kNumerical = 6,
+};
+
+void set_error(const std::string& msg);
+const char* last_error();
+
+#define MCTDHB_HIP_CHECK(expr)                                                            \
+    do {                                                                                  \
+        hipError_t _e = (expr);                                                           \
+        if (_e != hipSuccess) {                                                           \
+            ::mctdhb::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));       \
+            return ::mctdhb::kDeviceError;                                                \
+        }                                                                                 \
+    } while (0)
+
+// ------------------------------------------------------------ complex math
+template <typename R> struct cplx_t;
+template <> struct cplx_t<double> { using type = double2; };
+template <> struct cplx_t<float> { using type = float2; };
+template <typename R> using cplx = typename cplx_t<R>::type;
+
+template <typename R> __host__ __device__ __forceinline__ cplx<R> mk(R re, R im) {
+    cplx<R> z;
+    z.x = re;
+    z.y = im;
+    return z;
+}
+__host__ __device__ __forceinline__ double2 operator+(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__host__ __device__ __forceinline__ double2 operator-(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__host__ __device__ __forceinline__ float2 operator+(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ __forceinline__ float2 operator-(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+template <typename Z> __host__ __device__ __forceinline__ Z cmul(Z a, Z b) {
+    Z r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+// a * conj(b)... conj(a) * b, the inner-product kernel
+template <typename Z> __host__ __device__ __forceinline__ Z cconjmul(Z a, Z b) {
+    Z r;
+    r.x = a.x * b.x + a.y * b.y;
+    r.y = a.x * b.y - a.y * b.x;
+    return r;
+}
+// acc + a * b (complex)
+template <typename Z> __host__ __device__ __forceinline__ Z cfma(Z a, Z b, Z acc) {
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.x = fma(-a.y, b.y, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.y = fma(a.y, b.x, acc.y);
+    return acc;
+}
+template <typename Z, typename R> __host__ __device__ __forceinline__ Z cscale(Z a, R s) {
+    a.x *= s;
+    a.y *= s;
+    return a;
+}
+
+__device__ __forceinline__ double2 to_d2(double2 a) { return a; }
+__device__ __forceinline__ double2 to_d2(float2 a) { return make_double2((double) a.x, (double) a.y); }
+template <typename Z> __device__ __forceinline__ Z from_d2(double2 a);
+template <> __device__ __forceinline__ double2 from_d2<double2>(double2 a) { return a; }
+template <> __device__ __forceinline__ float2 from_d2<float2>(double2 a) { return make_float2((float) a.x, (float) a.y); }
+
+// ----------------------------------------------------------- wave reduce
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block-wide sum of NV doubles per thread into lane-0-of-block results.
+// `scratch` must hold NV * (blockDim.x / 64) doubles.  Returns the totals in
+// thread 0 (other threads get garbage).  Deterministic for a fixed launch.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* scratch) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+    for (int t = 0; t < NV; t++) v[t] = wave_sum(v[t]);
+    if (lane == 0) {
+#pragma unroll
+        for (int t = 0; t < NV; t++) scratch[t * nw + wid] = v[t];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int t = 0; t < NV; t++) {
+            double s = 0;
+            for (int w = 0; w < nw; w++) s += scratch[t * nw + w];
+            v[t] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------ XCD-aware blocks
+// The hardware deals consecutive workgroups round-robin over the 8 XCDs.
+// Remapping lets each XCD own a contiguous range of configurations, so the
+// coefficient windows its gathers touch stay in that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nblocks) {
+    constexpr uint32_t kXcd = 8;
+    const uint32_t per = nblocks / kXcd, rem = nblocks % kXcd;
+    const uint32_t x = bid % kXcd, r = bid / kXcd;
+    // XCD x owns per (+1 for the first rem XCDs) consecutive blocks
+    const uint32_t start = x * per + (x < rem ? x : rem);
+    return start + r;
+}
+
+inline unsigned div_up(uint64_t a, uint64_t b) { return (unsigned) ((a + b - 1) / b); }
+
+}  // namespace mctdhb

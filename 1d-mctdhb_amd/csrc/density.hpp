@@ -1,0 +1,26 @@
+// density.hpp - rho1 / rho2 block reductions over configurations.
+#pragma once
+
+#include "space.hpp"
+
+namespace mctdhb {
+
+enum : int { kDmFP64 = 0, kDmMixed = 1 };
+
+// Scratch needed by density_matrices (per-block partials and totals).
+size_t dm_workspace_bytes(const Space* S);
+int dm_num_elements(int M);
+
+// rho1 (M x M row-major, rho1[k*M+l] = <a+_k a_l>) and, if rho2 is non-null,
+// rho2 (M^4, index k + sM + qM^2 + lM^3 = <a+_k a+_s a_q a_l>), complex128.
+// coef is complex128 (prec = kDmFP64) or complex64 (kDmMixed); index is
+// kIdxTables or kIdxFly.  work must hold dm_workspace_bytes(S).
+// Per-orbital-count fused kernel (dm_m<M>.hip): writes per-block partials.
+template <int M> struct DmFused {
+    static int run(const Space* S, const void* c, int prec, int index, double2* part, unsigned nb, hipStream_t st);
+};
+
+int density_matrices(const Space* S, const void* coef, double2* rho1, double2* rho2, int prec, int index,
+                     void* work, hipStream_t st);
+
+}  // namespace mctdhb

@@ -1,0 +1,367 @@
+// hamiltonian.hip - many-body Hamiltonian action H.C on the device.
+//
+// Replaces reference src/configurational/hamiltonian.c:6-406
+// (apply_hamiltonian).  One configuration per lane; the occupation row is
+// read from the hash table, the coefficient block (M^2 one-body and M^4
+// two-body derived factors) is read with uniform addresses, and the jump
+// targets come either from the reference tables or from the closed-form rank
+// deltas of fock.hpp (W table staged in LDS).  See hamiltonian.hpp for the
+// rule grouping.
+#include "hamiltonian.hpp"
+#include "rowctx.hpp"
+
+namespace mctdhb {
+
+constexpr int kHcBlockG = 256;
+
+unsigned hc_grid_blocks(uint32_t dim) { return div_up(dim, kHcBlockG); }
+
+// ------------------------------------------------------- coefficient block
+
+__global__ void k_hc_coefficients(int M, const double2* __restrict__ hob, const double2* __restrict__ hint,
+                                  double2* __restrict__ cf64, float2* __restrict__ cf32) {
+    const int t = threadIdx.x;
+    const int M2 = M * M, M3 = M2 * M;
+    auto G = [&](int a, int b, int c, int d) { return hint[a + b * M + c * M2 + d * M3]; };
+    auto put = [&](int idx, double2 v) {
+        cf64[idx] = v;
+        if (cf32) cf32[idx] = make_float2((float) v.x, (float) v.y);
+    };
+    const int total = cf_size(M);
+    for (int idx = t; idx < total; idx += blockDim.x) {
+        double2 v = make_double2(0, 0);
+        if (idx < cf_off_B(M)) {  // A_kl = h_kl - g_kkkl
+            int k = idx / M, l = idx % M;
+            if (k != l) v = hob[k * M + l] - G(k, k, k, l);
+        } else if (idx < cf_off_d1(M)) {  // B_klm
+            int r = idx - cf_off_B(M);
+            int k = r / M2, l = (r / M) % M, m = r % M;
+            if (k != l) {
+                if (m == k) v = G(k, k, k, l);
+                else if (m == l) v = G(k, l, l, l);
+                else v = cscale(G(k, m, m, l), 2.0);
+            }
+        } else if (idx < cf_off_d2(M)) {  // d1_k = h_kk - g_kkkk / 2
+            int k = idx - cf_off_d1(M);
+            v = hob[k * M + k] - cscale(G(k, k, k, k), 0.5);
+        } else if (idx < cf_off_d3(M)) {  // d2_k = g_kkkk / 2
+            int k = idx - cf_off_d2(M);
+            v = cscale(G(k, k, k, k), 0.5);
+        } else if (idx < cf_off_ot(M)) {  // d3_ks = 2 g_ksks (k < s)
+            int r = idx - cf_off_d3(M);
+            int k = r / M, s = r % M;
+            if (k < s) v = cscale(G(k, s, k, s), 2.0);
+        } else if (idx < cf_off_tt(M)) {  // Got[k][q][l]
+            int r = idx - cf_off_ot(M);
+            int k = r / M2, q = (r / M) % M, l = r % M;
+            if (q != k && l != k && q <= l) v = (q == l) ? cscale(G(k, k, q, q), 0.5) : G(k, k, q, l);
+        } else {  // Gtt[k][s][q][l]
+            int r = idx - cf_off_tt(M);
+            int k = r / M3, s = (r / M2) % M, q = (r / M) % M, l = r % M;
+            if (k < s && q <= l && q != k && q != s && l != k && l != s)
+                v = (q == l) ? G(k, s, q, q) : cscale(G(k, s, q, l), 2.0);
+        }
+        put(idx, v);
+    }
+}
+
+int hc_coefficients(const Space* S, const double2* hob, const double2* hint, double2* cf64, float2* cf32,
+                    hipStream_t st) {
+    k_hc_coefficients<<<1, 256, 0, st>>>(S->norb, hob, hint, cf64, cf32);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
+// ------------------------------------------- reference-order generic row
+// Any M (up to 30), tables required, FP64: the reference's 13 rule blocks in
+// its order (hamiltonian.c:44-402).
+__device__ double2 hc_row_generic(uint32_t i, int M, const RowCtx& ctx, const double2* __restrict__ c,
+                                  const double2* __restrict__ hob, const double2* __restrict__ hint) {
+    const uint32_t dim = ctx.dim, M2 = M * M, M3 = M2 * M;
+    uint16_t occ[kMaxOrb];
+    for (int k = 0; k < M; k++) occ[k] = ctx.ht[(size_t) i * M + k];
+    auto G = [&](int a, int b, int cc, int d) { return hint[a + b * M + cc * M2 + d * M3]; };
+    auto otc = [&](int k) {
+        uint32_t cnt = 0;
+        for (int h = 0; h < k; h++) cnt += occ[h] > 1;
+        return cnt;
+    };
+    auto ttc = [&](int k, int s) {
+        uint32_t cnt = 0;
+        for (int h = 0; h < k; h++)
+            for (int g = h + 1; g < M; g++) cnt += (occ[h] > 0 && occ[g] > 0);
+        for (int g = k + 1; g < s; g++) cnt += occ[g] > 0;
+        return cnt;
+    };
+    const uint32_t sot = ctx.sot ? ctx.sot[i] : 0, stt = ctx.stt ? ctx.stt[i] : 0;
+    const double2 ci = c[i];
+    double2 w = make_double2(0, 0), z = make_double2(0, 0);
+    double bf;
+    uint32_t j;
+    for (int k = 0; k < M; k++) {
+        if (occ[k] < 1) continue;
+        w = w + cmul(cscale(hob[k * M + k], (double) occ[k]), ci);
+        for (int l = 0; l < M; l++) {
+            if (l == k) continue;
+            bf = sqrt((double) occ[k] * (occ[l] + 1));
+            j = ctx.map[i + k * dim + l * M * dim];
+            w = w + cmul(cscale(hob[k * M + l], bf), c[j]);
+        }
+    }
+    for (int k = 0; k < M; k++) {  // Rule 1
+        bf = occ[k] * (occ[k] - 1);
+        z = z + cscale(cmul(G(k, k, k, k), ci), bf);
+    }
+    for (int k = 0; k < M; k++) {  // Rule 2
+        if (occ[k] < 1) continue;
+        for (int s = k + 1; s < M; s++) {
+            bf = occ[k] * occ[s];
+            z = z + cmul(cscale(cscale(G(k, s, k, s), 4.0), bf), ci);
+        }
+    }
+    for (int k = 0; k < M; k++) {  // Rule 3
+        if (occ[k] < 2) continue;
+        for (int q = 0; q < M; q++) {
+            if (q == k) continue;
+            bf = sqrt((double) (occ[k] - 1) * occ[k] * (occ[q] + 1) * (occ[q] + 2));
+            j = ctx.mapot[sot + otc(k) * M2 + q + q * M];
+            z = z + cscale(cmul(G(k, k, q, q), c[j]), bf);
+        }
+    }
+    for (int k = 0; k < M; k++) {  // Rule 4
+        if (occ[k] < 2) continue;
+        for (int l = 0; l < M; l++) {
+            if (l == k) continue;
+            bf = (occ[k] - 1) * sqrt((double) occ[k] * (occ[l] + 1));
+            j = ctx.map[i + k * dim + l * M * dim];
+            z = z + cscale(cmul(cscale(G(k, k, k, l), 2.0), c[j]), bf);
+        }
+    }
+    for (int k = 0; k < M; k++) {  // Rule 5
+        if (occ[k] < 1) continue;
+        for (int s = 0; s < M; s++) {
+            if (s == k || occ[s] < 1) continue;
+            bf = occ[s] * sqrt((double) occ[k] * (occ[s] + 1));
+            j = ctx.map[i + k * dim + s * M * dim];
+            z = z + cscale(cmul(cscale(G(k, s, s, s), 2.0), c[j]), bf);
+        }
+    }
+    // Rule 6 (6.0 k<q<l, 6.1 q<k<l, 6.2 q<l<k)
+    for (int pass = 0; pass < 3; pass++)
+        for (int a = 0; a < M; a++)
+            for (int b = a + 1; b < M; b++)
+                for (int d = b + 1; d < M; d++) {
+                    int k, q, l;
+                    if (pass == 0) { k = a; q = b; l = d; }
+                    else if (pass == 1) { q = a; k = b; l = d; }
+                    else { q = a; l = b; k = d; }
+                    if (occ[k] < 2) continue;
+                    bf = sqrt((double) occ[k] * (occ[k] - 1) * (occ[q] + 1) * (occ[l] + 1));
+                    j = ctx.mapot[sot + otc(k) * M2 + q + l * M];
+                    z = z + cscale(cmul(cscale(G(k, k, q, l), 2.0), c[j]), bf);
+                }
+    // Rule 7 (7.0 q<k<s, 7.1 k<q<s, 7.2 k<s<q)
+    for (int pass = 0; pass < 3; pass++)
+        for (int a = 0; a < M; a++)
+            for (int b = a + 1; b < M; b++)
+                for (int d = b + 1; d < M; d++) {
+                    int k, s, q;
+                    if (pass == 0) { q = a; k = b; s = d; }
+                    else if (pass == 1) { k = a; q = b; s = d; }
+                    else { k = a; s = b; q = d; }
+                    if (occ[k] < 1 || occ[s] < 1) continue;
+                    bf = sqrt((double) occ[k] * occ[s] * (occ[q] + 1) * (occ[q] + 2));
+                    j = ctx.maptt[stt + ttc(k, s) * M2 + q + q * M];
+                    z = z + cscale(cmul(cscale(G(k, s, q, q), 2.0), c[j]), bf);
+                }
+    for (int s = 0; s < M; s++) {  // Rule 8
+        if (occ[s] < 1) continue;
+        for (int k = 0; k < M; k++) {
+            if (occ[k] < 1 || k == s) continue;
+            for (int l = 0; l < M; l++) {
+                if (l == k || l == s) continue;
+                bf = occ[s] * sqrt((double) occ[k] * (occ[l] + 1));
+                j = ctx.map[i + k * dim + l * M * dim];
+                z = z + cscale(cmul(cscale(G(k, s, s, l), 4.0), c[j]), bf);
+            }
+        }
+    }
+    for (int k = 0; k < M; k++) {  // Rule 9
+        if (occ[k] < 1) continue;
+        for (int s = k + 1; s < M; s++) {
+            if (occ[s] < 1) continue;
+            for (int q = 0; q < M; q++) {
+                if (q == s || q == k) continue;
+                for (int l = q + 1; l < M; l++) {
+                    if (l == k || l == s) continue;
+                    bf = sqrt((double) occ[k] * occ[s] * (occ[q] + 1) * (occ[l] + 1));
+                    j = ctx.maptt[stt + ttc(k, s) * M2 + q + l * M];
+                    z = z + cscale(cmul(cscale(G(k, s, q, l), 4.0), c[j]), bf);
+                }
+            }
+        }
+    }
+    return w + cscale(z, 0.5);
+}
+
+template <typename Zout> struct EpiPlainG {
+    Zout* out;
+    __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 v, double* lds) const {
+        if (valid) out[i] = v;
+    }
+};
+
+struct EpiLanczosG {
+    LanczosHcArgs a;
+    __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 row, double* lds) const {
+        const SilState* st = a.st;
+        const int it = a.it;
+        double2 o = make_double2(0, 0);
+        if (valid) {
+            o = cscale(row, st->s[it]);
+            if (it > 0) {
+                const double bp = st->beta[it - 1] * st->s[it - 1];
+                o = o - cscale(to_d2(((const double2*) a.w[it - 1])[i]), bp);
+            }
+            a.out[i] = o;
+        }
+        const int nslot = lanczos_hc_slots(it);
+        double* part = a.part + (size_t) blockIdx.x * nslot;
+        for (int k = 0; k <= it; k++) {
+            double v[2] = {0, 0};
+            if (valid) {
+                const double2 wk = to_d2(((const double2*) a.w[k])[i]);
+                const double2 p = cconjmul(wk, o);
+                v[0] = p.x * st->s[k];
+                v[1] = p.y * st->s[k];
+            }
+            block_sum<2>(v, lds);
+            if (threadIdx.x == 0) {
+                part[2 * k] = v[0];
+                part[2 * k + 1] = v[1];
+            }
+        }
+        double v1[1] = {0};
+        if (valid && it == 0) {
+            const double2 w0 = to_d2(((const double2*) a.w[0])[i]);
+            v1[0] = w0.x * w0.x + w0.y * w0.y;
+        }
+        block_sum<1>(v1, lds);
+        if (threadIdx.x == 0) {
+            part[2 * (it + 1)] = v1[0];
+            part[2 * (it + 1) + 1] = 0;
+        }
+    }
+};
+
+template <typename Epi>
+__global__ __launch_bounds__(kHcBlockG) void k_hc_generic(int M, RowCtx ctx, const double2* __restrict__ c,
+                                                         const double2* __restrict__ hob,
+                                                         const double2* __restrict__ hint, int stop_check,
+                                                         const SilState* st, Epi epi) {
+    __shared__ double red[kHcBlockG / 64 * 2];
+    if (stop_check && st->stop) return;
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t i = b * kHcBlockG + threadIdx.x;
+    const bool valid = i < ctx.dim;
+    double2 v = make_double2(0, 0);
+    if (valid) v = hc_row_generic(i, M, ctx, c, hob, hint);
+    epi(i, valid, v, red);
+}
+
+static RowCtx row_ctx_g(const Space* S) {
+    return make_row_ctx(S->dim, S->npar, S->d_ht, S->d_map, S->d_mapot, S->d_maptt, S->d_strideot, S->d_stridett);
+}
+
+static int dispatch_fused_plain(const Space* S, const void* c, const void* cf, void* out, int prec, int index,
+                                hipStream_t st) {
+    switch (S->norb) {
+        case 2: return HcFused<2>::plain(S, c, cf, out, prec, index, st);
+        case 3: return HcFused<3>::plain(S, c, cf, out, prec, index, st);
+        case 4: return HcFused<4>::plain(S, c, cf, out, prec, index, st);
+        case 5: return HcFused<5>::plain(S, c, cf, out, prec, index, st);
+        case 6: return HcFused<6>::plain(S, c, cf, out, prec, index, st);
+        case 7: return HcFused<7>::plain(S, c, cf, out, prec, index, st);
+        default: set_error("fused H.C kernels cover 2 <= norb <= 7"); return kInvalidArgument;
+    }
+}
+
+static int dispatch_fused_lanczos(const Space* S, const void* cf, const LanczosHcArgs& a, int prec, int index,
+                                  hipStream_t st) {
+    switch (S->norb) {
+        case 2: return HcFused<2>::lanczos(S, cf, a, prec, index, st);
+        case 3: return HcFused<3>::lanczos(S, cf, a, prec, index, st);
+        case 4: return HcFused<4>::lanczos(S, cf, a, prec, index, st);
+        case 5: return HcFused<5>::lanczos(S, cf, a, prec, index, st);
+        case 6: return HcFused<6>::lanczos(S, cf, a, prec, index, st);
+        case 7: return HcFused<7>::lanczos(S, cf, a, prec, index, st);
+        default: set_error("fused H.C kernels cover 2 <= norb <= 7"); return kInvalidArgument;
+    }
+}
+
+static bool use_generic(const Space* S) { return S->norb == 1 || S->norb > kFusedMaxOrb; }
+
+static int check_mode(const Space* S, int prec, int index) {
+    if (prec != kPrecFP64 && prec != kPrecMixed) {
+        set_error("unknown precision mode");
+        return kInvalidArgument;
+    }
+    if (index != kIndexTables && index != kIndexOnTheFly) {
+        set_error("unknown index mode");
+        return kInvalidArgument;
+    }
+    if ((index == kIndexTables || use_generic(S)) && !S->tables) {
+        set_error("this H.C form needs the jump tables; build the space with tables");
+        return kNotBuilt;
+    }
+    if (use_generic(S) && prec != kPrecFP64) {
+        set_error("mixed precision H.C covers 2 <= norb <= 8");
+        return kInvalidArgument;
+    }
+    return kOk;
+}
+
+int hc_apply(const Space* S, const void* coef, const double2* hob, const double2* hint, void* hc, int prec,
+             int index, hipStream_t st) {
+    int rc = check_mode(S, prec, index);
+    if (rc) return rc;
+    if (use_generic(S)) {
+        EpiPlainG<double2> epi{(double2*) hc};
+        k_hc_generic<<<hc_grid_blocks(S->dim), kHcBlockG, 0, st>>>(S->norb, row_ctx_g(S), (const double2*) coef,
+                                                                   hob, hint, 0, nullptr, epi);
+        MCTDHB_HIP_CHECK(hipGetLastError());
+        return kOk;
+    }
+    const int M = S->norb;
+    double2* cf64 = nullptr;
+    float2* cf32 = nullptr;
+    MCTDHB_HIP_CHECK(hipMallocAsync((void**) &cf64, sizeof(double2) * cf_size(M), st));
+    if (prec == kPrecMixed) MCTDHB_HIP_CHECK(hipMallocAsync((void**) &cf32, sizeof(float2) * cf_size(M), st));
+    rc = hc_coefficients(S, hob, hint, cf64, cf32, st);
+    if (!rc) rc = dispatch_fused_plain(S, coef, prec == kPrecFP64 ? (const void*) cf64 : (const void*) cf32, hc,
+                                       prec, index, st);
+    hipFreeAsync(cf64, st);
+    if (cf32) hipFreeAsync(cf32, st);
+    return rc;
+}
+
+int hc_lanczos(const Space* S, const double2* cf64, const float2* cf32, const LanczosHcArgs& a, int prec,
+               int index, hipStream_t st) {
+    int rc = check_mode(S, prec, index);
+    if (rc) return rc;
+    if (use_generic(S)) {
+        // generic path is FP64: hob/hint are carried in cf64 as [hob | hint]
+        const double2* hob = cf64;
+        const double2* hint = cf64 + S->norb * S->norb;
+        EpiLanczosG epi{a};
+        k_hc_generic<<<hc_grid_blocks(S->dim), kHcBlockG, 0, st>>>(S->norb, row_ctx_g(S),
+                                                                   (const double2*) a.w[a.it], hob, hint, 1,
+                                                                   a.st, epi);
+        MCTDHB_HIP_CHECK(hipGetLastError());
+        return kOk;
+    }
+    return dispatch_fused_lanczos(S, prec == kPrecFP64 ? (const void*) cf64 : (const void*) cf32, a, prec, index,
+                                  st);
+}
+
+}  // namespace mctdhb

@@ -1,0 +1,88 @@
+// hamiltonian.hpp - H.C on the device.
+//
+// The reference applies H in 13 rule blocks (hamiltonian.c:44-401), many of
+// which gather the same coefficient: every single-jump target i - e_k + e_l
+// is read by the one-body term, Rule 4, Rule 5 and Rule 8.  Grouping the
+// rules by jump target gives, with g[a,b,c,d] = hint[a + bM + cM^2 + dM^3]:
+//
+//  hc_i = E(n) c_i
+//       + sum_{k: n_k>=1} sum_{l!=k} F_kl(n) sqrt(n_k (n_l+1)) c[i - e_k + e_l]
+//       + sum_{k: n_k>=2} sum_{q<=l; q,l!=k} Got[k,q,l] b_ot c[i - 2e_k + e_q + e_l]
+//       + sum_{k<s: n_k,n_s>=1} sum_{q<=l; q,l!=k,s} Gtt[k,s,q,l] b_tt c[i - e_k - e_s + e_q + e_l]
+//
+//  E(n)    = sum_k (h_kk - g_kkkk/2) n_k + g_kkkk/2 n_k^2 + sum_{k<s} 2 g_ksks n_k n_s   (one-body diag, Rules 1-2)
+//  F_kl(n) = A_kl + sum_m n_m B_klm, A_kl = h_kl - g_kkkl,
+//            B_klk = g_kkkl, B_kll = g_klll, B_kls = 2 g_kssl (s != k,l)      (one-body, Rules 4, 5, 8)
+//  Got     = g_kkqq / 2 (q = l, Rule 3), g_kkql (q < l, Rule 6)
+//  Gtt     = g_ksqq (q = l, Rule 7), 2 g_ksql (q < l, Rule 9)
+//  b_ot    = sqrt(n_k (n_k-1) (n_q+1) (n_l+1 + [q=l])),  b_tt = sqrt(n_k n_s (n_q+1) (n_l+1 + [q=l]))
+//
+// which is the reference's H.C term for term, with at most one gather per
+// distinct target (130 instead of 230 at N=100, M=5).
+#pragma once
+
+#include "sil.hpp"
+#include "space.hpp"
+
+namespace mctdhb {
+
+// Index scheme of the H.C kernels.
+enum HcIndex : int {
+    kIndexTables = 0,  // reference jump tables (map, mapot, maptt)
+    kIndexOnTheFly = 1 // closed-form rank deltas, no table reads
+};
+// Arithmetic form.
+enum HcPrec : int {
+    kPrecFP64 = 0,   // complex128 throughout
+    kPrecMixed = 1   // complex64 gathers and FMAs (vectors stored as complex64)
+};
+
+// Coefficient block layout (complex entries), see the formula above.
+__host__ __device__ constexpr int cf_off_A(int M) { return 0; }
+__host__ __device__ constexpr int cf_off_B(int M) { return M * M; }
+__host__ __device__ constexpr int cf_off_d1(int M) { return M * M + M * M * M; }
+__host__ __device__ constexpr int cf_off_d2(int M) { return cf_off_d1(M) + M; }
+__host__ __device__ constexpr int cf_off_d3(int M) { return cf_off_d2(M) + M; }
+__host__ __device__ constexpr int cf_off_ot(int M) { return cf_off_d3(M) + M * M; }
+__host__ __device__ constexpr int cf_off_tt(int M) { return cf_off_ot(M) + M * M * M; }
+__host__ __device__ constexpr int cf_size(int M) { return cf_off_tt(M) + M * M * M * M; }
+
+constexpr int kFusedMaxOrb = 7;  // fused kernels instantiated for 2 <= M <= 7
+
+// Build the coefficient block from hob (M x M row-major) and hint (M^4),
+// both complex128 device arrays, into cf64 (and cf32 if non-null).
+int hc_coefficients(const Space* S, const double2* hob, const double2* hint, double2* cf64, float2* cf32,
+                    hipStream_t st);
+
+// hc = H coef.  prec / index select the kernel; coef/hc are complex128 for
+// FP64 and complex64 for mixed.  Uses the reference-order generic kernel for
+// M > 8 (tables required) and M == 1.
+int hc_apply(const Space* S, const void* coef, const double2* hob, const double2* hint, void* hc, int prec,
+             int index, hipStream_t st);
+
+// Lanczos form (iteration it): out = s_it H w_it - beta_{it-1} s_{it-1} w_{it-1}
+// (out is complex128) plus per-block partial sums, part[blk * nslot + 2k..2k+1]
+// = sum conj(s_k w_k) out for k <= it and, at it = 0, slot it+1 = |w_0|^2.
+// All scales come from the device SilState; the kernel is a no-op after a
+// Lanczos breakdown (st->stop).
+struct LanczosHcArgs {
+    const void* w[kMaxLanczos];
+    int it;
+    const SilState* st;
+    double2* out;
+    double* part;
+};
+__host__ __device__ inline int lanczos_hc_slots(int it) { return 2 * (it + 1) + 2; }
+int hc_lanczos(const Space* S, const double2* cf64, const float2* cf32, const LanczosHcArgs& a, int prec,
+               int index, hipStream_t st);
+
+unsigned hc_grid_blocks(uint32_t dim);
+
+// Per-orbital-count entry points of the fused kernel (hc_m<M>.hip).  The
+// mixed form always uses on-the-fly indices.
+template <int M> struct HcFused {
+    static int plain(const Space* S, const void* c, const void* cf, void* out, int prec, int index, hipStream_t st);
+    static int lanczos(const Space* S, const void* cf, const LanczosHcArgs& a, int prec, int index, hipStream_t st);
+};
+
+}  // namespace mctdhb

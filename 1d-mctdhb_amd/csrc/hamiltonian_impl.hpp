@@ -1,0 +1,255 @@
+// hamiltonian_impl.hpp - templates of the fused H.C kernel, instantiated
+// once per orbital count in hc_m<M>.hip (split so they compile in parallel).
+#pragma once
+
+#include "hamiltonian.hpp"
+#include "rowctx.hpp"
+
+#include <type_traits>
+
+namespace mctdhb {
+
+constexpr int kHcBlock = 256;
+
+// ----------------------------------------------------------- fused row
+
+template <typename Z> __device__ __forceinline__ double2 load_d2(const Z* p, uint32_t j) { return to_d2(p[j]); }
+
+// Gather c[j] as the compute type.
+template <typename R, typename Zin> __device__ __forceinline__ cplx<R> gather(const Zin* __restrict__ c, uint32_t j);
+template <> __device__ __forceinline__ double2 gather<double, double2>(const double2* __restrict__ c, uint32_t j) {
+    return c[j];
+}
+template <> __device__ __forceinline__ float2 gather<float, float2>(const float2* __restrict__ c, uint32_t j) {
+    return c[j];
+}
+
+template <int M, typename R, int IDX, typename Zin>
+__device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, const Zin* __restrict__ c,
+                                                const cplx<R>* __restrict__ cf, const uint32_t* __restrict__ Wl,
+                                                int ldw, const R* __restrict__ sql) {
+    using Z = cplx<R>;
+    Cfg<M, R, IDX> x;
+    x.load(i, ctx, Wl, ldw, sql);
+    const int* n = x.n;
+
+    // diagonal E(n)
+    Z e = mk<R>(0, 0);
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        const R nk = (R) n[k];
+        const Z d1 = cf[cf_off_d1(M) + k], d2 = cf[cf_off_d2(M) + k];
+        e.x += nk * (d1.x + nk * d2.x);
+        e.y += nk * (d1.y + nk * d2.y);
+#pragma unroll
+        for (int s = k + 1; s < M; s++) {
+            const Z d3 = cf[cf_off_d3(M) + k * M + s];
+            const R ns = nk * (R) n[s];
+            e.x += ns * d3.x;
+            e.y += ns * d3.y;
+        }
+    }
+    Z acc = cmul(e, gather<R, Zin>(c, i));
+
+    // single jumps k -> l
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        if (n[k] >= 1) {
+#pragma unroll
+            for (int l = 0; l < M; l++) {
+                if (l == k) continue;
+                Z F = cf[cf_off_A(M) + k * M + l];
+#pragma unroll
+                for (int m = 0; m < M; m++) {
+                    const Z B = cf[cf_off_B(M) + (k * M + l) * M + m];
+                    F.x += (R) n[m] * B.x;
+                    F.y += (R) n[m] * B.y;
+                }
+                const R b = x.s0[k] * x.s1[l];
+                acc = cfma(cscale(F, b), gather<R, Zin>(c, x.single(k, l)), acc);
+            }
+        }
+    }
+    // two particles out of the same orbital k
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        if (n[k] >= 2) {
+            const R a = x.sm[k] * x.s0[k];
+#pragma unroll
+            for (int q = 0; q < M; q++) {
+                if (q == k) continue;
+#pragma unroll
+                for (int l = q; l < M; l++) {
+                    if (l == k) continue;
+                    const Z G = cf[cf_off_ot(M) + (k * M + q) * M + l];
+                    const R b = a * x.s1[q] * (q == l ? x.s2[q] : x.s1[l]);
+                    acc = cfma(cscale(G, b), gather<R, Zin>(c, x.same(k, q, l)), acc);
+                }
+            }
+        }
+    }
+    // one particle out of each of k < s
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+#pragma unroll
+        for (int s = k + 1; s < M; s++) {
+            if (n[k] >= 1 && n[s] >= 1) {
+                const R a = x.s0[k] * x.s0[s];
+#pragma unroll
+                for (int q = 0; q < M; q++) {
+                    if (q == k || q == s) continue;
+#pragma unroll
+                    for (int l = q; l < M; l++) {
+                        if (l == k || l == s) continue;
+                        const Z G = cf[cf_off_tt(M) + ((k * M + s) * M + q) * M + l];
+                        const R b = a * x.s1[q] * (q == l ? x.s2[q] : x.s1[l]);
+                        acc = cfma(cscale(G, b), gather<R, Zin>(c, x.diff(k, s, q, l)), acc);
+                    }
+                }
+            }
+        }
+    }
+    return acc;
+}
+
+// --------------------------------------------------------------- epilogues
+
+template <typename Zout> struct EpiPlain {
+    Zout* out;
+    __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 v, double* lds) const {
+        if (valid) out[i] = from_d2<Zout>(v);
+    }
+};
+
+template <typename Zw> struct EpiLanczos {
+    LanczosHcArgs a;
+    __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 row, double* lds) const {
+        const SilState* st = a.st;
+        const int it = a.it;
+        double2 o = make_double2(0, 0);
+        if (valid) {
+            o = cscale(row, st->s[it]);
+            if (it > 0) {
+                const double bp = st->beta[it - 1] * st->s[it - 1];
+                o = o - cscale(to_d2(((const Zw*) a.w[it - 1])[i]), bp);
+            }
+            a.out[i] = o;
+        }
+        const int nslot = lanczos_hc_slots(it);
+        double* part = a.part + (size_t) blockIdx.x * nslot;
+        for (int k = 0; k <= it; k++) {
+            double v[2] = {0, 0};
+            if (valid) {
+                const double2 wk = to_d2(((const Zw*) a.w[k])[i]);
+                const double2 p = cconjmul(wk, o);
+                v[0] = p.x * st->s[k];
+                v[1] = p.y * st->s[k];
+            }
+            block_sum<2>(v, lds);
+            if (threadIdx.x == 0) {
+                part[2 * k] = v[0];
+                part[2 * k + 1] = v[1];
+            }
+        }
+        double v1[1] = {0};
+        if (valid && it == 0) {
+            const double2 w0 = to_d2(((const Zw*) a.w[0])[i]);
+            v1[0] = w0.x * w0.x + w0.y * w0.y;
+        }
+        block_sum<1>(v1, lds);
+        if (threadIdx.x == 0) {
+            part[2 * (it + 1)] = v1[0];
+            part[2 * (it + 1) + 1] = 0;
+        }
+    }
+};
+
+// ----------------------------------------------------------------- kernels
+
+template <int M, typename R, int IDX, typename Zin, typename Epi>
+__global__ __launch_bounds__(kHcBlock) void k_hc_fused(RowCtx ctx, const Zin* __restrict__ c,
+                                                       const cplx<R>* __restrict__ cf,
+                                                       const uint32_t* __restrict__ Wg, int ldw, int use_sq,
+                                                       int stop_check, const SilState* st, Epi epi) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    if (stop_check && st->stop) return;
+    double* red = (double*) smem;                         // block_sum scratch (kHcBlock/64 * 2 doubles)
+    uint32_t* Wl = (uint32_t*) (smem + 64);               // W table
+    const int wsz = (M - 1) * ldw;
+    R* sql = (R*) (smem + 64 + ((wsz * 4 + 15) & ~15));   // sqrt LUT
+    if constexpr (IDX == kIdxFly) {
+        for (int t = threadIdx.x; t < wsz; t += blockDim.x) Wl[t] = Wg[t];
+    }
+    if (use_sq) {
+        for (int t = threadIdx.x; t < ctx.npar + 3; t += blockDim.x) sql[t] = sqrt((R) t);
+    }
+    __syncthreads();
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t i = b * kHcBlock + threadIdx.x;
+    const bool valid = i < ctx.dim;
+    double2 v = make_double2(0, 0);
+    if (valid) v = to_d2(hc_row_fused<M, R, IDX, Zin>(i, ctx, c, cf, Wl, ldw, use_sq ? sql : nullptr));
+    epi(i, valid, v, red);
+}
+
+static inline RowCtx row_ctx(const Space* S) {
+    return make_row_ctx(S->dim, S->npar, S->d_ht, S->d_map, S->d_mapot, S->d_maptt, S->d_strideot, S->d_stridett);
+}
+
+static inline bool sq_in_lds(const Space* S) { return S->npar + 3 <= 2048; }
+
+// Layout matches k_hc_fused: 64 B reduction scratch, the W table region
+// (reserved in both index modes so the sqrt LUT offset is the same), sqrt LUT.
+static inline size_t fused_smem(const Space* S, bool fly, bool use_sq, size_t rsize) {
+    (void) fly;
+    size_t wsz = (size_t) (S->norb - 1) * S->ldw * 4;
+    size_t s = 64 + ((wsz + 15) & ~(size_t) 15);
+    if (use_sq) s += rsize * (S->npar + 3);
+    return s;
+}
+
+template <int M, typename R, typename Zin, typename Epi>
+static inline int launch_fused(const Space* S, const Zin* c, const cplx<R>* cf, int index, int stop_check,
+                        const SilState* st, Epi epi, hipStream_t stream) {
+    const bool fly = index == kIndexOnTheFly;
+    const bool use_sq = sq_in_lds(S);
+    const size_t smem = fused_smem(S, fly, use_sq, sizeof(R));
+    if (smem > 64 * 1024) {
+        set_error("on-the-fly index tables exceed the LDS budget for this (N, M); use table indexing");
+        return kInvalidArgument;
+    }
+    const unsigned nb = hc_grid_blocks(S->dim);
+    RowCtx ctx = row_ctx(S);
+    if (fly) {
+        k_hc_fused<M, R, kIdxFly, Zin, Epi><<<nb, kHcBlock, smem, stream>>>(
+            ctx, c, cf, S->d_W, S->ldw, use_sq, stop_check, st, epi);
+    } else {
+        if constexpr (std::is_same<R, double>::value)
+            k_hc_fused<M, R, kIdxTables, Zin, Epi><<<nb, kHcBlock, smem, stream>>>(
+                ctx, c, cf, S->d_W, S->ldw, use_sq, stop_check, st, epi);
+    }
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
+template <int M>
+int HcFused<M>::plain(const Space* S, const void* c, const void* cf, void* out, int prec, int index,
+                      hipStream_t st) {
+    if (prec == kPrecFP64)
+        return launch_fused<M, double, double2>(S, (const double2*) c, (const double2*) cf, index, 0, nullptr,
+                                                EpiPlain<double2>{(double2*) out}, st);
+    return launch_fused<M, float, float2>(S, (const float2*) c, (const float2*) cf, kIndexOnTheFly, 0, nullptr,
+                                          EpiPlain<float2>{(float2*) out}, st);
+}
+
+template <int M>
+int HcFused<M>::lanczos(const Space* S, const void* cf, const LanczosHcArgs& a, int prec, int index,
+                        hipStream_t st) {
+    if (prec == kPrecFP64)
+        return launch_fused<M, double, double2>(S, (const double2*) a.w[a.it], (const double2*) cf, index, 1, a.st,
+                                                EpiLanczos<double2>{a}, st);
+    return launch_fused<M, float, float2>(S, (const float2*) a.w[a.it], (const float2*) cf, kIndexOnTheFly, 1, a.st,
+                                          EpiLanczos<float2>{a}, st);
+}
+
+}  // namespace mctdhb

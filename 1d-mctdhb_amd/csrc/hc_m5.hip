@@ -1,0 +1,6 @@
+// hc_m5.hip - fused H.C kernels instantiated for M = 5 orbitals.
+#include "hamiltonian_impl.hpp"
+
+namespace mctdhb {
+template struct HcFused<5>;
+}  // namespace mctdhb

@@ -1,0 +1,6 @@
+// hc_m7.hip - fused H.C kernels instantiated for M = 7 orbitals.
+#include "hamiltonian_impl.hpp"
+
+namespace mctdhb {
+template struct HcFused<7>;
+}  // namespace mctdhb

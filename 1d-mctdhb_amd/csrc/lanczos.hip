@@ -1,0 +1,448 @@
+// lanczos.hip - the Lanczos tridiagonalisation and the short-iterative-
+// Lanczos (SIL) coefficient step on the device.
+//
+// Replaces reference src/linalg/multiconfig_lanczos.c:11-90 (lanczos) and
+// src/integrator/coefficients_integration.c:59-160 (propagate_coef_sil).
+//
+// Iteration it of the reference computes
+//   out = H v_it - beta_{it-1} v_{it-1};  alpha_it = <v_it, out>
+//   out -= alpha_it v_it;  ortho_k = <v_k, out> (k <= it);  out -= sum ortho_k v_k
+//   beta_it = |out|;  v_{it+1} = out / beta_it
+// Here the H.C kernel writes out and, in the same pass, the partial sums of
+// d_k = <v_k, out> for k <= it.  Because <v_k, out - alpha v_it> = d_k -
+// alpha <v_k, v_it>, the reorthogonalisation coefficients follow from d_k and
+// the Gram column <v_k, v_it> that the previous pass produced, so one more
+// streaming pass both subtracts every projection and accumulates |w|^2 and
+// the next Gram column.  Vectors stay unnormalised (v_k = s_k w_k).  All
+// scalars live in a device SilState; the step is a fixed kernel sequence with
+// no host synchronisation.
+#include "density.hpp"
+#include "hamiltonian.hpp"
+#include "sil_api.hpp"
+
+namespace mctdhb {
+
+constexpr int kStreamBlock = 256;
+constexpr int kStreamGrid = 1024;
+constexpr double kBreakdownTol = 1e-14;  // reference multiconfig_lanczos.c:34
+
+// -------------------------------------------------------------- scalars
+
+__global__ void k_sil_init(SilState* st, int lm) {
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k <= kMaxLanczos; k++) {
+        st->s[k] = 0;
+        st->beta[k] = 0;
+        st->alpha[k] = make_double2(0, 0);
+        st->cre[k] = make_double2(0, 0);
+        st->gram[k] = make_double2(0, 0);
+        st->y[k] = make_double2(0, 0);
+    }
+    st->s[0] = 1.0;
+    st->maxbeta = 0;
+    st->nrm2 = 0;
+    st->iter_done = lm;
+    st->stop = 0;
+    st->eig_fail = 0;
+}
+
+// Sum per-block partials: tot[t] = sum_b part[b * nslot + t], one block per slot.
+__global__ __launch_bounds__(256) void k_reduce_slots(const double* __restrict__ part, int nblocks, int nslot,
+                                                      double* __restrict__ tot, const SilState* st) {
+    __shared__ double red[4];
+    if (st && st->stop) return;
+    const int t = blockIdx.x;
+    double v[1] = {0};
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) v[0] += part[(size_t) b * nslot + t];
+    block_sum<1>(v, red);
+    if (threadIdx.x == 0) tot[t] = v[0];
+}
+
+// After the H.C pass of iteration it: alpha_it and the reorthogonalisation
+// coefficients (on w_k) for the next streaming pass.
+__global__ void k_sil_after_hc(SilState* st, const double* __restrict__ tot, int it) {
+    if (threadIdx.x != 0 || st->stop) return;
+    if (it == 0) st->gram[0] = make_double2(tot[2 * (it + 1)], 0);  // <v_0, v_0> = |C|^2
+    const double2 alpha = make_double2(tot[2 * it], tot[2 * it + 1]);
+    st->alpha[it] = alpha;
+    for (int k = 0; k <= it; k++) {
+        const double2 d = make_double2(tot[2 * k], tot[2 * k + 1]);
+        double2 ortho = d - cmul(alpha, st->gram[k]);
+        if (k == it) ortho = ortho + alpha;
+        st->cre[k] = cscale(ortho, st->s[k]);
+    }
+}
+
+// After the streaming pass that produced w_{it+1}: beta_it, the breakdown
+// test of the reference (beta / max beta < 1e-14 ends the recursion with
+// it + 1 vectors) and the Gram column <v_k, v_{it+1}>.
+__global__ void k_sil_after_reorth(SilState* st, const double* __restrict__ tot, int it) {
+    if (threadIdx.x != 0 || st->stop) return;
+    const double nrm2 = tot[2 * (it + 1)];
+    const double beta = sqrt(nrm2);
+    st->beta[it] = beta;
+    if (st->maxbeta < beta) st->maxbeta = beta;
+    if (beta / st->maxbeta < kBreakdownTol) {
+        st->iter_done = it + 1;
+        st->stop = 1;
+        return;
+    }
+    const double sn = 1.0 / beta;
+    st->s[it + 1] = sn;
+    for (int k = 0; k <= it; k++)
+        st->gram[k] = cscale(make_double2(tot[2 * k], tot[2 * k + 1]), st->s[k] * sn);
+    st->gram[it + 1] = make_double2(nrm2 * sn * sn, 0);
+}
+
+// exp(-i T prop_dt) e_1 from the eigen-decomposition of the real tridiagonal
+// T = (Re alpha, beta), as reference coefficients_integration.c:104-145.
+// Implicit-shift QL with Givens rotations (the EISPACK tql2 method), one
+// thread; n <= 15.
+__device__ int tridiag_eig_dev(int n, double* d, double* e, double* z) {
+    for (int r = 0; r < n; r++)
+        for (int c = 0; c < n; c++) z[r * n + c] = (r == c) ? 1.0 : 0.0;
+    e[n - 1] = 0.0;
+    for (int l = 0; l < n; l++) {
+        int sweeps = 0, m;
+        for (;;) {
+            for (m = l; m < n - 1; m++) {
+                const double scale = fabs(d[m]) + fabs(d[m + 1]);
+                if (fabs(e[m]) <= 2.220446049250313e-16 * scale) break;
+            }
+            if (m == l) break;
+            if (++sweeps > 60) return 1;
+            const double theta = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            const double rho = hypot(theta, 1.0);
+            double g = d[m] - d[l] + e[l] / (theta + (theta >= 0 ? rho : -rho));
+            double sn = 1.0, cs = 1.0, shift = 0.0;
+            bool early = false;
+            for (int i = m - 1; i >= l; i--) {
+                const double f = sn * e[i], b = cs * e[i];
+                double r = hypot(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) {
+                    d[i + 1] -= shift;
+                    e[m] = 0.0;
+                    early = true;
+                    break;
+                }
+                sn = f / r;
+                cs = g / r;
+                g = d[i + 1] - shift;
+                r = (d[i] - g) * sn + 2.0 * cs * b;
+                shift = sn * r;
+                d[i + 1] = g + shift;
+                g = cs * r - b;
+                for (int row = 0; row < n; row++) {
+                    const double zi1 = z[row * n + i + 1], zi = z[row * n + i];
+                    z[row * n + i + 1] = sn * zi + cs * zi1;
+                    z[row * n + i] = cs * zi - sn * zi1;
+                }
+            }
+            if (early) continue;
+            d[l] -= shift;
+            e[l] = g;
+            e[m] = 0.0;
+        }
+    }
+    return 0;
+}
+
+__global__ void k_sil_tridiag(SilState* st, double dt_re, double dt_im) {
+    if (threadIdx.x != 0) return;
+    const int n = st->iter_done;
+    double d[kMaxLanczos + 1], e[kMaxLanczos + 1], z[(kMaxLanczos + 1) * (kMaxLanczos + 1)];
+    for (int k = 0; k < n; k++) {
+        d[k] = st->alpha[k].x;
+        e[k] = st->beta[k];
+    }
+    if (tridiag_eig_dev(n, d, e, z)) {
+        st->eig_fail = 1;
+        return;
+    }
+    // y = Z exp(-i D prop_dt) Z^T e_1; exp(-i d dt) for complex dt = a + ib
+    // is exp(d b) (cos(d a) - i sin(d a)).
+    double2 aux[kMaxLanczos + 1];
+    for (int k = 0; k < n; k++) {
+        const double mag = exp(d[k] * dt_im), ph = -d[k] * dt_re;
+        aux[k] = make_double2(z[0 * n + k] * mag * cos(ph), z[0 * n + k] * mag * sin(ph));
+    }
+    for (int k = 0; k < n; k++) {
+        double2 y = make_double2(0, 0);
+        for (int j = 0; j < n; j++) y = y + cscale(aux[j], z[k * n + j]);
+        st->y[k] = cscale(y, st->s[k]);
+    }
+}
+
+// ---------------------------------------------------------- streaming
+
+template <typename Zw>
+__global__ __launch_bounds__(kStreamBlock) void k_to_complex64(const double2* __restrict__ a, Zw* __restrict__ b,
+                                                               uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        b[i] = from_d2<Zw>(a[i]);
+}
+
+struct WPtrs {
+    const void* w[kMaxLanczos];
+};
+
+// w_{it+1} = out - sum_{k<=it} cre_k w_k, with partials of <w_k, w_{it+1}>
+// (slots 2k, 2k+1) and |w_{it+1}|^2 (slot 2(it+1)).
+template <typename Zw, int NK>
+__global__ __launch_bounds__(kStreamBlock) void k_sil_reorth(const double2* __restrict__ out, WPtrs wp,
+                                                             Zw* __restrict__ wnew, uint32_t n,
+                                                             const SilState* __restrict__ st,
+                                                             double* __restrict__ part, int nslot) {
+    __shared__ double red[2 * NK + 1][kStreamBlock / 64];
+    if (st->stop) return;
+    double2 cre[NK];
+#pragma unroll
+    for (int k = 0; k < NK; k++) cre[k] = st->cre[k];
+    double acc[2 * NK + 1];
+#pragma unroll
+    for (int t = 0; t < 2 * NK + 1; t++) acc[t] = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double2 w = out[i];
+        double2 wk[NK];
+#pragma unroll
+        for (int k = 0; k < NK; k++) {
+            wk[k] = to_d2(((const Zw*) wp.w[k])[i]);
+            w = w - cmul(cre[k], wk[k]);
+        }
+        const Zw ws = from_d2<Zw>(w);
+        wnew[i] = ws;
+        const double2 wr = to_d2(ws);  // the stored vector defines v_{it+1}
+#pragma unroll
+        for (int k = 0; k < NK; k++) {
+            const double2 p = cconjmul(wk[k], wr);
+            acc[2 * k] += p.x;
+            acc[2 * k + 1] += p.y;
+        }
+        acc[2 * NK] += wr.x * wr.x + wr.y * wr.y;
+    }
+    block_sum<2 * NK + 1>(acc, &red[0][0]);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int t = 0; t < 2 * NK + 1; t++) part[(size_t) blockIdx.x * nslot + t] = acc[t];
+    }
+}
+
+// C' = sum_j y_j w_j (w_0 is C itself in FP64), in place, plus |C'|^2 partials.
+template <typename Zw>
+__global__ __launch_bounds__(kStreamBlock) void k_sil_recombine(double2* __restrict__ C, WPtrs wp, uint32_t n,
+                                                                const SilState* __restrict__ st,
+                                                                double* __restrict__ part) {
+    __shared__ double red[kStreamBlock / 64];
+    const int nd = st->iter_done;
+    double2 y[kMaxLanczos];
+    for (int k = 0; k < nd; k++) y[k] = st->y[k];
+    double acc[1] = {0};
+    if (!st->eig_fail) {
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+            double2 v = cmul(y[0], C[i]);
+            for (int k = 1; k < nd; k++) v = v + cmul(y[k], to_d2(((const Zw*) wp.w[k])[i]));
+            C[i] = v;
+            acc[0] += v.x * v.x + v.y * v.y;
+        }
+    }
+    block_sum<1>(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+__global__ __launch_bounds__(kStreamBlock) void k_scale_by_inv_norm(double2* __restrict__ C, uint32_t n,
+                                                                    const double* __restrict__ nrm2) {
+    const double s = 1.0 / sqrt(nrm2[0]);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        C[i] = cscale(C[i], s);
+}
+
+// Normalised Lanczos vectors for the reference-shaped lanczos() entry point.
+template <typename Zw>
+__global__ __launch_bounds__(kStreamBlock) void k_export_vector(const Zw* __restrict__ w, double2* __restrict__ v,
+                                                                uint32_t n, const SilState* __restrict__ st, int k) {
+    const double s = st->s[k];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        v[i] = cscale(to_d2(w[i]), s);
+}
+
+// ------------------------------------------------------------ the driver
+
+int sil_create(const Space* S, int lm, int prec, int index, Sil** out) {
+    *out = nullptr;
+    if (lm < 2 || lm > kMaxLanczos) {
+        set_error("Lanczos iterations must be between 2 and 15");
+        return kInvalidArgument;
+    }
+    if (prec != kPrecFP64 && prec != kPrecMixed) {
+        set_error("unknown precision mode");
+        return kInvalidArgument;
+    }
+    const bool generic = S->norb == 1 || S->norb > kFusedMaxOrb;
+    if (generic && prec != kPrecFP64) {
+        set_error("mixed precision SIL covers 2 <= norb <= 7");
+        return kInvalidArgument;
+    }
+    if ((generic || (index == kIndexTables && prec == kPrecFP64)) && !S->tables) {
+        set_error("this SIL form needs the jump tables; build the space with tables");
+        return kNotBuilt;
+    }
+    Sil* L = new Sil();
+    L->S = S;
+    L->lm = lm;
+    L->prec = prec;
+    L->index = prec == kPrecMixed ? kIndexOnTheFly : index;
+    L->generic = generic;
+    const size_t n = S->dim;
+    const size_t zw = prec == kPrecFP64 ? sizeof(double2) : sizeof(float2);
+    const int M = S->norb;
+    const size_t cfn = generic ? (size_t) M * M + (size_t) M * M * M * M : (size_t) cf_size(M);
+    const unsigned nbh = hc_grid_blocks(S->dim);
+    L->part_cap = std::max<size_t>((size_t) nbh * lanczos_hc_slots(kMaxLanczos),
+                                   (size_t) kStreamGrid * (2 * kMaxLanczos + 2));
+    bool ok = hipMalloc((void**) &L->d_st, sizeof(SilState)) == hipSuccess;
+    ok = ok && hipMalloc((void**) &L->cf64, sizeof(double2) * cfn) == hipSuccess;
+    if (prec == kPrecMixed) ok = ok && hipMalloc((void**) &L->cf32, sizeof(float2) * cfn) == hipSuccess;
+    ok = ok && hipMalloc((void**) &L->out, sizeof(double2) * n) == hipSuccess;
+    ok = ok && hipMalloc((void**) &L->part, sizeof(double) * L->part_cap) == hipSuccess;
+    ok = ok && hipMalloc((void**) &L->tot, sizeof(double) * 64) == hipSuccess;
+    for (int k = 1; k < lm && ok; k++) ok = hipMalloc(&L->w[k], zw * n) == hipSuccess;
+    if (prec == kPrecMixed) ok = ok && hipMalloc(&L->w0f, sizeof(float2) * n) == hipSuccess;
+    if (!ok) {
+        sil_destroy(L);
+        set_error("device allocation failed for the Lanczos workspace");
+        return kOutOfMemory;
+    }
+    *out = L;
+    return kOk;
+}
+
+void sil_destroy(Sil* L) {
+    if (!L) return;
+    hipFree(L->d_st);
+    hipFree(L->cf64);
+    hipFree(L->cf32);
+    hipFree(L->out);
+    hipFree(L->part);
+    hipFree(L->tot);
+    for (int k = 1; k < kMaxLanczos; k++) hipFree(L->w[k]);
+    hipFree(L->w0f);
+    delete L;
+}
+
+size_t sil_device_bytes(const Space* S, int lm, int prec) {
+    const size_t n = S->dim;
+    const size_t zw = prec == kPrecFP64 ? sizeof(double2) : sizeof(float2);
+    size_t b = (size_t) (lm - 1) * zw * n + sizeof(double2) * n;
+    if (prec == kPrecMixed) b += sizeof(float2) * n;
+    return b;
+}
+
+template <typename Zw, int NK>
+static void launch_reorth(Sil* L, int it, hipStream_t st) {
+    WPtrs wp{};
+    for (int k = 0; k <= it; k++) wp.w[k] = L->w[k];
+    const int nslot = 2 * NK + 1;
+    k_sil_reorth<Zw, NK><<<kStreamGrid, kStreamBlock, 0, st>>>(L->out, wp, (Zw*) L->w[it + 1], L->S->dim, L->d_st,
+                                                               L->part, nslot);
+    k_reduce_slots<<<nslot, 256, 0, st>>>(L->part, kStreamGrid, nslot, L->tot, L->d_st);
+}
+
+template <typename Zw>
+static void reorth_dispatch(Sil* L, int it, hipStream_t st) {
+    switch (it + 1) {
+#define CASE(K) \
+    case K: launch_reorth<Zw, K>(L, it, st); break;
+        CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12)
+        CASE(13) CASE(14)
+#undef CASE
+        default: break;
+    }
+}
+
+// Enqueue the Lanczos recursion (lm H.C applications).  C is w_0 (FP64).
+static int sil_run_lanczos(Sil* L, const double2* C, const double2* hob, const double2* hint, hipStream_t st) {
+    const Space* S = L->S;
+    const uint32_t n = S->dim;
+    int rc;
+    k_sil_init<<<1, 64, 0, st>>>(L->d_st, L->lm);
+    if (L->generic) {
+        const int M = S->norb;
+        MCTDHB_HIP_CHECK(hipMemcpyAsync(L->cf64, hob, sizeof(double2) * M * M, hipMemcpyDeviceToDevice, st));
+        MCTDHB_HIP_CHECK(hipMemcpyAsync(L->cf64 + M * M, hint, sizeof(double2) * M * M * M * M,
+                                        hipMemcpyDeviceToDevice, st));
+    } else {
+        rc = hc_coefficients(S, hob, hint, L->cf64, L->cf32, st);
+        if (rc) return rc;
+    }
+    if (L->prec == kPrecMixed) {
+        k_to_complex64<float2><<<kStreamGrid, kStreamBlock, 0, st>>>(C, (float2*) L->w0f, n);
+        L->w[0] = L->w0f;
+    } else {
+        L->w[0] = (void*) C;
+    }
+    const unsigned nbh = hc_grid_blocks(n);
+    for (int it = 0; it < L->lm; it++) {
+        LanczosHcArgs a{};
+        for (int k = 0; k <= it; k++) a.w[k] = L->w[k];
+        a.it = it;
+        a.st = L->d_st;
+        a.out = L->out;
+        a.part = L->part;
+        rc = hc_lanczos(S, L->cf64, L->cf32, a, L->prec, L->index, st);
+        if (rc) return rc;
+        const int nslot = lanczos_hc_slots(it);
+        k_reduce_slots<<<nslot, 256, 0, st>>>(L->part, nbh, nslot, L->tot, L->d_st);
+        k_sil_after_hc<<<1, 64, 0, st>>>(L->d_st, L->tot, it);
+        if (it == L->lm - 1) break;
+        if (L->prec == kPrecFP64) reorth_dispatch<double2>(L, it, st);
+        else reorth_dispatch<float2>(L, it, st);
+        k_sil_after_reorth<<<1, 64, 0, st>>>(L->d_st, L->tot, it);
+    }
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
+int sil_step(Sil* L, double2* C, const double2* hob, const double2* hint, double dt_re, double dt_im, int imagtime,
+             hipStream_t st) {
+    int rc = sil_run_lanczos(L, C, hob, hint, st);
+    if (rc) return rc;
+    k_sil_tridiag<<<1, 64, 0, st>>>(L->d_st, dt_re, dt_im);
+    WPtrs wp{};
+    for (int k = 0; k < L->lm; k++) wp.w[k] = L->w[k];
+    if (L->prec == kPrecFP64)
+        k_sil_recombine<double2><<<kStreamGrid, kStreamBlock, 0, st>>>(C, wp, L->S->dim, L->d_st, L->part);
+    else
+        k_sil_recombine<float2><<<kStreamGrid, kStreamBlock, 0, st>>>(C, wp, L->S->dim, L->d_st, L->part);
+    if (imagtime) {
+        k_reduce_slots<<<1, 256, 0, st>>>(L->part, kStreamGrid, 1, L->tot + 48, nullptr);
+        k_scale_by_inv_norm<<<kStreamGrid, kStreamBlock, 0, st>>>(C, L->S->dim, L->tot + 48);
+    }
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
+int sil_lanczos(Sil* L, const double2* C, const double2* hob, const double2* hint, double2* lvec, hipStream_t st) {
+    int rc = sil_run_lanczos(L, C, hob, hint, st);
+    if (rc) return rc;
+    const uint32_t n = L->S->dim;
+    MCTDHB_HIP_CHECK(hipMemcpyAsync(lvec, C, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
+    for (int k = 1; k < L->lm; k++) {
+        if (L->prec == kPrecFP64)
+            k_export_vector<double2><<<kStreamGrid, kStreamBlock, 0, st>>>((const double2*) L->w[k],
+                                                                           lvec + (size_t) k * n, n, L->d_st, k);
+        else
+            k_export_vector<float2><<<kStreamGrid, kStreamBlock, 0, st>>>((const float2*) L->w[k],
+                                                                          lvec + (size_t) k * n, n, L->d_st, k);
+    }
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
+int sil_read_state(const Sil* L, SilState* host, hipStream_t st) {
+    MCTDHB_HIP_CHECK(hipMemcpyAsync(host, L->d_st, sizeof(SilState), hipMemcpyDeviceToHost, st));
+    MCTDHB_HIP_CHECK(hipStreamSynchronize(st));
+    return kOk;
+}
+
+}  // namespace mctdhb

@@ -1,0 +1,39 @@
+// space.hpp - the device-resident configurational space (tables + LUTs).
+#pragma once
+
+#include <vector>
+
+#include "fock.hpp"
+
+namespace mctdhb {
+
+constexpr int kMaxOrb = 30;    // reference MAX_ORBITALS (mctdhb_types.h:29)
+constexpr int kMaxPar = 10000; // reference MAX_PARTICLES (mctdhb_types.h:31)
+constexpr int kSpaceTables = 1;  // also build the reference-layout jump tables
+
+struct Space {
+    int npar = 0, norb = 0;
+    uint32_t dim = 0;
+    int device = 0;
+    int ldw = 0;
+    std::vector<uint32_t> w_host;       // rank LUT, (M-1) x ldw
+    std::vector<uint32_t> subsdim_host; // reference subsdim layout
+    uint32_t* d_W = nullptr;
+    uint32_t* d_subsdim = nullptr;
+    uint16_t* d_ht = nullptr;
+    uint32_t* d_map = nullptr;
+    uint32_t* d_mapot = nullptr;
+    uint32_t* d_maptt = nullptr;
+    uint32_t* d_strideot = nullptr;
+    uint32_t* d_stridett = nullptr;
+    uint64_t mapot_len = 0, maptt_len = 0;
+    bool tables = false;
+};
+
+int space_dimension_exact(int npar, int norb, uint32_t* dim);
+int space_create(int npar, int norb, int flags, Space** out);
+void space_destroy(Space* S);
+uint32_t rank_host(int npar, int norb, const uint16_t* conf);
+void unrank_host(uint32_t i, int npar, int norb, uint16_t* conf);
+
+}  // namespace mctdhb

@@ -1,0 +1,260 @@
+"""Python front end of libmctdhb_hip.so, the MI355X MCTDHB configurational core.
+
+Mirrors the reference library's C interface (reference include/mctdhb_types.h
+and the headers under include/configurational, include/linalg,
+include/integrator) over ctypes:
+
+* ``Multiconf`` wraps the reference-named drop-in entry points
+  (``get_multiconf_struct``, ``apply_hamiltonian``, ``set_onebody_dm``,
+  ``set_twobody_dm``, ``lanczos``) on host numpy arrays;
+* ``DeviceSpace`` / ``DeviceSil`` wrap the device-resident ``mctdhb_*``
+  entry points on device pointers (torch tensors are plumbing only: they
+  own device memory and streams).
+
+There is no CPU fallback: loading fails loudly when the HIP library is not
+built, and every call raises ``MctdhbError`` with the library's message on a
+non-zero status.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libmctdhb_hip.so")
+
+FP64, MIXED = 0, 1
+INDEX_TABLES, INDEX_FLY = 0, 1
+SPACE_TABLES = 1
+IMAGTIME, REALTIME = 10, 11
+
+_lib = None
+
+
+class MctdhbError(RuntimeError):
+    pass
+
+
+def build(jobs: int = 8) -> str:
+    """Compile every HIP source for gfx950 with the package Makefile."""
+    subprocess.run(["make", "-s", f"-j{jobs}", "-C", PKG_ROOT], check=True)
+    return LIB_PATH
+
+
+class _OperatorMappings(C.Structure):
+    _fields_ = [("map", C.POINTER(C.c_uint32)), ("mapot", C.POINTER(C.c_uint32)),
+                ("maptt", C.POINTER(C.c_uint32)), ("strideot", C.POINTER(C.c_uint32)),
+                ("stridett", C.POINTER(C.c_uint32))]
+
+
+class _MultiConfiguration(C.Structure):
+    """reference _MultiConfiguration (mctdhb_types.h:161-169) + appended dev."""
+    _fields_ = [("npar", C.c_uint16), ("norb", C.c_uint16), ("dim", C.c_uint32),
+                ("hash_table", C.POINTER(C.c_uint16)), ("subspaces_dim", C.POINTER(C.c_uint32)),
+                ("op_maps", C.POINTER(_OperatorMappings)), ("dev", C.c_void_p)]
+
+
+_vp = C.c_void_p
+_i = C.c_int
+_cpp = C.POINTER(C.c_void_p)
+
+
+def lib():
+    """Load the HIP library (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MctdhbError(f"{LIB_PATH} is not built; run mctdhb_amd.build() (hipcc, gfx950)")
+    L = C.CDLL(LIB_PATH)
+    L.mctdhb_last_error.restype = C.c_char_p
+    L.mctdhb_device_count.argtypes = [C.POINTER(_i)]
+    L.mctdhb_space_create.argtypes = [C.c_uint16, C.c_uint16, _i, _cpp]
+    L.mctdhb_space_destroy.argtypes = [_vp]
+    L.mctdhb_space_info.argtypes = [_vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
+                                    C.POINTER(C.c_uint64), C.POINTER(_i)]
+    L.mctdhb_space_tables.argtypes = [_vp] + [_cpp] * 7
+    L.mctdhb_apply_hamiltonian_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]
+    L.mctdhb_density_dev.argtypes = [_vp, _vp, _vp, _vp, _i, _i, _vp]
+    L.mctdhb_sil_create.argtypes = [_vp, _i, _i, _i, _cpp]
+    L.mctdhb_sil_destroy.argtypes = [_vp]
+    L.mctdhb_sil_device_bytes.argtypes = [_vp, _i, _i]
+    L.mctdhb_sil_device_bytes.restype = C.c_size_t
+    L.mctdhb_sil_step_dev.argtypes = [_vp, _vp, _vp, _vp, C.c_double, C.c_double, _i, _vp]
+    L.mctdhb_lanczos_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+    L.mctdhb_sil_scalars.argtypes = [_vp, C.POINTER(_i), _vp, _vp, _vp]
+    # reference-named drop-in layer
+    L.space_dimension.argtypes = [C.c_uint16, C.c_uint16]
+    L.space_dimension.restype = C.c_uint32
+    L.get_multiconf_struct.argtypes = [C.c_uint16, C.c_uint16]
+    L.get_multiconf_struct.restype = C.POINTER(_MultiConfiguration)
+    L.destroy_multiconf_struct.argtypes = [C.POINTER(_MultiConfiguration)]
+    L.set_config.argtypes = [C.c_uint32, C.c_uint16, C.c_uint16, C.POINTER(C.c_uint16)]
+    L.hash_index.argtypes = [C.c_uint16, C.c_uint16, C.POINTER(C.c_uint32), C.POINTER(C.c_uint16)]
+    L.hash_index.restype = C.c_uint32
+    L.apply_hamiltonian.argtypes = [C.POINTER(_MultiConfiguration), _vp, C.POINTER(_vp), _vp, _vp]
+    L.set_onebody_dm.argtypes = [C.POINTER(_MultiConfiguration), _vp, C.POINTER(_vp)]
+    L.set_twobody_dm.argtypes = [C.POINTER(_MultiConfiguration), _vp, _vp]
+    L.lanczos.argtypes = [C.POINTER(_MultiConfiguration), C.POINTER(_vp), _vp, _i, _vp, _vp, C.POINTER(_vp)]
+    L.lanczos.restype = _i
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise MctdhbError(f"libmctdhb_hip error {rc}: {lib().mctdhb_last_error().decode()}")
+
+
+def device_count() -> int:
+    n = _i()
+    check(lib().mctdhb_device_count(C.byref(n)))
+    return n.value
+
+
+def _ptr(x) -> int:
+    """Device address of a torch tensor (or an int address)."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def _stream(s):
+    if s is None:
+        return None
+    return s.cuda_stream if hasattr(s, "cuda_stream") else int(s)
+
+
+class DeviceSpace:
+    """Configurational space (hash table, optional jump tables) on the GPU."""
+
+    def __init__(self, npar: int, norb: int, tables: bool = True):
+        h = _vp()
+        check(lib().mctdhb_space_create(npar, norb, SPACE_TABLES if tables else 0, C.byref(h)))
+        self._h = h
+        self.npar, self.norb = npar, norb
+        dim, ot, tt, ht = C.c_uint32(), C.c_uint64(), C.c_uint64(), _i()
+        check(lib().mctdhb_space_info(h, C.byref(dim), C.byref(ot), C.byref(tt), C.byref(ht)))
+        self.dim, self.mapot_len, self.maptt_len, self.has_tables = dim.value, ot.value, tt.value, bool(ht.value)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mctdhb_space_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def table_pointers(self) -> dict:
+        p = [_vp() for _ in range(7)]
+        check(lib().mctdhb_space_tables(self._h, *[C.byref(x) for x in p]))
+        names = ["hash_table", "subspaces_dim", "map", "mapot", "maptt", "strideot", "stridett"]
+        return {n: x.value for n, x in zip(names, p)}
+
+    def apply_hamiltonian(self, coef, hob, hint, out, prec=FP64, index=INDEX_FLY, stream=None):
+        check(lib().mctdhb_apply_hamiltonian_dev(self._h, _ptr(coef), _ptr(hob), _ptr(hint), _ptr(out),
+                                                 prec, index, _stream(stream)))
+
+    def density(self, coef, rho1, rho2=None, prec=FP64, index=INDEX_FLY, stream=None):
+        check(lib().mctdhb_density_dev(self._h, _ptr(coef), _ptr(rho1), _ptr(rho2), prec, index,
+                                       _stream(stream)))
+
+
+class DeviceSil:
+    """Krylov workspace of the short-iterative-Lanczos step."""
+
+    def __init__(self, space: DeviceSpace, lanczos_iter: int, prec=FP64, index=INDEX_FLY):
+        h = _vp()
+        check(lib().mctdhb_sil_create(space.handle, lanczos_iter, prec, index, C.byref(h)))
+        self._h, self.space, self.lm, self.prec = h, space, lanczos_iter, prec
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mctdhb_sil_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def step(self, coef, hob, hint, prop_dt: complex, imagtime: bool, stream=None):
+        check(lib().mctdhb_sil_step_dev(self._h, _ptr(coef), _ptr(hob), _ptr(hint), float(np.real(prop_dt)),
+                                        float(np.imag(prop_dt)), int(bool(imagtime)), _stream(stream)))
+
+    def lanczos(self, coef, hob, hint, lvec, stream=None):
+        check(lib().mctdhb_lanczos_dev(self._h, _ptr(coef), _ptr(hob), _ptr(hint), _ptr(lvec), _stream(stream)))
+
+    def scalars(self, stream=None):
+        done = _i()
+        diag = np.zeros(2 * self.lm)
+        offd = np.zeros(self.lm)
+        check(lib().mctdhb_sil_scalars(self._h, C.byref(done), diag.ctypes.data, offd.ctypes.data,
+                                       _stream(stream)))
+        return done.value, diag.view(np.complex128), offd
+
+
+class Multiconf:
+    """The reference-named drop-in layer on host numpy arrays
+    (get_multiconf_struct / apply_hamiltonian / set_onebody_dm /
+    set_twobody_dm / lanczos, reference hamiltonian.h:7-13,
+    density_matrices.h:7-12, multiconfig_lanczos.h:31-39)."""
+
+    def __init__(self, npar: int, norb: int):
+        self._p = lib().get_multiconf_struct(npar, norb)
+        s = self._p.contents
+        self.npar, self.norb, self.dim = int(s.npar), int(s.norb), int(s.dim)
+        m2 = self.norb * self.norb
+        om = s.op_maps.contents
+        self.hash_table = np.ctypeslib.as_array(s.hash_table, (self.dim * self.norb,))
+        self.map = np.ctypeslib.as_array(om.map, (self.dim * m2,))
+        self.strideot = np.ctypeslib.as_array(om.strideot, (self.dim,))
+        self.stridett = np.ctypeslib.as_array(om.stridett, (self.dim,))
+
+    def close(self):
+        if getattr(self, "_p", None):
+            lib().destroy_multiconf_struct(self._p)
+            self._p = None
+
+    __del__ = close
+
+    @staticmethod
+    def _rows(mat: np.ndarray):
+        rows = (_vp * mat.shape[0])(*[mat[r].ctypes.data for r in range(mat.shape[0])])
+        return rows
+
+    def apply_hamiltonian(self, coef, hob, hint):
+        coef = np.ascontiguousarray(coef, np.complex128)
+        hob = np.ascontiguousarray(hob, np.complex128).reshape(self.norb, self.norb)
+        hint = np.ascontiguousarray(hint, np.complex128)
+        out = np.zeros(self.dim, np.complex128)
+        lib().apply_hamiltonian(self._p, coef.ctypes.data, self._rows(hob), hint.ctypes.data, out.ctypes.data)
+        return out
+
+    def onebody_dm(self, coef):
+        coef = np.ascontiguousarray(coef, np.complex128)
+        rho = np.zeros((self.norb, self.norb), np.complex128)
+        lib().set_onebody_dm(self._p, coef.ctypes.data, self._rows(rho))
+        return rho
+
+    def twobody_dm(self, coef):
+        coef = np.ascontiguousarray(coef, np.complex128)
+        rho = np.zeros(self.norb ** 4, np.complex128)
+        lib().set_twobody_dm(self._p, coef.ctypes.data, rho.ctypes.data)
+        return rho
+
+    def lanczos(self, coef, hob, hint, lm):
+        hob = np.ascontiguousarray(hob, np.complex128).reshape(self.norb, self.norb)
+        hint = np.ascontiguousarray(hint, np.complex128)
+        lvec = np.zeros((lm, self.dim), np.complex128)
+        lvec[0] = coef
+        diag = np.zeros(lm, np.complex128)
+        offd = np.zeros(lm, np.complex128)
+        done = lib().lanczos(self._p, self._rows(hob), hint.ctypes.data, lm, diag.ctypes.data,
+                             offd.ctypes.data, self._rows(lvec))
+        return done, diag, offd, lvec

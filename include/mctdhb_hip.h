@@ -1,0 +1,153 @@
+/*
+ * mctdhb_hip.h - C ABI of libmctdhb_hip.so, the MI355X (gfx950) MCTDHB
+ * configurational core.
+ *
+ * Two layers, both plain C (no device or torch types in any signature):
+ *
+ * 1. Reference-named drop-in entry points.  Same names, argument order and
+ *    meaning as the reference C library; host arrays in, host arrays out,
+ *    the work done on the GPU.  They keep the reference's fail-stop error
+ *    behaviour (message on stdout, exit(EXIT_FAILURE)).  Each declaration
+ *    cites the reference interface it replaces (file:line).
+ *
+ * 2. Device-resident entry points (mctdhb_*).  Inputs and outputs are device
+ *    pointers (complex128 = interleaved doubles, complex64 = interleaved
+ *    floats), `stream` is a hipStream_t passed as void* (NULL = default
+ *    stream), calls are asynchronous on that stream, and every function
+ *    returns 0 or an error code with the message in mctdhb_last_error().
+ */
+#ifndef MCTDHB_HIP_H
+#define MCTDHB_HIP_H
+
+#include <stddef.h>
+
+#include "mctdhb_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ================================================ 1. reference drop-in */
+
+/* include/configurational/space.h:14-221, src/configurational/space.c */
+void      assert_space_parameters(uint16_t npar, uint16_t norb);                           /* space.c:61-82 */
+uint32_t  space_dimension(uint16_t npar, uint16_t norb);                                   /* space.c:84-117 */
+uint32_t* get_subspaces_dim(uint16_t npar, uint16_t norb);                                 /* space.c:119-135 */
+void      set_config(uint32_t hash_i, uint16_t npar, uint16_t norb, uint16_t* conf);       /* space.c:137-156 */
+uint32_t  hash_index(uint16_t npar, uint16_t norb, uint32_t* subspaces_dim, uint16_t* conf); /* space.c:158-178 */
+uint16_t* get_hash_table(uint16_t npar, uint16_t norb);                                    /* space.c:180-199 */
+uint32_t* get_single_jump_map(uint16_t npar, uint16_t norb, uint32_t* subsdim,
+                              uint16_t* hash_table);                                       /* space.c:201-251 */
+uint32_t* get_double_diffjump_map(uint16_t npar, uint16_t norb, uint32_t* subsdim,
+                                  uint16_t* hash_table, uint32_t* strides);                /* space.c:296-358 */
+uint32_t* get_double_equaljump_map(uint16_t npar, uint16_t norb, uint32_t* subsdim,
+                                   uint16_t* hash_table, uint32_t* strides);               /* space.c:398-451 */
+uint32_t  jump1_index(uint16_t norb, uint32_t dim, uint32_t inp_index, uint32_t* map1jump,
+                      uint16_t from_orb, uint16_t to_orb);                                 /* space.c:453-463 */
+uint32_t  jump2same_index(uint16_t norb, uint16_t* conf, uint32_t conf_stride, uint32_t* mapot,
+                          uint16_t from_orb, uint16_t to_orb1, uint16_t to_orb2);          /* space.c:465-486 */
+uint32_t  jump2diff_index(uint16_t norb, uint16_t* conf, uint32_t conf_stride, uint32_t* maptt,
+                          uint16_t from_orb1, uint16_t from_orb2, uint16_t to_orb1,
+                          uint16_t to_orb2);                                               /* space.c:488-524 */
+
+/* include/assistant/types_definition.h:22-113, src/assistant/types_definition.c */
+MultiConfiguration get_multiconf_struct(uint16_t npar, uint16_t norb);          /* types_definition.c:110-141 */
+void               destroy_multiconf_struct(MultiConfiguration multiconf);       /* types_definition.c:347-360 */
+ManyBodyState      get_manybody_state(uint16_t npar, uint16_t norb, uint16_t grid_size); /* :143-160 */
+void               destroy_manybody_sate(ManyBodyState state);                   /* :361-373 (reference spelling) */
+WorkspaceLanczos   get_lanczos_workspace(uint16_t iter, uint32_t space_dim);     /* :162-179 */
+void               destroy_lanczos_workspace(WorkspaceLanczos lan_work);         /* :374-389 */
+CoefWorkspace      get_coef_workspace(CoefIntegrator coef_integ_method, uint32_t space_dim,
+                                      uint16_t lan_it);                          /* :181-198 */
+void               destroy_coef_workspace(CoefWorkspace coef_work);              /* :408-421 */
+
+/* include/configurational/hamiltonian.h:7-13 */
+void apply_hamiltonian(MultiConfiguration multiconf, Carray coef, Cmatrix hob, Carray hint,
+                       Carray hc);                                               /* hamiltonian.c:6-406 */
+
+/* include/configurational/density_matrices.h:7-12 */
+void set_onebody_dm(MultiConfiguration multiconf, Carray coef, Cmatrix rho);    /* density_matrices.c:8-52 */
+void set_twobody_dm(MultiConfiguration multiconf, Carray coef, Carray rho);     /* density_matrices.c:54-443 */
+
+/* include/linalg/multiconfig_lanczos.h:31-39 */
+int lanczos(MultiConfiguration multiconf, Cmatrix hob_mat, Carray hint_mat, int iter, Carray diag,
+            Carray offd, Cmatrix lvec);                                          /* multiconfig_lanczos.c:11-90 */
+
+/* include/integrator/coefficients_integration.h:25-26 */
+void propagate_coef_sil(MCTDHBDataStruct mctdhb, Carray cnext);                  /* coefficients_integration.c:59-160 */
+
+/* include/assistant/integrator_monitor.h:53 */
+dcomplex total_energy(ManyBodyState psi);                                        /* integrator_monitor.c:55-88 */
+
+/* ================================================= 2. device-resident */
+
+typedef struct mctdhb_space mctdhb_space; /* configurational space on one GPU */
+typedef struct mctdhb_sil   mctdhb_sil;   /* Krylov workspace of the SIL step */
+
+/* status codes */
+#define MCTDHB_OK 0
+#define MCTDHB_EINVAL 1
+#define MCTDHB_EOVERFLOW 2
+#define MCTDHB_EDEVICE 3
+#define MCTDHB_ENOMEM 4
+#define MCTDHB_ENOTBUILT 5
+#define MCTDHB_ENUMERIC 6
+
+/* space flags */
+#define MCTDHB_SPACE_TABLES 1 /* also build the reference-layout jump tables */
+
+/* arithmetic forms */
+#define MCTDHB_FP64 0  /* complex128 throughout (the parity form) */
+#define MCTDHB_MIXED 1 /* complex64 H.C and density sums (Kahan), FP64 Lanczos scalars */
+
+/* index schemes */
+#define MCTDHB_INDEX_TABLES 0 /* read the reference jump tables */
+#define MCTDHB_INDEX_FLY 1    /* closed-form rank deltas, no table traffic */
+
+const char* mctdhb_last_error(void);
+int         mctdhb_device_count(int* count);
+
+/* Build the Fock-space hash table (and with MCTDHB_SPACE_TABLES the
+ * map/mapot/maptt/strides of get_multiconf_struct) on the current device. */
+int mctdhb_space_create(uint16_t npar, uint16_t norb, int flags, mctdhb_space** out);
+int mctdhb_space_destroy(mctdhb_space* space);
+int mctdhb_space_info(const mctdhb_space* space, uint32_t* dim, uint64_t* mapot_len,
+                      uint64_t* maptt_len, int* has_tables);
+/* Device pointers of the tables (NULL where not built). */
+int mctdhb_space_tables(const mctdhb_space* space, const uint16_t** hash_table,
+                        const uint32_t** subspaces_dim, const uint32_t** map, const uint32_t** mapot,
+                        const uint32_t** maptt, const uint32_t** strideot, const uint32_t** stridett);
+
+/* hc = H coef.  hob: M x M row-major complex128; hint: M^4 complex128 with
+ * hint[k + sM + qM^2 + lM^3].  coef/hc complex128 (FP64) or complex64 (MIXED). */
+int mctdhb_apply_hamiltonian_dev(const mctdhb_space* space, const void* coef, const void* hob,
+                                 const void* hint, void* hc, int prec, int index, void* stream);
+
+/* rho1 (M x M, <a+_k a_l>) and rho2 (M^4, <a+_k a+_s a_q a_l>), complex128;
+ * rho2 may be NULL.  coef complex128 (FP64) or complex64 (MIXED). */
+int mctdhb_density_dev(mctdhb_space* space, const void* coef, void* rho1, void* rho2, int prec,
+                       int index, void* stream);
+
+/* SIL workspace for lanczos_iter Krylov vectors (2..15). */
+int mctdhb_sil_create(const mctdhb_space* space, int lanczos_iter, int prec, int index,
+                      mctdhb_sil** out);
+int mctdhb_sil_destroy(mctdhb_sil* sil);
+size_t mctdhb_sil_device_bytes(const mctdhb_space* space, int lanczos_iter, int prec);
+/* coef (complex128, device) <- exp(-i H prop_dt) coef; renormalised when
+ * imagtime != 0 (prop_dt = -i dt in imaginary time, dt in real time). */
+int mctdhb_sil_step_dev(mctdhb_sil* sil, void* coef, const void* hob, const void* hint,
+                        double prop_dt_re, double prop_dt_im, int imagtime, void* stream);
+/* The Lanczos recursion alone; lvec (lanczos_iter x dim complex128) gets the
+ * normalised Krylov vectors, lvec row 0 = coef. */
+int mctdhb_lanczos_dev(mctdhb_sil* sil, const void* coef, const void* hob, const void* hint,
+                       void* lvec, void* stream);
+/* Host copy of the recursion scalars of the last call (synchronises):
+ * diag (2*lanczos_iter doubles, complex), offd (lanczos_iter doubles). */
+int mctdhb_sil_scalars(const mctdhb_sil* sil, int* iter_done, double* diag, double* offd,
+                       void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
