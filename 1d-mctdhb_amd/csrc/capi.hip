@@ -135,6 +135,28 @@ extern "C" int mctdhb_apply_hamiltonian_dev(const mctdhb_space* sp, const void* 
                     (hipStream_t) stream);
 }
 
+extern "C" size_t mctdhb_hc_coef_bytes(const mctdhb_space* sp, int prec) {
+    return sp ? hc_coef_bytes(sp->S, prec) : 0;
+}
+
+extern "C" int mctdhb_hc_coefficients_dev(const mctdhb_space* sp, const void* hob, const void* hint, void* cf,
+                                          int prec, void* stream) {
+    if (!sp || !hob || !hint || !cf) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    return hc_build_coef(sp->S, (const double2*) hob, (const double2*) hint, cf, prec, (hipStream_t) stream);
+}
+
+extern "C" int mctdhb_apply_hamiltonian_cf_dev(const mctdhb_space* sp, const void* coef, const void* cf, void* hc,
+                                               int prec, int index, void* stream) {
+    if (!sp || !coef || !cf || !hc) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    return hc_apply_cf(sp->S, coef, cf, hc, prec, index, (hipStream_t) stream);
+}
+
 extern "C" int mctdhb_density_dev(mctdhb_space* sp, const void* coef, void* rho1, void* rho2, int prec, int index,
                                   void* stream) {
     if (!sp || !coef || !rho1) {

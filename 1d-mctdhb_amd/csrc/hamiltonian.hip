@@ -345,6 +345,44 @@ int hc_apply(const Space* S, const void* coef, const double2* hob, const double2
     return rc;
 }
 
+size_t hc_coef_bytes(const Space* S, int prec) {
+    const int M = S->norb;
+    if (use_generic(S)) return sizeof(double2) * ((size_t) M * M + (size_t) M * M * M * M);
+    // FP64 block, followed by the complex64 copy for the mixed form
+    size_t b = sizeof(double2) * cf_size(M);
+    if (prec == kPrecMixed) b += sizeof(float2) * cf_size(M);
+    return b;
+}
+
+int hc_build_coef(const Space* S, const double2* hob, const double2* hint, void* cf, int prec, hipStream_t st) {
+    const int M = S->norb;
+    if (use_generic(S)) {
+        MCTDHB_HIP_CHECK(hipMemcpyAsync(cf, hob, sizeof(double2) * M * M, hipMemcpyDeviceToDevice, st));
+        MCTDHB_HIP_CHECK(hipMemcpyAsync((double2*) cf + M * M, hint, sizeof(double2) * M * M * M * M,
+                                        hipMemcpyDeviceToDevice, st));
+        return kOk;
+    }
+    double2* cf64 = (double2*) cf;
+    float2* cf32 = prec == kPrecMixed ? (float2*) (cf64 + cf_size(M)) : nullptr;
+    return hc_coefficients(S, hob, hint, cf64, cf32, st);
+}
+
+int hc_apply_cf(const Space* S, const void* coef, const void* cf, void* hc, int prec, int index, hipStream_t st) {
+    int rc = check_mode(S, prec, index);
+    if (rc) return rc;
+    const int M = S->norb;
+    if (use_generic(S)) {
+        const double2* hob = (const double2*) cf;
+        EpiPlainG<double2> epi{(double2*) hc};
+        k_hc_generic<<<hc_grid_blocks(S->dim), kHcBlockG, 0, st>>>(S->norb, row_ctx_g(S), (const double2*) coef,
+                                                                   hob, hob + M * M, 0, nullptr, epi);
+        MCTDHB_HIP_CHECK(hipGetLastError());
+        return kOk;
+    }
+    const void* blk = prec == kPrecFP64 ? cf : (const void*) ((const double2*) cf + cf_size(M));
+    return dispatch_fused_plain(S, coef, blk, hc, prec, index, st);
+}
+
 int hc_lanczos(const Space* S, const double2* cf64, const float2* cf32, const LanczosHcArgs& a, int prec,
                int index, hipStream_t st) {
     int rc = check_mode(S, prec, index);

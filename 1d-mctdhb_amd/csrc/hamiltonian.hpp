@@ -60,6 +60,12 @@ int hc_coefficients(const Space* S, const double2* hob, const double2* hint, dou
 int hc_apply(const Space* S, const void* coef, const double2* hob, const double2* hint, void* hc, int prec,
              int index, hipStream_t st);
 
+// Coefficient-block entry points: bytes needed, build, and H.C with a
+// prebuilt block (the generic kernel's block is [hob | hint] in FP64).
+size_t hc_coef_bytes(const Space* S, int prec);
+int hc_build_coef(const Space* S, const double2* hob, const double2* hint, void* cf, int prec, hipStream_t st);
+int hc_apply_cf(const Space* S, const void* coef, const void* cf, void* hc, int prec, int index, hipStream_t st);
+
 // Lanczos form (iteration it): out = s_it H w_it - beta_{it-1} s_{it-1} w_{it-1}
 // (out is complex128) plus per-block partial sums, part[blk * nslot + 2k..2k+1]
 // = sum conj(s_k w_k) out for k <= it and, at it = 0, slot it+1 = |w_0|^2.

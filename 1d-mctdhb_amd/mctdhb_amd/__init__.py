@@ -78,6 +78,10 @@ def lib():
                                     C.POINTER(C.c_uint64), C.POINTER(_i)]
     L.mctdhb_space_tables.argtypes = [_vp] + [_cpp] * 7
     L.mctdhb_apply_hamiltonian_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]
+    L.mctdhb_hc_coef_bytes.argtypes = [_vp, _i]
+    L.mctdhb_hc_coef_bytes.restype = C.c_size_t
+    L.mctdhb_hc_coefficients_dev.argtypes = [_vp, _vp, _vp, _vp, _i, _vp]
+    L.mctdhb_apply_hamiltonian_cf_dev.argtypes = [_vp, _vp, _vp, _vp, _i, _i, _vp]
     L.mctdhb_density_dev.argtypes = [_vp, _vp, _vp, _vp, _i, _i, _vp]
     L.mctdhb_sil_create.argtypes = [_vp, _i, _i, _i, _cpp]
     L.mctdhb_sil_destroy.argtypes = [_vp]
@@ -162,6 +166,16 @@ class DeviceSpace:
     def apply_hamiltonian(self, coef, hob, hint, out, prec=FP64, index=INDEX_FLY, stream=None):
         check(lib().mctdhb_apply_hamiltonian_dev(self._h, _ptr(coef), _ptr(hob), _ptr(hint), _ptr(out),
                                                  prec, index, _stream(stream)))
+
+    def coef_bytes(self, prec=FP64) -> int:
+        return int(lib().mctdhb_hc_coef_bytes(self._h, prec))
+
+    def hc_coefficients(self, hob, hint, cf, prec=FP64, stream=None):
+        check(lib().mctdhb_hc_coefficients_dev(self._h, _ptr(hob), _ptr(hint), _ptr(cf), prec, _stream(stream)))
+
+    def apply_hamiltonian_cf(self, coef, cf, out, prec=FP64, index=INDEX_FLY, stream=None):
+        check(lib().mctdhb_apply_hamiltonian_cf_dev(self._h, _ptr(coef), _ptr(cf), _ptr(out), prec, index,
+                                                    _stream(stream)))
 
     def density(self, coef, rho1, rho2=None, prec=FP64, index=INDEX_FLY, stream=None):
         check(lib().mctdhb_density_dev(self._h, _ptr(coef), _ptr(rho1), _ptr(rho2), prec, index,

@@ -123,6 +123,15 @@ int mctdhb_space_tables(const mctdhb_space* space, const uint16_t** hash_table,
 int mctdhb_apply_hamiltonian_dev(const mctdhb_space* space, const void* coef, const void* hob,
                                  const void* hint, void* hc, int prec, int index, void* stream);
 
+/* The (hob, hint)-derived coefficient block of the H.C kernels, built once
+ * and reused across applications with the same one- and two-body matrices.
+ * cf must hold mctdhb_hc_coef_bytes(space, prec) bytes of device memory. */
+size_t mctdhb_hc_coef_bytes(const mctdhb_space* space, int prec);
+int    mctdhb_hc_coefficients_dev(const mctdhb_space* space, const void* hob, const void* hint, void* cf,
+                                  int prec, void* stream);
+int    mctdhb_apply_hamiltonian_cf_dev(const mctdhb_space* space, const void* coef, const void* cf, void* hc,
+                                       int prec, int index, void* stream);
+
 /* rho1 (M x M, <a+_k a_l>) and rho2 (M^4, <a+_k a+_s a_q a_l>), complex128;
  * rho2 may be NULL.  coef complex128 (FP64) or complex64 (MIXED). */
 int mctdhb_density_dev(mctdhb_space* space, const void* coef, void* rho1, void* rho2, int prec,

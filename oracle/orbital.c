@@ -406,7 +406,7 @@ orc_solve_tridiag(uint32_t n, const ocplx* upper, const ocplx* lower, const ocpl
 
 /* Reference tridiagonal_solver.c:161-212: cyclic system through
  * Sherman-Morrison with gamma = mid[0] (or -upper[0] if mid[0] == 0).
- * upper[n-1] couples row n-1 to x_0 and lower[n-1] row 0 to x_{n-1}. */
+ * upper[n-1] couples row 0 to x_{n-1} and lower[n-1] row n-1 to x_0. */
 void
 orc_solve_cyclic_tridiag_sm(uint32_t n, const ocplx* upper, const ocplx* lower, ocplx* mid,
                             const ocplx* rhs, ocplx* ans)

@@ -1,73 +1,3 @@
-"""Seeded synthetic inputs shared by the tests and bench.py.
-
-Coefficients: re, im ~ N(0, 1), normalised to 1.  One-body h = (A + A^H)/2
-with A ~ CN(0, 1).  Two-body g: G ~ scale * CN(0, 1) averaged over the group
-generated by g[ksql] -> g[skql], g[ksql] -> g[kslq] and
-g[ksql] -> conj(g[lqsk]) (index k + sM + qM^2 + lM^3), which yields the
-symmetries the reference's H.C rules assume (exchange within each pair and
-Hermiticity).  numpy's PCG64 generator with a fixed seed per stream makes
-every input reproducible.
-"""
-from __future__ import annotations
-
-import itertools
-
-import numpy as np
-
-STREAM_COEF, STREAM_H, STREAM_G, STREAM_ORB = 0, 1, 2, 3
-
-
-def rng(seed: int, stream: int) -> np.random.Generator:
-    return np.random.Generator(np.random.PCG64([seed, stream]))
-
-
-def coefficients(dim: int, seed: int = 7) -> np.ndarray:
-    r = rng(seed, STREAM_COEF)
-    c = r.standard_normal(dim) + 1j * r.standard_normal(dim)
-    return c / np.linalg.norm(c)
-
-
-def onebody(norb: int, seed: int = 7, scale: float = 1.0) -> np.ndarray:
-    r = rng(seed, STREAM_H)
-    a = scale * (r.standard_normal((norb, norb)) + 1j * r.standard_normal((norb, norb)))
-    return 0.5 * (a + a.conj().T)
-
-
-def twobody(norb: int, seed: int = 7, scale: float = 0.1) -> np.ndarray:
-    """Flat M^4 array g[k + sM + qM^2 + lM^3] with the reference symmetries."""
-    M = norb
-    r = rng(seed, STREAM_G)
-    G = scale * (r.standard_normal((M,) * 4) + 1j * r.standard_normal((M,) * 4))
-    # G[k, s, q, l] is element (k, s, q, l); generate the 8-element group
-    acc = np.zeros_like(G)
-    for swap_ks, swap_ql, herm in itertools.product((0, 1), repeat=3):
-        T = G
-        if herm:  # g[k,s,q,l] -> conj(g[l,q,s,k])
-            T = np.conj(np.transpose(T, (3, 2, 1, 0)))
-        if swap_ks:
-            T = np.transpose(T, (1, 0, 2, 3))
-        if swap_ql:
-            T = np.transpose(T, (0, 1, 3, 2))
-        acc = acc + T
-    acc /= 8.0
-    # flat index k + sM + qM^2 + lM^3 == Fortran order of acc[k, s, q, l]
-    return np.ascontiguousarray(acc.reshape(-1, order="F"))
-
-
-def hermite_gauss(norb: int, x: np.ndarray, seed: int = 7, noise: float = 1e-3) -> np.ndarray:
-    """First ``norb`` harmonic-oscillator eigenfunctions plus seeded complex
-    noise, as rows of an (norb, len(x)) array (not re-orthonormalised)."""
-    r = rng(seed, STREAM_ORB)
-    out = np.zeros((norb, x.size), complex)
-    h_prev = np.zeros_like(x)
-    h_cur = np.ones_like(x)
-    for n in range(norb):
-        if n == 0:
-            h = h_cur
-        else:
-            h = 2 * x * h_cur - 2 * (n - 1) * h_prev
-            h_prev, h_cur = h_cur, h
-        norm = 1.0 / np.sqrt(2.0 ** n * float(np.prod(np.arange(1, n + 1))) * np.sqrt(np.pi))
-        out[n] = norm * h * np.exp(-x * x / 2)
-    out += noise * (r.standard_normal(out.shape) + 1j * r.standard_normal(out.shape))
-    return out
+"""Seeded synthetic inputs (re-exported from the package, shared with bench.py)."""
+from mctdhb_amd.inputs import *  # noqa: F401,F403
+from mctdhb_amd.inputs import coefficients, hermite_gauss, onebody, rng, twobody  # noqa: F401

@@ -1,0 +1,316 @@
+"""Pins the oracle's orbital split-step pieces (reference calculus.c,
+orbital_matrices.c, split_linear_orbitals.c, tridiagonal_solver.c,
+split_nonlinear_orbitals.c, lapack_interface.c, orbital_integration.c)
+against closed forms, numpy/scipy linear algebra and the reference's
+benchmark.dat known answers."""
+import math
+
+import numpy as np
+import pytest
+
+from mctdhb_amd import inputs
+
+
+def simpson_weights(n, h):
+    """Quadrature weights of the reference rule: composite 1/3 rule, with the
+    last four points on the 3/8 rule when n is even (calculus.c:8-39)."""
+    w = np.zeros(n)
+    last = n if n % 2 == 1 else n - 3
+    for i in range(0, last - 2, 2):
+        w[i] += h / 3
+        w[i + 1] += 4 * h / 3
+        w[i + 2] += h / 3
+    if n % 2 == 0:
+        for i, c in zip(range(n - 4, n), (1, 3, 3, 1)):
+            w[i] += 3 * h / 8 * c
+    return w
+
+
+@pytest.mark.parametrize("n", [5, 6, 51, 52, 257])
+def test_simpson_exact_on_cubics(orc, n):
+    x = np.linspace(-1.3, 2.1, n)
+    h = x[1] - x[0]
+    f = 2 - x + 0.5 * x ** 2 - 0.75 * x ** 3 + 1j * (x ** 3 + 3 * x)
+    exact = (2 * x - x ** 2 / 2 + x ** 3 / 6 - 0.75 * x ** 4 / 4 + 1j * (x ** 4 / 4 + 1.5 * x ** 2))
+    want = exact[-1] - exact[0]
+    assert abs(orc.csimps(f, h) - want) < 1e-12
+    assert abs(np.dot(simpson_weights(n, h), f) - want) < 1e-12
+
+
+def test_dxfd_plane_wave_closed_form(orc):
+    n = 129  # periodic grid, last point repeats the first
+    L = 10.0
+    x = np.linspace(0, L, n)
+    dx = x[1] - x[0]
+    kw = 2 * np.pi * 3 / L
+    f = np.exp(1j * kw * x)
+    out = np.zeros(n, complex)
+    orc.lib().orc_dxfd(n, dx, f, out)
+    factor = 1j * (8 * np.sin(kw * dx) - np.sin(2 * kw * dx)) / (6 * dx)
+    np.testing.assert_allclose(out, factor * f, atol=1e-11)
+
+
+@pytest.mark.parametrize("n", [256, 100, 64])
+def test_dft_matches_numpy(orc, n):
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    out = np.zeros(n, complex)
+    orc.lib().orc_dft(n, -1, 1 / math.sqrt(n), a, out)
+    np.testing.assert_allclose(out, np.fft.fft(a, norm="ortho"), atol=1e-12)
+    orc.lib().orc_dft(n, +1, 1 / math.sqrt(n), a, out)
+    np.testing.assert_allclose(out, np.fft.ifft(a, norm="ortho"), atol=1e-12)
+
+
+def _tri(n, seed, cyclic):
+    rng = np.random.default_rng(seed)
+    up = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    lo = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    mid = 4 + rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    A = np.diag(mid) + np.diag(up[: n - 1], 1) + np.diag(lo[: n - 1], -1)
+    if cyclic:  # corner convention of tridiagonal_solver.c:161-212
+        A[0, n - 1] = up[n - 1]
+        A[n - 1, 0] = lo[n - 1]
+    return up, lo, mid, A
+
+
+@pytest.mark.parametrize("n", [5, 17, 256])
+def test_tridiagonal_solvers_match_dense(orc, n):
+    rhs = np.arange(n) * (0.3 - 0.1j) + 1.0
+    up, lo, mid, A = _tri(n, n, False)
+    x = np.zeros(n, complex)
+    orc.lib().orc_solve_tridiag(n, up, lo, mid, rhs, x)
+    np.testing.assert_allclose(A @ x, rhs, atol=1e-11)
+    up, lo, mid, A = _tri(n, n + 1, True)
+    keep = mid.copy()
+    orc.lib().orc_solve_cyclic_tridiag_sm(n, up, lo, mid, rhs, x)
+    np.testing.assert_allclose(A @ x, rhs, atol=1e-10)
+    assert np.array_equal(mid, keep)  # restored after the Sherman-Morrison trick
+
+
+def _free_eq(orc, imagtime, ng=128, L=20.0, dt=0.01):
+    x = np.linspace(-L / 2, L / 2, ng + 1)
+    return orc.make_orbeq(1, ng + 1, -L / 2, L / 2, dt, 0.0, np.zeros(ng + 1), imagtime=imagtime), x
+
+
+@pytest.mark.parametrize("imagtime", [False, True])
+def test_crank_nicolson_plane_wave_cayley_factor(orc, imagtime):
+    eq, x = _free_eq(orc, imagtime)
+    L = x[-1] - x[0]
+    kw = 2 * np.pi * 5 / L
+    f = np.exp(1j * kw * x)
+    out = f.copy()
+    orc.lib().orc_advance_linear_cn(eq, out)
+    # (i I - a2 dt/2 D2) psi' = (i I + a2 dt/2 D2) psi, D2 e^{ikx} = lam e^{ikx}
+    dt = 0.5 * (eq.prop_dt_re + 1j * eq.prop_dt_im)
+    lam = (2 * np.cos(kw * eq.dx) - 2) / eq.dx ** 2
+    a2 = eq.d2coef
+    fac = (1j + a2 * dt * lam / 2) / (1j - a2 * dt * lam / 2)
+    np.testing.assert_allclose(out, fac * f, atol=1e-12)
+
+
+@pytest.mark.parametrize("imagtime", [False, True])
+def test_fft_half_step_plane_wave_phase(orc, imagtime):
+    eq, x = _free_eq(orc, imagtime)
+    L = x[-1] - x[0] + eq.dx  # period of the Ng-point grid
+    L = eq.dx * (eq.grid_size - 1)
+    kw = 2 * np.pi * 5 / L
+    f = np.exp(1j * kw * x)
+    out = f.copy()
+    orc.lib().orc_advance_linear_fft(eq, out)
+    dt = 0.5 * (eq.prop_dt_re + 1j * eq.prop_dt_im)
+    fac = np.exp(-1j * dt * (-eq.d2coef * kw * kw))
+    np.testing.assert_allclose(out, fac * f, atol=1e-11)
+
+
+def test_orthonormalize_point_by_point_update_closed_form(orc):
+    # Two functions: F1 = a F0 + b G with G orthogonal to F0.  The reference
+    # re-evaluates <F0, F1> before each grid point while F1 changes, which
+    # makes the coefficient at point k equal a * prod_{p<k} (1 - w_p |F0_p|^2).
+    n = 201
+    x = np.linspace(-8, 8, n)
+    dx = x[1] - x[0]
+    w = simpson_weights(n, dx)
+    F0 = np.exp(-x ** 2 / 2).astype(complex)
+    F0 /= np.sqrt(np.dot(w, np.abs(F0) ** 2))
+    G = x * np.exp(-x ** 2 / 2) * (1 + 0.2j)
+    G = G - np.dot(w, F0.conj() * G) * F0
+    a = 0.3 - 0.4j
+    F = np.array([F0, a * F0 + 0.8 * G])
+    got = F.copy().ravel()
+    orc.lib().orc_orthonormalize(n, dx, 2, got)
+    got = got.reshape(2, n)
+    coef = a * np.concatenate([[1.0], np.cumprod(1 - w * np.abs(F0) ** 2)[:-1]])
+    want = F[1] - coef * F0
+    want /= np.sqrt(np.dot(w, np.abs(want) ** 2))
+    np.testing.assert_allclose(got[0], F0, atol=1e-13)
+    np.testing.assert_allclose(got[1], want, atol=1e-11)
+
+
+def _hg_eq(orc, norb, ng=512, L=16.0, g=1.0):
+    x = np.linspace(-L / 2, L / 2, ng + 1)
+    orb = inputs.hermite_gauss(norb, x, noise=0.0)
+    eq = orc.make_orbeq(norb, ng + 1, -L / 2, L / 2, 0.005, g, 0.5 * x ** 2)
+    return eq, x, orb
+
+
+def test_orbital_hob_hermite_gauss_levels(orc):
+    eq, x, orb = _hg_eq(orc, 4)
+    hob = np.zeros(16, complex)
+    orc.lib().orc_orbital_hob(eq, np.ascontiguousarray(orb.ravel()), hob)
+    hob = hob.reshape(4, 4)
+    np.testing.assert_allclose(np.diag(hob).real, np.arange(4) + 0.5, atol=1e-5)  # O(dx^4) FD
+    assert np.max(np.abs(hob - np.diag(np.diag(hob)))) < 1e-5
+    assert np.allclose(hob, hob.conj().T, atol=1e-14)
+
+
+def test_orbital_hint_matches_direct_quadrature(orc):
+    norb = 4
+    x = np.linspace(-8, 8, 258)  # even point count exercises the 3/8 tail
+    orb = inputs.hermite_gauss(norb, x, seed=3, noise=0.05)
+    g = 1.7
+    eq = orc.make_orbeq(norb, x.size, x[0], x[-1], 0.005, g, 0.5 * x ** 2)
+    hint = np.zeros(norb ** 4, complex)
+    orc.lib().orc_orbital_hint(eq, np.ascontiguousarray(orb.ravel()), hint)
+    w = simpson_weights(x.size, x[1] - x[0])
+    direct = g * np.einsum("p,ap,bp,cp,dp->abcd", w, orb.conj(), orb.conj(), orb, orb)
+    np.testing.assert_allclose(hint, direct.reshape(-1, order="F"), atol=1e-13)
+    # Gaussian ground state: g / sqrt(2 pi)
+    eq0, _, orb0 = _hg_eq(orc, 1)
+    h0 = np.zeros(1, complex)
+    orc.lib().orc_orbital_hint(eq0, np.ascontiguousarray(orb0.ravel()), h0)
+    assert abs(h0[0] - 1.0 / np.sqrt(2 * np.pi)) < 1e-8
+
+
+def test_hermitian_eig_inverse_regularization(orc):
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((5, 5)) + 1j * rng.standard_normal((5, 5))
+    A = A @ A.conj().T + 0.5 * np.eye(5)
+    lam = np.zeros(5)
+    V = np.zeros(25, complex)
+    assert orc.lib().orc_herm_eig(5, np.ascontiguousarray(A.ravel()), lam, V) == 0
+    V = V.reshape(5, 5)
+    np.testing.assert_allclose(lam, np.linalg.eigvalsh(A), atol=1e-12)
+    np.testing.assert_allclose(A @ V, V * lam, atol=1e-11)
+    inv = np.zeros(25, complex)
+    assert orc.lib().orc_herm_inverse(5, np.ascontiguousarray(A.ravel()), inv) == 0
+    np.testing.assert_allclose(inv.reshape(5, 5) @ A, np.eye(5), atol=1e-11)
+    x = 0.3
+    R = np.ascontiguousarray(A.ravel()).copy()
+    orc.lib().orc_regularization(5, x, R)
+    wv, U = np.linalg.eigh(A)
+    want = A + (U * (x * np.exp(-wv / x))) @ U.conj().T
+    np.testing.assert_allclose(R.reshape(5, 5), want, atol=1e-11)
+
+
+def test_nonlinear_rhs_single_orbital_gross_pitaevskii(orc):
+    # M = 1: rho1 = N, rho2 = N(N-1), rinv = 1/N, so the nonlinear part is
+    # -h00 phi + g (N-1) |phi|^2 phi - (N-1) g0000 phi
+    npar, g = 7, 0.9
+    eq, x, orb = _hg_eq(orc, 1, ng=256, g=g)
+    hob = np.array([1.3 + 0j])
+    hint = np.array([0.21 + 0j])
+    rho2 = np.array([npar * (npar - 1) + 0j])
+    rinv = np.array([1.0 / npar + 0j])
+    der = np.zeros(x.size, complex)
+    phi = np.ascontiguousarray(orb[0])
+    orc.lib().orc_nonlinear_rhs(eq, phi, hob, hint, rho2, rinv, der)
+    V = 0.5 * x ** 2
+    nl = -hob[0] * phi + g * (npar - 1) * np.abs(phi) ** 2 * phi - (npar - 1) * hint[0] * phi
+    want = -1j * (-1j) * (nl + V * phi)  # imaginary time: -i * time_fac = -1
+    np.testing.assert_allclose(der, want, atol=1e-12)
+
+
+def test_nonlinear_rhs_is_orthogonal_to_orbital_space(orc):
+    # kinetic action + nonlinear part = (1 - P) applied to the mean-field
+    # action, so <phi_j, T phi_k + NL_k> = 0 on the grid quadrature.
+    npar, norb, g = 6, 3, 1.3
+    x = np.linspace(-8, 8, 257)
+    dx = x[1] - x[0]
+    orb = inputs.hermite_gauss(norb, x, seed=9, noise=0.02)
+    # exact orthonormalisation in the quadrature inner product (Cholesky of
+    # the Gram matrix); the reference's point-by-point Gram-Schmidt only
+    # reduces overlaps, it does not remove them
+    w = simpson_weights(x.size, dx)
+    gram = (orb.conj() * w) @ orb.T
+    orb = np.linalg.solve(np.linalg.cholesky(gram.T), orb)
+    flat = np.ascontiguousarray(orb.ravel())
+    eq = orc.make_orbeq(norb, x.size, x[0], x[-1], 0.005, g, 0.5 * x ** 2, imagtime=False)
+    S = orc.Space(npar, norb)
+    c = inputs.coefficients(S.dim, 4)
+    r1, r2, rinv = S.sync_density(c)
+    hob = np.zeros(norb * norb, complex)
+    hint = np.zeros(norb ** 4, complex)
+    orc.lib().orc_orbital_hob(eq, flat, hob)
+    orc.lib().orc_orbital_hint(eq, flat, hint)
+    der = np.zeros(flat.size, complex)
+    orc.lib().orc_nonlinear_rhs(eq, flat, hob, hint, r2, np.ascontiguousarray(rinv.ravel()), der)
+    nl = (1j * der).reshape(norb, -1)  # real time: der = -i (NL + V phi)
+    F = flat.reshape(norb, -1)
+    dF = np.zeros_like(F)
+    for k in range(norb):
+        orc.lib().orc_dxfd(x.size, dx, np.ascontiguousarray(F[k]), dF[k])
+    for j in range(norb):
+        for k in range(norb):
+            kin = -eq.d2coef * np.dot(w, dF[j].conj() * dF[k])  # <phi_j, -a2 d2 phi_k> by parts
+            proj = np.dot(w, F[j].conj() * nl[k]) + kin
+            assert abs(proj) < 1e-9, (j, k, proj)
+
+
+def test_splitstep_is_second_order_in_dt(orc):
+    # Strang splitting: linear half step, RK4 nonlinear step, linear half
+    # step; the composed real-time orbital step converges as dt^2.
+    npar, norb = 4, 2
+    x = np.linspace(-8, 8, 129)
+    orb0 = inputs.hermite_gauss(norb, x, seed=2, noise=0.02)
+    S = orc.Space(npar, norb)
+    c = inputs.coefficients(S.dim, 2)
+    r1, r2, rinv = S.sync_density(c)
+    rinv = np.ascontiguousarray(rinv.ravel())
+    finals = {}
+    for dt in (0.02, 0.01, 0.005, 0.0025):
+        eq = orc.make_orbeq(norb, x.size, x[0], x[-1], dt, 1.0, 0.5 * x ** 2, imagtime=False)
+        f = np.ascontiguousarray(orb0.ravel()).copy()
+        hob = np.zeros(norb * norb, complex)
+        hint = np.zeros(norb ** 4, complex)
+        for _ in range(int(round(0.08 / dt))):
+            orc.lib().orc_splitstep_orb(eq, 0, 1, f, hob, hint, r2, rinv)
+        finals[dt] = f
+    e1 = np.max(np.abs(finals[0.01] - finals[0.0025]))
+    e2 = np.max(np.abs(finals[0.005] - finals[0.0025]))
+    assert e1 / e2 > 4.0  # at least second order (Richardson ratio 5 for dt^2)
+
+
+def _ground_state(orc, npar, norb, g, steps=1500, dt=0.01, ng=256, L=16.0):
+    x = np.linspace(-L / 2, L / 2, ng + 1)
+    dx = x[1] - x[0]
+    eq = orc.make_orbeq(norb, x.size, x[0], x[-1], dt, g, 0.5 * x ** 2, imagtime=True)
+    orb = np.ascontiguousarray(inputs.hermite_gauss(norb, x, seed=1, noise=1e-3).ravel())
+    orc.lib().orc_orthonormalize(x.size, dx, norb, orb)
+    S = orc.Space(npar, norb)
+    coef = inputs.coefficients(S.dim, 1)
+    r1, r2, rinv = S.sync_density(coef)
+    r1 = np.ascontiguousarray(r1.ravel())
+    rinv = np.ascontiguousarray(rinv.ravel())
+    hob = np.zeros(norb * norb, complex)
+    hint = np.zeros(norb ** 4, complex)
+    orc.lib().orc_orbital_hob(eq, orb, hob)
+    orc.lib().orc_orbital_hint(eq, orb, hint)
+    lvec = np.zeros(5 * S.dim, complex)
+    for _ in range(steps):
+        orc.lib().orc_propagate_step(S._p, eq, 1, 1, 5, orb, coef, hob, hint, r1, r2, rinv, lvec)
+    e = orc.total_energy(norb, r1, r2, hob, hint).real / npar
+    occ = np.linalg.eigvalsh(r1.reshape(norb, norb)) / npar
+    return e, occ[-1]
+
+
+@pytest.mark.parametrize("npar,norb,g,energy,condensate", [(4, 3, 1.0, 0.9685859, 0.97),
+                                                            (5, 5, 1.0, 1.0889904, 0.96)])
+def test_ground_state_energy_known_answer(orc, npar, norb, g, energy, condensate):
+    """benchmark.dat:20 and :23 (harmonic trap): energy per particle and
+    condensate fraction, which the reference states agree with ultracold.org
+    to at least 4 digits in energy (benchmark.dat:5-6; grid and dt are not
+    recorded there).  Imaginary-time propagation with the FFT linear step,
+    Lanczos m=5, dt=0.01, 257 points over [-8, 8], t = 15."""
+    e, occ = _ground_state(orc, npar, norb, g)
+    assert abs(e - energy) < 3e-4, e
+    assert abs(occ - condensate) < 0.01
