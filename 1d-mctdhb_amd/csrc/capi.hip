@@ -15,6 +15,7 @@
 #include "density.hpp"
 #include "hamiltonian.hpp"
 #include "mctdhb_hip.h"
+#include "orbital.hpp"
 #include "sil_api.hpp"
 
 namespace mctdhb {
@@ -220,6 +221,25 @@ extern "C" int mctdhb_lanczos_dev(mctdhb_sil* s, const void* coef, const void* h
                        (hipStream_t) stream);
 }
 
+extern "C" int mctdhb_lowest_state_dev(mctdhb_sil* s, void* coef, const void* hob, const void* hint,
+                                       double* energy, void* stream) {
+    if (!s || !coef || !hob || !hint) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    int rc = sil_ground(s->L, (double2*) coef, (const double2*) hob, (const double2*) hint, (hipStream_t) stream);
+    if (rc || !energy) return rc;
+    SilState h;
+    rc = sil_read_state(s->L, &h, (hipStream_t) stream);
+    if (rc) return rc;
+    if (h.eig_fail) {
+        set_error("tridiagonal eigensolver did not converge");
+        return kNumerical;
+    }
+    *energy = h.eground;
+    return kOk;
+}
+
 extern "C" int mctdhb_sil_scalars(const mctdhb_sil* s, int* iter_done, double* diag, double* offd, void* stream) {
     if (!s) {
         set_error("null argument");
@@ -241,6 +261,138 @@ extern "C" int mctdhb_sil_scalars(const mctdhb_sil* s, int* iter_done, double* d
         if (offd) offd[k] = h.beta[k];
     }
     return kOk;
+}
+
+// ------------------------------------------------------------- orbitals
+
+struct mctdhb_orbital {
+    OrbWork* w = nullptr;
+};
+
+extern "C" int mctdhb_orbital_create(int norb, int grid_size, int bounds, double xi, double xf, double tstep,
+                                     double g, double d2coef, double d1_re, double d1_im, int imagtime,
+                                     const double* pot, mctdhb_orbital** out) {
+    if (!out || !pot) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    *out = nullptr;
+    if (bounds != kZeroBounds && bounds != kPeriodicBounds) {
+        set_error("bounds must be ZERO_BOUNDS (1000) or PERIODIC_BOUNDS (1001)");
+        return kInvalidArgument;
+    }
+    OrbEq eq;
+    eq.norb = norb;
+    eq.grid_size = grid_size;
+    eq.bounds = bounds;
+    eq.dx = (xf - xi) / (grid_size - 1);
+    eq.g = g;
+    eq.d2coef = d2coef;
+    eq.tstep = tstep;
+    eq.d1coef = make_double2(d1_re, d1_im);
+    // reference types_definition.c:87-95
+    eq.prop_dt = imagtime ? make_double2(0, -tstep) : make_double2(tstep, 0);
+    eq.time_fac = imagtime ? make_double2(0, -1) : make_double2(1, 0);
+    OrbWork* w = nullptr;
+    int rc = orbwork_create(eq, imagtime, pot, &w);
+    if (rc) return rc;
+    mctdhb_orbital* o = new mctdhb_orbital();
+    o->w = w;
+    *out = o;
+    return kOk;
+}
+
+extern "C" int mctdhb_orbital_destroy(mctdhb_orbital* o) {
+    if (!o) return kOk;
+    hipDeviceSynchronize();
+    orbwork_destroy(o->w);
+    delete o;
+    return kOk;
+}
+
+extern "C" int mctdhb_orbital_set_potential(mctdhb_orbital* o, const double* pot) {
+    if (!o || !pot) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    return orbwork_set_potential(o->w, pot);
+}
+
+#define ORB_CHECK(...)                      \
+    do {                                    \
+        if (!o || __VA_ARGS__) {            \
+            set_error("null argument");     \
+            return kInvalidArgument;        \
+        }                                   \
+    } while (0)
+
+extern "C" int mctdhb_linear_fft_dev(mctdhb_orbital* o, void* orb, void* stream) {
+    ORB_CHECK(!orb);
+    return linear_fft(o->w, (double2*) orb, (hipStream_t) stream);
+}
+
+extern "C" int mctdhb_linear_cn_dev(mctdhb_orbital* o, void* orb, void* stream) {
+    ORB_CHECK(!orb);
+    return linear_cn(o->w, (double2*) orb, (hipStream_t) stream);
+}
+
+extern "C" int mctdhb_orbital_matrices_dev(mctdhb_orbital* o, const void* orb, void* hob, void* hint, void* stream) {
+    ORB_CHECK(!orb || !hob || !hint);
+    return orbital_matrices(o->w, (const double2*) orb, (double2*) hob, (double2*) hint, (hipStream_t) stream);
+}
+
+extern "C" int mctdhb_nonlinear_rhs_dev(mctdhb_orbital* o, const void* orb, const void* hob, const void* hint,
+                                        const void* rho2, const void* rinv, void* der, void* stream) {
+    ORB_CHECK(!orb || !hob || !hint || !rho2 || !rinv || !der);
+    return nonlinear_rhs(o->w, (const double2*) orb, (const double2*) hob, (const double2*) hint,
+                         (const double2*) rho2, (const double2*) rinv, (double2*) der, (hipStream_t) stream);
+}
+
+extern "C" int mctdhb_orthonormalize_dev(mctdhb_orbital* o, void* orb, void* stream) {
+    ORB_CHECK(!orb);
+    return orthonormalize(o->w, (double2*) orb, (hipStream_t) stream);
+}
+
+extern "C" int mctdhb_splitstep_dev(mctdhb_orbital* o, int use_fft, void* orb, void* hob, void* hint,
+                                    const void* rho2, const void* rinv, void* stream) {
+    ORB_CHECK(!orb || !hob || !hint || !rho2 || !rinv);
+    return splitstep(o->w, use_fft, (double2*) orb, (double2*) hob, (double2*) hint, (const double2*) rho2,
+                     (const double2*) rinv, (hipStream_t) stream);
+}
+
+extern "C" int mctdhb_density_sync_dev(mctdhb_space* sp, const void* coef, void* rho1, void* rho2, void* rinv,
+                                       void* eigvals, double reg_factor, int prec, void* stream) {
+    if (!sp || !coef || !rho1 || !rho2 || !rinv) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    int rc = mctdhb_density_dev(sp, coef, rho1, rho2, prec, kIndexOnTheFly, stream);
+    if (rc) return rc;
+    return density_sync_small(sp->S->norb, (double2*) rho1, (double2*) rinv, (double*) eigvals, reg_factor,
+                              (hipStream_t) stream);
+}
+
+extern "C" int mctdhb_propagate_step_dev(mctdhb_space* sp, mctdhb_sil* s, mctdhb_orbital* o, int use_fft,
+                                         void* orb, void* coef, void* hob, void* hint, void* rho1, void* rho2,
+                                         void* rinv, void* stream) {
+    if (!sp || !s || !o || !orb || !coef || !hob || !hint || !rho1 || !rho2 || !rinv) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    hipStream_t st = (hipStream_t) stream;
+    OrbWork* w = o->w;
+    int rc = splitstep(w, use_fft, (double2*) orb, (double2*) hob, (double2*) hint, (const double2*) rho2,
+                       (const double2*) rinv, st);
+    if (rc) return rc;
+    // driver.c:179-186: periodic bounds, then h and g from the new orbitals
+    if (w->eq.bounds == kPeriodicBounds && (rc = periodic_bounds(w, (double2*) orb, st))) return rc;
+    rc = orbital_matrices(w, (const double2*) orb, (double2*) hob, (double2*) hint, st);
+    if (rc) return rc;
+    if (sp->S->norb == 1) return kOk;
+    const double2 dt = w->eq.prop_dt;
+    rc = sil_step(s->L, (double2*) coef, (const double2*) hob, (const double2*) hint, dt.x, dt.y, w->imagtime, st);
+    if (rc) return rc;
+    return mctdhb_density_sync_dev(sp, coef, rho1, rho2, rinv, nullptr, 1e-9, MCTDHB_FP64, stream);
 }
 
 // ------------------------------------------------------------------ layer 1
@@ -641,6 +793,25 @@ extern "C" int lanczos(MultiConfiguration mc, Cmatrix hob, Carray hint, int lm, 
         }
     }
     return done;
+}
+
+extern "C" double lowest_state_lanczos(uint16_t niter, MultiConfiguration mc, Cmatrix Ho, Carray Hint, Carray C) {
+    mctdhb_space* sp = ensure_dev(mc);
+    const size_t n = sp->S->dim;
+    mctdhb_sil* s = host_sil(sp, niter);
+    stage_alloc(&sp->d_coef, n);
+    stage_hob_hint(sp, Ho, Hint);
+    h2d(sp->d_coef, C, sizeof(dcomplex) * n);
+    double e = 0;
+    if (mctdhb_lowest_state_dev(s, sp->d_coef, sp->d_hob, sp->d_hint, &e, nullptr)) {
+        printf("\n\nERROR IN DIAGONALIZATION\n\n");
+        exit(EXIT_FAILURE);
+    }
+    int done = 0;
+    if (mctdhb_sil_scalars(s, &done, nullptr, nullptr, nullptr)) fail_stop("lowest_state_lanczos");
+    if (done < niter) printf("\n\nWARNING : lanczos iterations exit before expected with %d iterations\n\n", done);
+    d2h(C, sp->d_coef, sizeof(dcomplex) * n);
+    return e;
 }
 
 extern "C" void propagate_coef_sil(MCTDHBDataStruct mctdhb, Carray cnext) {

@@ -135,8 +135,9 @@ template <typename Zw> struct EpiLanczos {
             }
             a.out[i] = o;
         }
-        const int nslot = lanczos_hc_slots(it);
-        double* part = a.part + (size_t) blockIdx.x * nslot;
+        // slot-major partials: slot t of block b at part[t * gridDim.x + b]
+        double* part = a.part + blockIdx.x;
+        const size_t ps = gridDim.x;
         for (int k = 0; k <= it; k++) {
             double v[2] = {0, 0};
             if (valid) {
@@ -147,8 +148,8 @@ template <typename Zw> struct EpiLanczos {
             }
             block_sum<2>(v, lds);
             if (threadIdx.x == 0) {
-                part[2 * k] = v[0];
-                part[2 * k + 1] = v[1];
+                part[(2 * k) * ps] = v[0];
+                part[(2 * k + 1) * ps] = v[1];
             }
         }
         double v1[1] = {0};
@@ -158,8 +159,8 @@ template <typename Zw> struct EpiLanczos {
         }
         block_sum<1>(v1, lds);
         if (threadIdx.x == 0) {
-            part[2 * (it + 1)] = v1[0];
-            part[2 * (it + 1) + 1] = 0;
+            part[(2 * (it + 1)) * ps] = v1[0];
+            part[(2 * (it + 1) + 1) * ps] = 0;
         }
     }
 };

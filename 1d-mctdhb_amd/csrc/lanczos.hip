@@ -41,19 +41,22 @@ __global__ void k_sil_init(SilState* st, int lm) {
     st->s[0] = 1.0;
     st->maxbeta = 0;
     st->nrm2 = 0;
+    st->eground = 0;
     st->iter_done = lm;
     st->stop = 0;
     st->eig_fail = 0;
 }
 
-// Sum per-block partials: tot[t] = sum_b part[b * nslot + t], one block per slot.
+// Sum per-block partials (slot-major): tot[t] = sum_b part[t * nblocks + b],
+// one block per slot.
 __global__ __launch_bounds__(256) void k_reduce_slots(const double* __restrict__ part, int nblocks, int nslot,
                                                       double* __restrict__ tot, const SilState* st) {
     __shared__ double red[4];
     if (st && st->stop) return;
     const int t = blockIdx.x;
     double v[1] = {0};
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) v[0] += part[(size_t) b * nslot + t];
+    const double* p = part + (size_t) t * nblocks;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) v[0] += p[b];
     block_sum<1>(v, red);
     if (threadIdx.x == 0) tot[t] = v[0];
 }
@@ -149,9 +152,12 @@ __device__ int tridiag_eig_dev(int n, double* d, double* e, double* z) {
 }
 
 __global__ void k_sil_tridiag(SilState* st, double dt_re, double dt_im) {
+    // LDS rather than per-thread arrays: a lone thread's dynamically
+    // indexed arrays would live in scratch memory
+    __shared__ double d[kMaxLanczos + 1], e[kMaxLanczos + 1], z[(kMaxLanczos + 1) * (kMaxLanczos + 1)];
+    __shared__ double2 aux[kMaxLanczos + 1];
     if (threadIdx.x != 0) return;
     const int n = st->iter_done;
-    double d[kMaxLanczos + 1], e[kMaxLanczos + 1], z[(kMaxLanczos + 1) * (kMaxLanczos + 1)];
     for (int k = 0; k < n; k++) {
         d[k] = st->alpha[k].x;
         e[k] = st->beta[k];
@@ -162,7 +168,6 @@ __global__ void k_sil_tridiag(SilState* st, double dt_re, double dt_im) {
     }
     // y = Z exp(-i D prop_dt) Z^T e_1; exp(-i d dt) for complex dt = a + ib
     // is exp(d b) (cos(d a) - i sin(d a)).
-    double2 aux[kMaxLanczos + 1];
     for (int k = 0; k < n; k++) {
         const double mag = exp(d[k] * dt_im), ph = -d[k] * dt_re;
         aux[k] = make_double2(z[0 * n + k] * mag * cos(ph), z[0 * n + k] * mag * sin(ph));
@@ -172,6 +177,33 @@ __global__ void k_sil_tridiag(SilState* st, double dt_re, double dt_im) {
         for (int j = 0; j < n; j++) y = y + cscale(aux[j], z[k * n + j]);
         st->y[k] = cscale(y, st->s[k]);
     }
+}
+
+// Ground-state mode (reference lowest_state_lanczos, multiconfig_lanczos.c:
+// 131-169): eigenvector of the smallest Ritz value (the first index found
+// when several tie, with the reference's 1e15 sentinel).
+__global__ void k_sil_ground(SilState* st) {
+    __shared__ double d[kMaxLanczos + 1], e[kMaxLanczos + 1], z[(kMaxLanczos + 1) * (kMaxLanczos + 1)];
+    if (threadIdx.x != 0) return;
+    const int n = st->iter_done;
+    for (int k = 0; k < n; k++) {
+        d[k] = st->alpha[k].x;
+        e[k] = st->beta[k];
+    }
+    if (tridiag_eig_dev(n, d, e, z)) {
+        st->eig_fail = 1;
+        return;
+    }
+    double lowest = 1e15;
+    int j = 0;
+    for (int k = 0; k < n; k++) {
+        if (lowest > d[k]) {
+            lowest = d[k];
+            j = k;
+        }
+    }
+    st->eground = lowest;
+    for (int k = 0; k < n; k++) st->y[k] = make_double2(z[k * n + j] * st->s[k], 0.0);
 }
 
 // ---------------------------------------------------------- streaming
@@ -224,7 +256,7 @@ __global__ __launch_bounds__(kStreamBlock) void k_sil_reorth(const double2* __re
     block_sum<2 * NK + 1>(acc, &red[0][0]);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int t = 0; t < 2 * NK + 1; t++) part[(size_t) blockIdx.x * nslot + t] = acc[t];
+        for (int t = 0; t < 2 * NK + 1; t++) part[(size_t) t * gridDim.x + blockIdx.x] = acc[t];
     }
 }
 
@@ -418,6 +450,20 @@ int sil_step(Sil* L, double2* C, const double2* hob, const double2* hint, double
         k_reduce_slots<<<1, 256, 0, st>>>(L->part, kStreamGrid, 1, L->tot + 48, nullptr);
         k_scale_by_inv_norm<<<kStreamGrid, kStreamBlock, 0, st>>>(C, L->S->dim, L->tot + 48);
     }
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
+int sil_ground(Sil* L, double2* C, const double2* hob, const double2* hint, hipStream_t st) {
+    int rc = sil_run_lanczos(L, C, hob, hint, st);
+    if (rc) return rc;
+    k_sil_ground<<<1, 64, 0, st>>>(L->d_st);
+    WPtrs wp{};
+    for (int k = 0; k < L->lm; k++) wp.w[k] = L->w[k];
+    if (L->prec == kPrecFP64)
+        k_sil_recombine<double2><<<kStreamGrid, kStreamBlock, 0, st>>>(C, wp, L->S->dim, L->d_st, L->part);
+    else
+        k_sil_recombine<float2><<<kStreamGrid, kStreamBlock, 0, st>>>(C, wp, L->S->dim, L->d_st, L->part);
     MCTDHB_HIP_CHECK(hipGetLastError());
     return kOk;
 }

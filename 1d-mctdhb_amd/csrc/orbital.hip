@@ -1,0 +1,785 @@
+// orbital.hip - the orbital split-step on the device.
+//
+// Replaces reference src/integrator/orbital_integration.c:351-434
+// (propagate_splitstep_orb) and what it calls:
+//   split_linear_orbitals.c:22-80, :288-402  (CN and FFT linear half steps)
+//   tridiagonal_solver.c:6-212               (Thomas / Sherman-Morrison)
+//   orbital_matrices.c:21-287                (h_kl, g_ksql by Simpson quadrature)
+//   calculus.c:8-39, :116-191, :287-305      (Simpson, Gram-Schmidt, FD derivative)
+//   split_nonlinear_orbitals.c:62-151        (nonlinear right-hand side)
+//   lapack_interface.c:10-163, synchronize.c:41-56 (M x M eigen/regularise/invert)
+// The odelib Runge-Kutta the reference links is absent; the nonlinear step
+// uses the classical RK4 tableau.
+#include <cmath>
+#include <vector>
+
+#include "orbital.hpp"
+
+namespace mctdhb {
+
+// ------------------------------------------------------------- helpers
+
+__device__ __forceinline__ double2 dconj2(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
+    const double d = b.x * b.x + b.y * b.y;
+    return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+
+// Weight of point p in the reference's Simpson rule on n points
+// (calculus.c:8-39: composite 1/3 rule, 3/8 rule on the last 4 points when
+// n is even).
+__device__ __forceinline__ double simpson_w(int p, int n, double h) {
+    if (n % 2 == 1) {
+        if (p == 0 || p == n - 1) return h / 3;
+        return (p % 2) ? 4 * h / 3 : 2 * h / 3;
+    }
+    const int e = n - 4;  // end of the composite part
+    double w = 0;
+    if (e > 0 && p <= e) w = (p == 0 || p == e) ? h / 3 : ((p % 2) ? 4 * h / 3 : 2 * h / 3);
+    if (p == e) w += 3 * h / 8;
+    else if (p == n - 3 || p == n - 2) w += 9 * h / 8;
+    else if (p == n - 1) w += 3 * h / 8;
+    return w;
+}
+
+// 4th-order FD derivative at point p with the reference's periodic wrap
+// (calculus.c:287-305; the last point repeats the first).
+__device__ __forceinline__ double2 dxfd_at(const double2* f, int n, int p, double r) {
+    int a, b, c, d;  // (f[a] - f[b] + 8 (f[c] - f[d])) r
+    if (p == 0 || p == n - 1) { a = n - 3; b = 2; c = 1; d = n - 2; }
+    else if (p == 1) { a = n - 2; b = 3; c = 2; d = 0; }
+    else if (p == n - 3) { a = n - 5; b = 0; c = n - 2; d = n - 4; }
+    else if (p == n - 2) { a = n - 4; b = 1; c = 0; d = n - 3; }
+    else { a = p - 2; b = p + 2; c = p + 1; d = p - 1; }
+    const double2 fa = f[a], fb = f[b], fc = f[c], fd = f[d];
+    return make_double2((fa.x - fb.x + 8 * (fc.x - fd.x)) * r, (fa.y - fb.y + 8 * (fc.y - fd.y)) * r);
+}
+
+template <int NV> __device__ __forceinline__ void block_sum_bcast(double (&v)[NV], double* lds) {
+    block_sum<NV>(v, lds);
+    if (threadIdx.x == 0)
+        for (int t = 0; t < NV; t++) lds[t] = v[t];
+    __syncthreads();
+    for (int t = 0; t < NV; t++) v[t] = lds[t];
+    __syncthreads();
+}
+
+// ------------------------------------------------------------- FFT step
+
+__device__ __forceinline__ int bitrev(int i, int bits) { return __brev((unsigned) i) >> (32 - bits); }
+
+__device__ void fft_inplace(double2* X, int n, int log2n, const double2* __restrict__ tw, bool inverse) {
+    for (int s = 1; s <= log2n; s++) {
+        const int half = 1 << (s - 1), step = n >> s;
+        for (int j = threadIdx.x; j < n / 2; j += blockDim.x) {
+            const int grp = j >> (s - 1), pos = j & (half - 1);
+            const int i1 = (grp << s) + pos, i2 = i1 + half;
+            double2 w = tw[pos * step];
+            if (inverse) w.y = -w.y;
+            const double2 b = cmul(X[i2], w), a = X[i1];
+            X[i1] = a + b;
+            X[i2] = a - b;
+        }
+        __syncthreads();
+    }
+}
+
+// One block per orbital: forward DFT, multiply by exp(-i dt/2 T(k)), inverse
+// DFT (both scaled 1/sqrt(n) as the reference's DFTI descriptor), last point
+// = first (split_linear_orbitals.c:375-402).
+__global__ __launch_bounds__(256) void k_fft_linear(double2* __restrict__ orb, int grid, int n, int log2n,
+                                                    const double2* __restrict__ tw,
+                                                    const double2* __restrict__ fexp) {
+    extern __shared__ double2 X[];
+    double2* o = orb + (size_t) blockIdx.x * grid;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) X[bitrev(i, log2n)] = o[i];
+    __syncthreads();
+    fft_inplace(X, n, log2n, tw, false);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) X[i] = cmul(fexp[i], X[i]);
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int r = bitrev(i, log2n);
+        if (i < r) {
+            const double2 t = X[i];
+            X[i] = X[r];
+            X[r] = t;
+        }
+    }
+    __syncthreads();
+    fft_inplace(X, n, log2n, tw, true);
+    const double s = 1.0 / (double) n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) o[i] = cscale(X[i], s);
+    __syncthreads();
+    if (threadIdx.x == 0) o[n] = cscale(X[0], s);
+}
+
+// Lengths that are not powers of two: direct transforms, twiddles
+// exp(-2 pi i t / n) for t < n (tw), one block per orbital, two passes.
+__global__ __launch_bounds__(256) void k_dft_forward_mul(const double2* __restrict__ orb, int grid, int n,
+                                                         const double2* __restrict__ tw,
+                                                         const double2* __restrict__ fexp,
+                                                         double2* __restrict__ tmp) {
+    extern __shared__ double2 X[];
+    const double2* o = orb + (size_t) blockIdx.x * grid;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) X[i] = o[i];
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        double2 acc = make_double2(0, 0);
+        int t = 0;
+        for (int j = 0; j < n; j++) {
+            acc = cfma(X[j], tw[t], acc);
+            t += k;
+            if (t >= n) t -= n;
+        }
+        tmp[(size_t) blockIdx.x * n + k] = cmul(fexp[k], acc);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dft_inverse(double2* __restrict__ orb, int grid, int n,
+                                                     const double2* __restrict__ tw,
+                                                     const double2* __restrict__ tmp) {
+    extern __shared__ double2 X[];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) X[i] = tmp[(size_t) blockIdx.x * n + i];
+    __syncthreads();
+    double2* o = orb + (size_t) blockIdx.x * grid;
+    const double s = 1.0 / (double) n;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        double2 acc = make_double2(0, 0);
+        int t = 0;
+        for (int j = 0; j < n; j++) {
+            acc = cfma(X[j], dconj2(tw[t]), acc);
+            t += k;
+            if (t >= n) t -= n;
+        }
+        o[k] = cscale(acc, s);
+        if (k == 0) o[n] = cscale(acc, s);
+    }
+}
+
+int linear_fft(OrbWork* w, double2* orb, hipStream_t st) {
+    const int M = w->eq.norb, grid = w->eq.grid_size, n = w->nfft;
+    if (w->log2n >= 0) {
+        k_fft_linear<<<M, 256, sizeof(double2) * n, st>>>(orb, grid, n, w->log2n, w->d_tw, w->d_fexp);
+    } else {
+        k_dft_forward_mul<<<M, 256, sizeof(double2) * n, st>>>(orb, grid, n, w->d_tw, w->d_fexp, w->d_stage);
+        k_dft_inverse<<<M, 256, sizeof(double2) * n, st>>>(orb, grid, n, w->d_tw, w->d_stage);
+    }
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
+// ------------------------------------------------------ Crank-Nicolson
+
+// Thomas algorithm of tridiagonal_solver.c:6-92, including the reduced
+// system used when mid[0] == 0.  u, l, z: scratch of n entries.
+__device__ void thomas(int n, const double2* up, const double2* lo, const double2* mid, const double2* rhs,
+                       double2* ans, double2* u, double2* l, double2* z) {
+    if (mid[0].x == 0 && mid[0].y == 0) {
+        const double2 r0u = cdiv(rhs[0], up[0]);
+        const double2 r1 = rhs[1] - cmul(mid[1], r0u);
+        const double2 r2 = rhs[2] - cmul(lo[1], r0u);
+        u[1] = lo[0];
+        z[1] = r1;
+        l[1] = make_double2(0, 0);
+        u[2] = mid[2] - cmul(l[1], up[1]);
+        z[2] = r2 - cmul(l[1], z[1]);
+        for (int i = 2; i < n - 1; i++) {
+            l[i] = cdiv(lo[i], u[i]);
+            u[i + 1] = mid[i + 1] - cmul(l[i], up[i]);
+            z[i + 1] = rhs[i + 1] - cmul(l[i], z[i]);
+        }
+        ans[n - 1] = cdiv(z[n - 1], u[n - 1]);
+        for (int i = 2; i <= n - 1; i++) {
+            const int k = n - i;
+            ans[k] = cdiv(z[k] - cmul(up[k], ans[k + 1]), u[k]);
+        }
+        ans[0] = ans[1];
+        ans[1] = r0u;
+        return;
+    }
+    u[0] = mid[0];
+    z[0] = rhs[0];
+    for (int i = 0; i < n - 1; i++) {
+        l[i] = cdiv(lo[i], u[i]);
+        u[i + 1] = mid[i + 1] - cmul(l[i], up[i]);
+        z[i + 1] = rhs[i + 1] - cmul(l[i], z[i]);
+    }
+    ans[n - 1] = cdiv(z[n - 1], u[n - 1]);
+    for (int i = 2; i <= n; i++) {
+        const int k = n - i;
+        ans[k] = cdiv(z[k] - cmul(up[k], ans[k + 1]), u[k]);
+    }
+}
+
+// One thread per orbital: CN right-hand side (split_linear_orbitals.c:288-333)
+// and the (cyclic) tridiagonal solve (:335-373).  scratch: 8 * grid per orbital.
+__global__ void k_cn_sequential(double2* __restrict__ orb, int M, int grid, int bounds,
+                                const double2* __restrict__ cn, double2 rhs_up, double2 rhs_lo, double2 rhs_mid,
+                                double2* __restrict__ scratch) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= M) return;
+    double2* f = orb + (size_t) k * grid;
+    double2* S = scratch + (size_t) k * 8 * grid;
+    double2 *rhs = S, *x = S + grid, *w = S + 2 * grid, *U = S + 3 * grid;
+    double2 *u = S + 4 * grid, *l = S + 5 * grid, *z = S + 6 * grid, *mid = S + 7 * grid;
+    const double2 *up = cn, *lo = cn + grid, *mid0 = cn + 2 * grid;
+    const int n = grid;
+    // right-hand side
+    if (bounds == kZeroBounds) rhs[0] = cmul(rhs_mid, f[0]) + cmul(rhs_up, f[1]);
+    else rhs[0] = cmul(rhs_mid, f[0]) + cmul(rhs_up, f[1]) + cmul(rhs_lo, f[n - 2]);
+    for (int i = 1; i < n - 1; i++) rhs[i] = cmul(rhs_mid, f[i]) + cmul(rhs_up, f[i + 1]) + cmul(rhs_lo, f[i - 1]);
+    if (bounds == kZeroBounds) rhs[n - 1] = cmul(rhs_mid, f[n - 1]) + cmul(rhs_lo, f[n - 2]);
+    else {
+        rhs[n - 2] = cmul(rhs_mid, f[n - 2]) + cmul(rhs_up, f[0]) + cmul(rhs_lo, f[n - 3]);
+        rhs[n - 1] = rhs[0];
+    }
+    if (bounds == kZeroBounds) {
+        thomas(n, up, lo, mid0, rhs, f, u, l, z);
+        return;
+    }
+    // cyclic system of size m = grid - 1 by Sherman-Morrison
+    // (tridiagonal_solver.c:161-212): upper[m-1] couples row 0 to x_{m-1},
+    // lower[m-1] couples row m-1 to x_0.
+    const int m = n - 1;
+    for (int i = 0; i < m; i++) mid[i] = mid0[i];
+    double2 factor;
+    if (mid[0].x == 0 && mid[0].y == 0) {
+        factor = up[0];
+        mid[0] = make_double2(-factor.x, -factor.y);
+    } else {
+        factor = mid[0];
+        mid[0] = make_double2(0, 0);
+    }
+    for (int i = 0; i < m; i++) U[i] = make_double2(0, 0);
+    U[0] = factor;
+    U[m - 1] = lo[m - 1];
+    const double2 V0 = make_double2(1, 0), Vm = cdiv(up[m - 1], factor);
+    mid[m - 1] = mid[m - 1] - cdiv(cmul(up[m - 1], lo[m - 1]), factor);
+    thomas(m, up, lo, mid, rhs, x, u, l, z);
+    thomas(m, up, lo, mid, U, w, u, l, z);
+    const double2 vx = cmul(V0, x[0]) + cmul(Vm, x[m - 1]);
+    const double2 vw = cmul(V0, w[0]) + cmul(Vm, w[m - 1]);
+    const double2 fac = cdiv(vx, make_double2(1.0 + vw.x, vw.y));
+    for (int i = 0; i < m; i++) f[i] = x[i] - cmul(fac, w[i]);
+    f[m] = f[0];
+}
+
+int linear_cn(OrbWork* w, double2* orb, hipStream_t st) {
+    const OrbEq& q = w->eq;
+    // dt = prop_dt / 2, rhs coefficients of cn_rhs
+    const double2 dt = cscale(q.prop_dt, 0.5);
+    const double a2 = q.d2coef, dx = q.dx;
+    const double2 a2dt = cscale(dt, a2 / dx / dx);
+    const double2 a1dt = cscale(cmul(q.d1coef, dt), 1.0 / dx / 4);
+    const double2 up = cscale(a2dt, 0.5) + a1dt, lo = cscale(a2dt, 0.5) - a1dt;
+    const double2 mid = make_double2(-a2dt.x, 1.0 - a2dt.y);
+    double2* scratch = nullptr;
+    MCTDHB_HIP_CHECK(hipMallocAsync((void**) &scratch, sizeof(double2) * 8 * q.grid_size * q.norb, st));
+    k_cn_sequential<<<div_up(q.norb, 64), 64, 0, st>>>(orb, q.norb, q.grid_size, q.bounds, w->d_cn, up, lo, mid,
+                                                       scratch);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    MCTDHB_HIP_CHECK(hipFreeAsync(scratch, st));
+    return kOk;
+}
+
+// --------------------------------------------------- orbital matrices
+
+// hob[i][j] for i <= j (one block each), conj fill, real diagonal
+// (orbital_matrices.c:21-70).
+__global__ __launch_bounds__(256) void k_hob(const double2* __restrict__ orb, int M, int n, double dx, double a2,
+                                             double2 a1, const double* __restrict__ V, double2* __restrict__ hob) {
+    __shared__ double red[8];
+    const int i = blockIdx.x / M, j = blockIdx.x % M;
+    if (j < i) return;
+    const double2 *fi = orb + (size_t) i * n, *fj = orb + (size_t) j * n;
+    const double r = 1.0 / (12 * dx);
+    double v[2] = {0, 0};
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+        const double2 di = dxfd_at(fi, n, p, r), dj = dxfd_at(fj, n, p, r);
+        double2 part = cscale(cconjmul(di, dj), -a2);
+        part = part + cmul(a1, cconjmul(fi[p], dj));
+        part = part + cscale(cconjmul(fi[p], fj[p]), V[p]);
+        const double wp = simpson_w(p, n, dx);
+        v[0] += wp * part.x;
+        v[1] += wp * part.y;
+    }
+    block_sum<2>(v, red);
+    if (threadIdx.x == 0) {
+        if (i == j) hob[i * M + i] = make_double2(v[0], 0);
+        else {
+            hob[i * M + j] = make_double2(v[0], v[1]);
+            hob[j * M + i] = make_double2(v[0], -v[1]);
+        }
+    }
+}
+
+// hint[a + bM + cM^2 + dM^3] = g int conj(phi_a phi_b) phi_c phi_d, one block
+// per element (orbital_matrices.c:72-287 evaluates the same integrals).
+__global__ __launch_bounds__(256) void k_hint(const double2* __restrict__ orb, int M, int n, double dx, double g,
+                                              double2* __restrict__ hint) {
+    __shared__ double red[8];
+    const int e = blockIdx.x;
+    const int a = e % M, b = (e / M) % M, c = (e / (M * M)) % M, d = e / (M * M * M);
+    const double2 *fa = orb + (size_t) a * n, *fb = orb + (size_t) b * n;
+    const double2 *fc = orb + (size_t) c * n, *fd = orb + (size_t) d * n;
+    double v[2] = {0, 0};
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+        const double2 t = cmul(dconj2(cmul(fa[p], fb[p])), cmul(fc[p], fd[p]));
+        const double wp = simpson_w(p, n, dx);
+        v[0] += wp * t.x;
+        v[1] += wp * t.y;
+    }
+    block_sum<2>(v, red);
+    if (threadIdx.x == 0) hint[e] = make_double2(g * v[0], g * v[1]);
+}
+
+int orbital_matrices(OrbWork* w, const double2* orb, double2* hob, double2* hint, hipStream_t st) {
+    const OrbEq& q = w->eq;
+    const int M = q.norb;
+    k_hob<<<M * M, 256, 0, st>>>(orb, M, q.grid_size, q.dx, q.d2coef, q.d1coef, w->d_pot, hob);
+    k_hint<<<M * M * M * M, 256, 0, st>>>(orb, M, q.grid_size, q.dx, q.g, hint);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
+// ------------------------------------------------------ nonlinear part
+
+// der[k][x] = -i time_fac (nonlinear(k, x) + V(x) orb[k][x]),
+// nonlinear(k, x) restating split_nonlinear_orbitals.c:62-151.
+__global__ __launch_bounds__(256) void k_nonlinear(const double2* __restrict__ orb, int M, int n, double g,
+                                                   double2 tf, const double* __restrict__ V,
+                                                   const double2* __restrict__ hob,
+                                                   const double2* __restrict__ hint,
+                                                   const double2* __restrict__ rho,
+                                                   const double2* __restrict__ rinv, double2* __restrict__ der) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= M * n) return;
+    const int k = idx / n, x = idx % n;
+    const int mm = M * M, mmm = mm * M;
+    double2 f[32];
+    for (int a = 0; a < M; a++) f[a] = orb[(size_t) a * n + x];
+    double2 red = make_double2(0, 0);
+    for (int a = 0; a < M; a++) {
+        red = red - cmul(hob[a * M + k], f[a]);
+        for (int q = 0; q < M; q++) {
+            double2 rm = cmul(rinv[k * M + a], rho[a + M * a + mm * q + mmm * q]);
+            red = red + cscale(cmul(cmul(cmul(rm, dconj2(f[a])), f[q]), f[q]), g);
+            for (int j = 0; j < M; j++) red = red - cmul(cmul(rm, f[j]), hint[j + a * M + q * mm + q * mmm]);
+            for (int l = q + 1; l < M; l++) {
+                rm = cscale(cmul(rinv[k * M + a], rho[a + M * a + mm * q + mmm * l]), 2.0);
+                red = red + cscale(cmul(cmul(cmul(rm, dconj2(f[a])), f[l]), f[q]), g);
+                for (int j = 0; j < M; j++) red = red - cmul(cmul(rm, f[j]), hint[j + a * M + l * mm + q * mmm]);
+            }
+        }
+        for (int s = a + 1; s < M; s++) {
+            for (int q = 0; q < M; q++) {
+                double2 rm = cmul(rinv[k * M + a], rho[a + M * s + mm * q + mmm * q]);
+                double2 sw = cmul(rinv[k * M + s], rho[a + M * s + mm * q + mmm * q]);
+                double2 t = cmul(rm, dconj2(f[s])) + cmul(sw, dconj2(f[a]));
+                red = red + cscale(cmul(cmul(t, f[q]), f[q]), g);
+                for (int j = 0; j < M; j++) {
+                    red = red - cmul(cmul(rm, f[j]), hint[j + s * M + q * mm + q * mmm]);
+                    red = red - cmul(cmul(sw, f[j]), hint[j + a * M + q * mm + q * mmm]);
+                }
+                for (int l = q + 1; l < M; l++) {
+                    rm = cscale(cmul(rinv[k * M + a], rho[a + M * s + mm * q + mmm * l]), 2.0);
+                    sw = cscale(cmul(rinv[k * M + s], rho[a + M * s + mm * q + mmm * l]), 2.0);
+                    t = cmul(rm, dconj2(f[s])) + cmul(sw, dconj2(f[a]));
+                    red = red + cscale(cmul(cmul(t, f[l]), f[q]), g);
+                    for (int j = 0; j < M; j++) {
+                        red = red - cmul(cmul(rm, f[j]), hint[j + s * M + l * mm + q * mmm]);
+                        red = red - cmul(cmul(sw, f[j]), hint[j + a * M + l * mm + q * mmm]);
+                    }
+                }
+            }
+        }
+    }
+    const double2 total = red + cscale(f[k], V[x]);
+    // -i * tf * total
+    const double2 mi_tf = make_double2(tf.y, -tf.x);
+    der[idx] = cmul(mi_tf, total);
+}
+
+int nonlinear_rhs(OrbWork* w, const double2* orb, const double2* hob, const double2* hint, const double2* rho2,
+                  const double2* rinv, double2* der, hipStream_t st) {
+    const OrbEq& q = w->eq;
+    const int total = q.norb * q.grid_size;
+    k_nonlinear<<<div_up(total, 256), 256, 0, st>>>(orb, q.norb, q.grid_size, q.g, q.time_fac, w->d_pot, hob, hint,
+                                                    rho2, rinv, der);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
+__global__ void k_axpy_stage(const double2* __restrict__ y0, const double2* __restrict__ k, double a,
+                             double2* __restrict__ out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = y0[i] + cscale(k[i], a);
+}
+
+__global__ void k_rk4_combine(double2* __restrict__ y, const double2* __restrict__ k1, const double2* __restrict__ k2,
+                              const double2* __restrict__ k3, const double2* __restrict__ k4, double dt, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double2 s = k1[i] + cscale(k2[i], 2.0) + cscale(k3[i], 2.0) + k4[i];
+        y[i] = y[i] + cscale(s, dt / 6);
+    }
+}
+
+// --------------------------------------------------------- Gram-Schmidt
+
+// One block: renormalise F_0, then for i >= 1 and j < i apply the
+// reference's point-by-point projection (calculus.c:170-191).  With
+// S_p = <F_j, F_i> evaluated after points < p were updated,
+// S_{p+1} = S_p (1 - w_p |F_j[p]|^2), so S_p = S_0 prod_{q<p} (1 - w_q |F_j[q]|^2):
+// one reduction and one multiplicative scan per (i, j).
+__global__ __launch_bounds__(1024) void k_gram_schmidt(double2* __restrict__ F, int M, int n, double dx) {
+    __shared__ double red[64];
+    __shared__ double scan[1024];
+    const int T = blockDim.x;
+    const int per = (n + T - 1) / T;  // contiguous points per thread
+    const int p0 = threadIdx.x * per;
+    auto renorm = [&](double2* f) {
+        double v[1] = {0};
+        for (int p = threadIdx.x; p < n; p += T) v[0] += simpson_w(p, n, dx) * (f[p].x * f[p].x + f[p].y * f[p].y);
+        block_sum_bcast<1>(v, red);
+        const double s = sqrt(1.0 / v[0]);
+        for (int p = threadIdx.x; p < n; p += T) f[p] = cscale(f[p], s);
+        __syncthreads();
+    };
+    renorm(F);
+    for (int i = 1; i < M; i++) {
+        double2* Fi = F + (size_t) i * n;
+        for (int j = 0; j < i; j++) {
+            const double2* Fj = F + (size_t) j * n;
+            double v[2] = {0, 0};
+            for (int p = threadIdx.x; p < n; p += T) {
+                const double2 t = cconjmul(Fj[p], Fi[p]);
+                const double wp = simpson_w(p, n, dx);
+                v[0] += wp * t.x;
+                v[1] += wp * t.y;
+            }
+            block_sum_bcast<2>(v, red);
+            const double2 S0 = make_double2(v[0], v[1]);
+            // exclusive multiplicative scan of (1 - w_p |F_j[p]|^2)
+            double prod = 1.0;
+            for (int p = p0; p < p0 + per && p < n; p++)
+                prod *= 1.0 - simpson_w(p, n, dx) * (Fj[p].x * Fj[p].x + Fj[p].y * Fj[p].y);
+            scan[threadIdx.x] = prod;
+            __syncthreads();
+            for (int off = 1; off < T; off <<= 1) {
+                const double o = threadIdx.x >= (unsigned) off ? scan[threadIdx.x - off] : 1.0;
+                __syncthreads();
+                scan[threadIdx.x] *= o;
+                __syncthreads();
+            }
+            double run = threadIdx.x > 0 ? scan[threadIdx.x - 1] : 1.0;
+            for (int p = p0; p < p0 + per && p < n; p++) {
+                const double2 coef = cscale(S0, run);
+                Fi[p] = Fi[p] - cmul(coef, Fj[p]);
+                run *= 1.0 - simpson_w(p, n, dx) * (Fj[p].x * Fj[p].x + Fj[p].y * Fj[p].y);
+            }
+            __syncthreads();
+        }
+        renorm(Fi);
+    }
+}
+
+int orthonormalize(OrbWork* w, double2* orb, hipStream_t st) {
+    k_gram_schmidt<<<1, 1024, 0, st>>>(orb, w->eq.norb, w->eq.grid_size, w->eq.dx);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
+__global__ void k_periodic(double2* orb, int M, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < M) orb[(size_t) k * n + n - 1] = orb[(size_t) k * n];
+}
+
+int periodic_bounds(OrbWork* w, double2* orb, hipStream_t st) {
+    k_periodic<<<1, 64, 0, st>>>(orb, w->eq.norb, w->eq.grid_size);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
+int splitstep(OrbWork* w, int use_fft, double2* orb, double2* hob, double2* hint, const double2* rho2,
+              const double2* rinv, hipStream_t st) {
+    const OrbEq& q = w->eq;
+    const int total = q.norb * q.grid_size;
+    int rc = kOk;
+    if (q.bounds == kPeriodicBounds && (rc = periodic_bounds(w, orb, st))) return rc;
+    rc = use_fft ? linear_fft(w, orb, st) : linear_cn(w, orb, st);
+    if (rc) return rc;
+    rc = orbital_matrices(w, orb, hob, hint, st);
+    if (rc) return rc;
+    double2 *k1 = w->d_stage, *k2 = k1 + total, *k3 = k2 + total, *k4 = k3 + total, *y = k4 + total;
+    const double dt = q.tstep;
+    const unsigned nb = div_up(total, 256);
+    if ((rc = nonlinear_rhs(w, orb, hob, hint, rho2, rinv, k1, st))) return rc;
+    k_axpy_stage<<<nb, 256, 0, st>>>(orb, k1, 0.5 * dt, y, total);
+    if ((rc = nonlinear_rhs(w, y, hob, hint, rho2, rinv, k2, st))) return rc;
+    k_axpy_stage<<<nb, 256, 0, st>>>(orb, k2, 0.5 * dt, y, total);
+    if ((rc = nonlinear_rhs(w, y, hob, hint, rho2, rinv, k3, st))) return rc;
+    k_axpy_stage<<<nb, 256, 0, st>>>(orb, k3, dt, y, total);
+    if ((rc = nonlinear_rhs(w, y, hob, hint, rho2, rinv, k4, st))) return rc;
+    k_rk4_combine<<<nb, 256, 0, st>>>(orb, k1, k2, k3, k4, dt, total);
+    rc = use_fft ? linear_fft(w, orb, st) : linear_cn(w, orb, st);
+    if (rc) return rc;
+    if (w->imagtime) rc = orthonormalize(w, orb, st);
+    return rc;
+}
+
+// ---------------------------------------------------- M x M algebra
+
+// Hermitian eigen-decomposition by cyclic complex Jacobi rotations (one
+// thread; a is overwritten), eigenvalues ascending with eigenvectors in the
+// columns of v.  Returns 0 on convergence.
+__device__ int herm_eig_dev(int n, double2* a, double* lam, double2* v) {
+    for (int r = 0; r < n; r++)
+        for (int c = 0; c < n; c++) v[r * n + c] = make_double2(r == c ? 1.0 : 0.0, 0.0);
+    int ok = 0;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0, tot = 0;
+        for (int r = 0; r < n; r++)
+            for (int c = 0; c < n; c++) {
+                const double m2 = a[r * n + c].x * a[r * n + c].x + a[r * n + c].y * a[r * n + c].y;
+                tot += m2;
+                if (r != c) off += m2;
+            }
+        if (off <= 1e-30 * tot || off == 0) {
+            ok = 1;
+            break;
+        }
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                const double2 apq = a[p * n + q];
+                const double mag = sqrt(apq.x * apq.x + apq.y * apq.y);
+                if (mag == 0) continue;
+                const double app = a[p * n + p].x, aqq = a[q * n + q].x;
+                const double2 ph = cscale(apq, 1.0 / mag);
+                const double tau = (aqq - app) / (2 * mag);
+                const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1 + tau * tau));
+                const double c = 1 / sqrt(1 + t * t), s = t * c;
+                for (int r = 0; r < n; r++) {
+                    const double2 arp = a[r * n + p], arq = a[r * n + q];
+                    a[r * n + p] = cscale(arp, c) - cmul(cscale(dconj2(ph), s), arq);
+                    a[r * n + q] = cmul(cscale(ph, s), arp) + cscale(arq, c);
+                }
+                for (int r = 0; r < n; r++) {
+                    const double2 apr = a[p * n + r], aqr = a[q * n + r];
+                    a[p * n + r] = cscale(apr, c) - cmul(cscale(ph, s), aqr);
+                    a[q * n + r] = cmul(cscale(dconj2(ph), s), apr) + cscale(aqr, c);
+                }
+                a[p * n + q] = make_double2(0, 0);
+                a[q * n + p] = make_double2(0, 0);
+                for (int r = 0; r < n; r++) {
+                    const double2 vrp = v[r * n + p], vrq = v[r * n + q];
+                    v[r * n + p] = cscale(vrp, c) - cmul(cscale(dconj2(ph), s), vrq);
+                    v[r * n + q] = cmul(cscale(ph, s), vrp) + cscale(vrq, c);
+                }
+            }
+    }
+    for (int r = 0; r < n; r++) lam[r] = a[r * n + r].x;
+    for (int x = 0; x < n - 1; x++) {
+        int best = x;
+        for (int y = x + 1; y < n; y++)
+            if (lam[y] < lam[best]) best = y;
+        if (best != x) {
+            const double t = lam[x];
+            lam[x] = lam[best];
+            lam[best] = t;
+            for (int r = 0; r < n; r++) {
+                const double2 tv = v[r * n + x];
+                v[r * n + x] = v[r * n + best];
+                v[r * n + best] = tv;
+            }
+        }
+    }
+    return ok ? 0 : 1;
+}
+
+// rho1 eigenvalues (synchronize.c:49), regularisation when the smallest
+// natural occupation is below reg_factor times the largest
+// (lapack_interface.c:126-163, synchronize.c:50-54), then the inverse from
+// the lower triangle (lapack_interface.c:10-59).  One thread, M <= 30.
+__global__ void k_density_sync_small(int M, double2* __restrict__ rho1, double2* __restrict__ rinv,
+                                     double* __restrict__ eig_out, double reg_factor, double2* __restrict__ work) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double2* a = work;              // M*M
+    double2* v = work + M * M;      // M*M
+    double2* inv = work + 2 * M * M;
+    double lam[30];
+    // zheev reads the upper triangle with a real diagonal
+    for (int r = 0; r < M; r++)
+        for (int c = 0; c < M; c++)
+            a[r * M + c] = r == c ? make_double2(rho1[r * M + r].x, 0)
+                                  : (c > r ? rho1[r * M + c] : dconj2(rho1[c * M + r]));
+    herm_eig_dev(M, a, lam, v);
+    if (eig_out)
+        for (int r = 0; r < M; r++) eig_out[r] = lam[r];
+    if (lam[0] / lam[M - 1] < reg_factor) {
+        const double x = lam[M - 1] * reg_factor;
+        for (int i = 0; i < M; i++)
+            for (int j = 0; j < M; j++) {
+                double2 z = make_double2(0, 0);
+                for (int k = 0; k < M; k++)
+                    z = z + cmul(cscale(v[i * M + k], x * exp(-lam[k] / x)), dconj2(v[j * M + k]));
+                rho1[i * M + j] = rho1[i * M + j] + z;
+            }
+    }
+    // Gauss-Jordan inverse with partial pivoting of the Hermitian matrix
+    // given by the lower triangle and real diagonal
+    for (int r = 0; r < M; r++)
+        for (int c = 0; c < M; c++) {
+            a[r * M + c] = r == c ? make_double2(rho1[r * M + r].x, 0)
+                                  : (c < r ? rho1[r * M + c] : dconj2(rho1[c * M + r]));
+            inv[r * M + c] = make_double2(r == c ? 1.0 : 0.0, 0);
+        }
+    for (int col = 0; col < M; col++) {
+        int piv = col;
+        double best = 0;
+        for (int r = col; r < M; r++) {
+            const double m2 = a[r * M + col].x * a[r * M + col].x + a[r * M + col].y * a[r * M + col].y;
+            if (m2 > best) {
+                best = m2;
+                piv = r;
+            }
+        }
+        if (best == 0) break;
+        if (piv != col)
+            for (int c = 0; c < M; c++) {
+                double2 t = a[col * M + c];
+                a[col * M + c] = a[piv * M + c];
+                a[piv * M + c] = t;
+                t = inv[col * M + c];
+                inv[col * M + c] = inv[piv * M + c];
+                inv[piv * M + c] = t;
+            }
+        const double2 d = cdiv(make_double2(1, 0), a[col * M + col]);
+        for (int c = 0; c < M; c++) {
+            a[col * M + c] = cmul(a[col * M + c], d);
+            inv[col * M + c] = cmul(inv[col * M + c], d);
+        }
+        for (int r = 0; r < M; r++) {
+            if (r == col) continue;
+            const double2 f = a[r * M + col];
+            if (f.x == 0 && f.y == 0) continue;
+            for (int c = 0; c < M; c++) {
+                a[r * M + c] = a[r * M + c] - cmul(f, a[col * M + c]);
+                inv[r * M + c] = inv[r * M + c] - cmul(f, inv[col * M + c]);
+            }
+        }
+    }
+    for (int i = 0; i < M * M; i++) rinv[i] = inv[i];
+}
+
+int density_sync_small(int norb, double2* rho1, double2* rinv, double* eigvals, double reg_factor, hipStream_t st) {
+    if (norb > 30) {
+        set_error("norb exceeds 30");
+        return kInvalidArgument;
+    }
+    double2* work = nullptr;
+    MCTDHB_HIP_CHECK(hipMallocAsync((void**) &work, sizeof(double2) * 3 * norb * norb, st));
+    k_density_sync_small<<<1, 64, 0, st>>>(norb, rho1, rinv, eigvals, reg_factor, work);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    MCTDHB_HIP_CHECK(hipFreeAsync(work, st));
+    return kOk;
+}
+
+// ------------------------------------------------------------- set-up
+
+int orbwork_create(const OrbEq& eq, int imagtime, const double* pot_host, OrbWork** out) {
+    *out = nullptr;
+    if (eq.norb < 1 || eq.norb > 30 || eq.grid_size < 6) {
+        set_error("invalid orbital equation (norb in 1..30, grid_size >= 6)");
+        return kInvalidArgument;
+    }
+    OrbWork* w = new OrbWork();
+    w->eq = eq;
+    w->imagtime = imagtime;
+    const int grid = eq.grid_size, n = grid - 1, M = eq.norb;
+    w->nfft = n;
+    w->log2n = (n & (n - 1)) == 0 ? __builtin_ctz((unsigned) n) : -1;
+    if (w->log2n >= 0 && n > 4096) w->log2n = -1;
+    if (n > 4096) {
+        set_error("grid_size - 1 above 4096 is not supported by the FFT kernels");
+        delete w;
+        return kInvalidArgument;
+    }
+    // FFT exponential (split_linear_orbitals.c:56-80) and twiddles on the host
+    std::vector<double2> fexp(n), tw(n);
+    const double2 dt = make_double2(0.5 * eq.prop_dt.x, 0.5 * eq.prop_dt.y);
+    for (int i = 0; i < n; i++) {
+        const double freq = (i <= (n - 1) / 2) ? (2 * M_PI * i) / (n * eq.dx) : (2 * M_PI * (i - n)) / (n * eq.dx);
+        // -i dt (i a1 k - a2 k^2) = dt (a1 k) + i dt a2 k^2
+        const double2 ia1k = make_double2(-eq.d1coef.y * freq, eq.d1coef.x * freq);
+        const double2 arg0 = make_double2(ia1k.x - eq.d2coef * freq * freq, ia1k.y);
+        // z = -i * dt * arg0
+        const double2 dta = make_double2(dt.x * arg0.x - dt.y * arg0.y, dt.x * arg0.y + dt.y * arg0.x);
+        const double2 z = make_double2(dta.y, -dta.x);
+        const double m = exp(z.x);
+        fexp[i] = make_double2(m * cos(z.y), m * sin(z.y));
+        const long long r = i % n;
+        const double ang = 2 * M_PI * (double) r / (double) n;
+        tw[i] = make_double2(cos(ang), -sin(ang));
+    }
+    // CN tridiagonal (split_linear_orbitals.c:22-54)
+    std::vector<double2> cn(3 * (size_t) grid, make_double2(0, 0));
+    {
+        const double a2 = eq.d2coef, dx = eq.dx;
+        const double2 a2dt = make_double2(dt.x * a2 / dx / dx, dt.y * a2 / dx / dx);
+        const double2 a1dt = make_double2((eq.d1coef.x * dt.x - eq.d1coef.y * dt.y) / dx / 4,
+                                          (eq.d1coef.x * dt.y + eq.d1coef.y * dt.x) / dx / 4);
+        const double2 minus_half = make_double2(-0.5 * a2dt.x, -0.5 * a2dt.y);
+        for (int i = 0; i < grid; i++) cn[2 * grid + i] = make_double2(a2dt.x, 1.0 + a2dt.y);
+        for (int i = 0; i < grid - 2; i++) {
+            cn[i] = make_double2(minus_half.x - a1dt.x, minus_half.y - a1dt.y);
+            cn[grid + i] = make_double2(minus_half.x + a1dt.x, minus_half.y + a1dt.y);
+        }
+        if (eq.bounds == kZeroBounds) {
+            cn[grid - 2] = make_double2(minus_half.x - a1dt.x, minus_half.y - a1dt.y);
+            cn[grid + grid - 2] = make_double2(minus_half.x + a1dt.x, minus_half.y + a1dt.y);
+        } else {
+            cn[grid - 2] = make_double2(minus_half.x + a1dt.x, minus_half.y + a1dt.y);
+            cn[grid + grid - 2] = make_double2(minus_half.x - a1dt.x, minus_half.y - a1dt.y);
+        }
+    }
+    bool ok = hipMalloc((void**) &w->d_pot, sizeof(double) * grid) == hipSuccess;
+    ok = ok && hipMalloc((void**) &w->d_fexp, sizeof(double2) * n) == hipSuccess;
+    ok = ok && hipMalloc((void**) &w->d_tw, sizeof(double2) * n) == hipSuccess;
+    ok = ok && hipMalloc((void**) &w->d_cn, sizeof(double2) * 3 * grid) == hipSuccess;
+    ok = ok && hipMalloc((void**) &w->d_stage, sizeof(double2) * 5 * (size_t) M * grid) == hipSuccess;
+    ok = ok && hipMalloc((void**) &w->d_red, sizeof(double) * 1024) == hipSuccess;
+    if (!ok) {
+        orbwork_destroy(w);
+        set_error("device allocation failed for the orbital workspace");
+        return kOutOfMemory;
+    }
+    MCTDHB_HIP_CHECK(hipMemcpy(w->d_fexp, fexp.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+    MCTDHB_HIP_CHECK(hipMemcpy(w->d_tw, tw.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+    MCTDHB_HIP_CHECK(hipMemcpy(w->d_cn, cn.data(), sizeof(double2) * 3 * grid, hipMemcpyHostToDevice));
+    int rc = orbwork_set_potential(w, pot_host);
+    if (rc) {
+        orbwork_destroy(w);
+        return rc;
+    }
+    *out = w;
+    return kOk;
+}
+
+int orbwork_set_potential(OrbWork* w, const double* pot_host) {
+    MCTDHB_HIP_CHECK(hipMemcpy(w->d_pot, pot_host, sizeof(double) * w->eq.grid_size, hipMemcpyHostToDevice));
+    return kOk;
+}
+
+void orbwork_destroy(OrbWork* w) {
+    if (!w) return;
+    hipFree(w->d_pot);
+    hipFree(w->d_fexp);
+    hipFree(w->d_tw);
+    hipFree(w->d_cn);
+    hipFree(w->d_stage);
+    hipFree(w->d_red);
+    delete w;
+}
+
+}  // namespace mctdhb

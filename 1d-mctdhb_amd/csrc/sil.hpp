@@ -22,6 +22,7 @@ struct SilState {
     double2 y[kMaxLanczos + 1];       // recombination coefficients (on w_k)
     double maxbeta;
     double nrm2;                      // |C'|^2 of the recombined vector
+    double eground;                   // lowest Ritz value (ground-state mode)
     int iter_done;
     int stop;                         // breakdown: later kernels are no-ops
     int eig_fail;

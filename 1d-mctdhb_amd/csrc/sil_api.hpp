@@ -28,6 +28,9 @@ size_t sil_device_bytes(const Space* S, int lm, int prec);
 // C, hob (M x M), hint (M^4) are complex128 device arrays.
 int sil_step(Sil* L, double2* C, const double2* hob, const double2* hint, double dt_re, double dt_im, int imagtime,
              hipStream_t st);
+// C <- the Ritz vector of the lowest Ritz value (not renormalised); the
+// value is left in SilState::eground.
+int sil_ground(Sil* L, double2* C, const double2* hob, const double2* hint, hipStream_t st);
 // Lanczos recursion only; lvec (lm x dim complex128, device) receives the
 // normalised vectors.
 int sil_lanczos(Sil* L, const double2* C, const double2* hob, const double2* hint, double2* lvec, hipStream_t st);

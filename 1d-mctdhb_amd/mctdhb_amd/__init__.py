@@ -90,6 +90,19 @@ def lib():
     L.mctdhb_sil_step_dev.argtypes = [_vp, _vp, _vp, _vp, C.c_double, C.c_double, _i, _vp]
     L.mctdhb_lanczos_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
     L.mctdhb_sil_scalars.argtypes = [_vp, C.POINTER(_i), _vp, _vp, _vp]
+    L.mctdhb_lowest_state_dev.argtypes = [_vp, _vp, _vp, _vp, C.POINTER(C.c_double), _vp]
+    L.mctdhb_orbital_create.argtypes = [_i, _i, _i, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, C.c_double, _i, _vp, _cpp]
+    L.mctdhb_orbital_destroy.argtypes = [_vp]
+    L.mctdhb_orbital_set_potential.argtypes = [_vp, _vp]
+    L.mctdhb_linear_fft_dev.argtypes = [_vp, _vp, _vp]
+    L.mctdhb_linear_cn_dev.argtypes = [_vp, _vp, _vp]
+    L.mctdhb_orbital_matrices_dev.argtypes = [_vp, _vp, _vp, _vp, _vp]
+    L.mctdhb_nonlinear_rhs_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.mctdhb_orthonormalize_dev.argtypes = [_vp, _vp, _vp]
+    L.mctdhb_splitstep_dev.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.mctdhb_density_sync_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, _i, _vp]
+    L.mctdhb_propagate_step_dev.argtypes = [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     # reference-named drop-in layer
     L.space_dimension.argtypes = [C.c_uint16, C.c_uint16]
     L.space_dimension.restype = C.c_uint32
@@ -104,6 +117,8 @@ def lib():
     L.set_twobody_dm.argtypes = [C.POINTER(_MultiConfiguration), _vp, _vp]
     L.lanczos.argtypes = [C.POINTER(_MultiConfiguration), C.POINTER(_vp), _vp, _i, _vp, _vp, C.POINTER(_vp)]
     L.lanczos.restype = _i
+    L.lowest_state_lanczos.argtypes = [C.c_uint16, C.POINTER(_MultiConfiguration), C.POINTER(_vp), _vp, _vp]
+    L.lowest_state_lanczos.restype = C.c_double
     _lib = L
     return L
 
@@ -204,6 +219,14 @@ class DeviceSil:
     def lanczos(self, coef, hob, hint, lvec, stream=None):
         check(lib().mctdhb_lanczos_dev(self._h, _ptr(coef), _ptr(hob), _ptr(hint), _ptr(lvec), _stream(stream)))
 
+    def lowest_state(self, coef, hob, hint, stream=None):
+        """coef <- lowest Ritz vector (reference lowest_state_lanczos,
+        multiconfig_lanczos.c:93-172); returns the lowest Ritz value."""
+        e = C.c_double()
+        check(lib().mctdhb_lowest_state_dev(self._h, _ptr(coef), _ptr(hob), _ptr(hint), C.byref(e),
+                                            _stream(stream)))
+        return e.value
+
     def scalars(self, stream=None):
         done = _i()
         diag = np.zeros(2 * self.lm)
@@ -211,6 +234,68 @@ class DeviceSil:
         check(lib().mctdhb_sil_scalars(self._h, C.byref(done), diag.ctypes.data, offd.ctypes.data,
                                        _stream(stream)))
         return done.value, diag.view(np.complex128), offd
+
+
+class DeviceOrbital:
+    """Orbital-equation workspace (reference _OrbitalEquation +
+    _OrbitalWorkspace, mctdhb_types.h:136-149, :199-212) on the GPU."""
+
+    def __init__(self, norb, grid_size, xi, xf, tstep, g, pot, imagtime=True, bounds=1001, d2coef=-0.5,
+                 d1coef=0j):
+        pot = np.ascontiguousarray(pot, np.float64)
+        h = _vp()
+        check(lib().mctdhb_orbital_create(norb, grid_size, bounds, xi, xf, tstep, g, d2coef,
+                                          float(np.real(d1coef)), float(np.imag(d1coef)), int(bool(imagtime)),
+                                          pot.ctypes.data, C.byref(h)))
+        self._h, self.norb, self.grid_size, self.imagtime = h, norb, grid_size, imagtime
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mctdhb_orbital_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_potential(self, pot):
+        pot = np.ascontiguousarray(pot, np.float64)
+        check(lib().mctdhb_orbital_set_potential(self._h, pot.ctypes.data))
+
+    def linear_fft(self, orb, stream=None):
+        check(lib().mctdhb_linear_fft_dev(self._h, _ptr(orb), _stream(stream)))
+
+    def linear_cn(self, orb, stream=None):
+        check(lib().mctdhb_linear_cn_dev(self._h, _ptr(orb), _stream(stream)))
+
+    def orbital_matrices(self, orb, hob, hint, stream=None):
+        check(lib().mctdhb_orbital_matrices_dev(self._h, _ptr(orb), _ptr(hob), _ptr(hint), _stream(stream)))
+
+    def nonlinear_rhs(self, orb, hob, hint, rho2, rinv, der, stream=None):
+        check(lib().mctdhb_nonlinear_rhs_dev(self._h, _ptr(orb), _ptr(hob), _ptr(hint), _ptr(rho2), _ptr(rinv),
+                                             _ptr(der), _stream(stream)))
+
+    def orthonormalize(self, orb, stream=None):
+        check(lib().mctdhb_orthonormalize_dev(self._h, _ptr(orb), _stream(stream)))
+
+    def splitstep(self, orb, hob, hint, rho2, rinv, use_fft=True, stream=None):
+        check(lib().mctdhb_splitstep_dev(self._h, int(bool(use_fft)), _ptr(orb), _ptr(hob), _ptr(hint),
+                                         _ptr(rho2), _ptr(rinv), _stream(stream)))
+
+
+def density_sync(space: DeviceSpace, coef, rho1, rho2, rinv, eigvals=None, reg_factor=1e-9, prec=FP64,
+                 stream=None):
+    check(lib().mctdhb_density_sync_dev(space.handle, _ptr(coef), _ptr(rho1), _ptr(rho2), _ptr(rinv),
+                                        _ptr(eigvals), reg_factor, prec, _stream(stream)))
+
+
+def propagate_step(space: DeviceSpace, sil: DeviceSil, orbital: DeviceOrbital, orb, coef, hob, hint, rho1, rho2,
+                   rinv, use_fft=True, stream=None):
+    check(lib().mctdhb_propagate_step_dev(space.handle, sil._h, orbital.handle, int(bool(use_fft)), _ptr(orb),
+                                          _ptr(coef), _ptr(hob), _ptr(hint), _ptr(rho1), _ptr(rho2), _ptr(rinv),
+                                          _stream(stream)))
 
 
 class Multiconf:
@@ -272,3 +357,12 @@ class Multiconf:
         done = lib().lanczos(self._p, self._rows(hob), hint.ctypes.data, lm, diag.ctypes.data,
                              offd.ctypes.data, self._rows(lvec))
         return done, diag, offd, lvec
+
+    def lowest_state_lanczos(self, coef, hob, hint, niter):
+        """Returns (lowest Ritz value, Ritz vector) like reference
+        lowest_state_lanczos (multiconfig_lanczos.c:93-172)."""
+        c = np.array(coef, np.complex128, copy=True)
+        hob = np.ascontiguousarray(hob, np.complex128).reshape(self.norb, self.norb)
+        hint = np.ascontiguousarray(hint, np.complex128)
+        e = lib().lowest_state_lanczos(niter, self._p, self._rows(hob), hint.ctypes.data, c.ctypes.data)
+        return e, c

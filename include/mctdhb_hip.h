@@ -150,10 +150,53 @@ int mctdhb_sil_step_dev(mctdhb_sil* sil, void* coef, const void* hob, const void
  * normalised Krylov vectors, lvec row 0 = coef. */
 int mctdhb_lanczos_dev(mctdhb_sil* sil, const void* coef, const void* hob, const void* hint,
                        void* lvec, void* stream);
+/* coef <- Ritz vector of the lowest Ritz value of the lanczos_iter-step
+ * recursion started from coef (not renormalised); *energy (host, may be
+ * NULL; synchronises) <- that Ritz value.  multiconfig_lanczos.c:93-172 */
+int mctdhb_lowest_state_dev(mctdhb_sil* sil, void* coef, const void* hob, const void* hint, double* energy,
+                            void* stream);
 /* Host copy of the recursion scalars of the last call (synchronises):
  * diag (2*lanczos_iter doubles, complex), offd (lanczos_iter doubles). */
 int mctdhb_sil_scalars(const mctdhb_sil* sil, int* iter_done, double* diag, double* offd,
                        void* stream);
+
+/* ---- orbital step (device-resident) ---------------------------------- */
+
+typedef struct mctdhb_orbital mctdhb_orbital; /* orbital-equation workspace */
+
+/* Grid of grid_size points on [xi, xf] (dx = (xf - xi)/(grid_size - 1), the
+ * last point repeating the first for PERIODIC_BOUNDS), kinetic coefficients
+ * d2coef (real) and d1coef (complex), contact interaction g, step tstep in
+ * imaginary (prop_dt = -i tstep) or real time, as reference
+ * get_orbital_equation / get_orbital_workspace (types_definition.c:30-108,
+ * :200-244).  pot: grid_size doubles on the host. */
+int mctdhb_orbital_create(int norb, int grid_size, int bounds, double xi, double xf, double tstep, double g,
+                          double d2coef, double d1coef_re, double d1coef_im, int imagtime, const double* pot,
+                          mctdhb_orbital** out);
+int mctdhb_orbital_destroy(mctdhb_orbital* w);
+int mctdhb_orbital_set_potential(mctdhb_orbital* w, const double* pot);
+/* Device arrays: orb norb x grid_size, hob M x M, hint M^4, rho2 M^4,
+ * rinv M x M (inverse one-body density matrix), all complex128. */
+int mctdhb_linear_fft_dev(mctdhb_orbital* w, void* orb, void* stream);       /* split_linear_orbitals.c:375-402 */
+int mctdhb_linear_cn_dev(mctdhb_orbital* w, void* orb, void* stream);        /* split_linear_orbitals.c:335-373 */
+int mctdhb_orbital_matrices_dev(mctdhb_orbital* w, const void* orb, void* hob, void* hint,
+                                void* stream);                                /* synchronize.c:34-39 */
+int mctdhb_nonlinear_rhs_dev(mctdhb_orbital* w, const void* orb, const void* hob, const void* hint,
+                             const void* rho2, const void* rinv, void* der,
+                             void* stream);                                   /* orbital_integration.c:242-281 */
+int mctdhb_orthonormalize_dev(mctdhb_orbital* w, void* orb, void* stream);   /* calculus.c:170-191 */
+int mctdhb_splitstep_dev(mctdhb_orbital* w, int use_fft, void* orb, void* hob, void* hint, const void* rho2,
+                         const void* rinv, void* stream);                     /* orbital_integration.c:351-434 */
+/* rho1, rho2 of coef, then natural occupations (eigvals, M doubles, device,
+ * may be NULL), regularisation and rinv = rho1^-1 (synchronize.c:41-56). */
+int mctdhb_density_sync_dev(mctdhb_space* space, const void* coef, void* rho1, void* rho2, void* rinv,
+                            void* eigvals, double reg_factor, int prec, void* stream);
+/* One MCTDHB time step for a time-independent trap and interaction
+ * (driver.c:159-202): orbital split step, h/g rebuild, SIL coefficient
+ * step, density sync.  All arrays device-resident. */
+int mctdhb_propagate_step_dev(mctdhb_space* space, mctdhb_sil* sil, mctdhb_orbital* w, int use_fft, void* orb,
+                              void* coef, void* hob, void* hint, void* rho1, void* rho2, void* rinv,
+                              void* stream);
 
 #ifdef __cplusplus
 }

@@ -148,6 +148,25 @@ def test_sil_step_matches_oracle(amd, orc, npar, norb, lm, imag):
     assert gu.rel_err(gu.host(dc), ref) < 1e-10
 
 
+@pytest.mark.parametrize("npar,norb,lm,prec", [(6, 4, 8, 0), (10, 5, 12, 0), (10, 5, 12, 1), (3, 9, 6, 0)])
+def test_lowest_state_matches_oracle(amd, orc, npar, norb, lm, prec):
+    print(f"lowest N={npar} M={norb} lm={lm} prec={prec}", flush=True)
+    S = orc.Space(npar, norb)
+    c, h, g = _problem(npar, norb, 37)
+    e_ref, ref = S.lowest_state_lanczos(c, h, g, lm)
+    D = amd.DeviceSpace(npar, norb, tables=False)
+    sil = amd.DeviceSil(D, lm, prec=prec, index=amd.INDEX_FLY)
+    dc = gu.dev(c)
+    e = sil.lowest_state(dc, gu.dev(h), gu.dev(g))
+    tol = 1e-10 if prec == amd.FP64 else 1e-5
+    assert abs(e - e_ref) <= tol * abs(e_ref)
+    assert gu.rel_err(gu.host(dc), ref) < tol
+    # the Ritz value is the Rayleigh quotient of the (unit) Ritz vector
+    v = gu.host(dc)
+    hv = S.apply_hamiltonian(v, h, g)
+    assert abs(np.vdot(v, hv).real / np.vdot(v, v).real - e_ref) < 1e-8 * abs(e_ref)
+
+
 def test_lanczos_breakdown_on_device(amd, orc):
     # H couples orbitals 0 and 1 only; e_0 spans a 2-dimensional Krylov space
     npar, norb = 1, 3
@@ -176,6 +195,10 @@ def test_dropin_reference_entry_points(amd, orc):
     assert gu.rel_err(mc.apply_hamiltonian(c, h, g), S.apply_hamiltonian(c, h, g)) < 1e-12
     assert gu.rel_err(mc.onebody_dm(c), S.onebody_dm(c)) < 1e-12
     assert gu.rel_err(mc.twobody_dm(c), S.twobody_dm(c)) < 1e-12
+    e, v = mc.lowest_state_lanczos(c, h, g, 7)
+    re, rv = S.lowest_state_lanczos(c, h, g, 7)
+    assert abs(e - re) < 1e-10 * abs(re)
+    assert gu.rel_err(v, rv) < 1e-10
     done, diag, offd, V = mc.lanczos(c, h, g, 6)
     rdone, rdiag, roffd, rV = S.lanczos(c, h, g, 6)
     assert done == rdone
