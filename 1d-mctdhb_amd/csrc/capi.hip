@@ -9,7 +9,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "density.hpp"
@@ -316,6 +318,28 @@ extern "C" int mctdhb_orbital_set_potential(mctdhb_orbital* o, const double* pot
         return kInvalidArgument;
     }
     return orbwork_set_potential(o->w, pot);
+}
+
+extern "C" int mctdhb_orbital_set_interaction(mctdhb_orbital* o, double g) {
+    if (!o) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    o->w->eq.g = g;
+    return kOk;
+}
+
+extern "C" int mctdhb_orbital_set_rk_order(mctdhb_orbital* o, int order) {
+    if (!o) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    if (order != 2 && order != 4 && order != 5) {
+        set_error("Runge-Kutta order must be 2, 4 or 5");
+        return kInvalidArgument;
+    }
+    o->w->rk_order = order;
+    return kOk;
 }
 
 #define ORB_CHECK(...)                      \
@@ -869,4 +893,380 @@ extern "C" dcomplex total_energy(ManyBodyState psi) {
     dcomplex e;
     memcpy(&e, out, sizeof(e));
     return e;
+}
+
+// ----------------------------------------------- layer 1: orbital side
+//
+// The reference-named orbital entry points work on the reference's host
+// structs.  The device workspace of an _OrbitalEquation lives in a registry
+// keyed by the struct's address (sync_orbital_matrices receives no
+// workspace); it is rebuilt when a parameter it was built from changes, and
+// freed by destroy_orbital_equation.
+
+namespace {
+
+struct DevOrbital {
+    mctdhb_orbital* o = nullptr;
+    // parameters the device workspace was built from
+    int norb = 0, grid = 0, bounds = 0, imagtime = 0;
+    double xi = 0, xf = 0, tstep = 0, d2coef = 0, d1[2] = {0, 0};
+    // host <-> device staging
+    double2 *d_orb = nullptr, *d_hob = nullptr, *d_hint = nullptr, *d_rho2 = nullptr, *d_rinv = nullptr;
+};
+
+std::mutex g_orb_mu;
+std::unordered_map<const void*, DevOrbital*> g_orb;
+double g_reg_factor = 1e-9;  // DEFAULT_REGULARIZATION_FACTOR (synchronize.h:23)
+
+void free_dev_orbital(DevOrbital* d) {
+    if (!d) return;
+    mctdhb_orbital_destroy(d->o);
+    hipFree(d->d_orb);
+    hipFree(d->d_hob);
+    hipFree(d->d_hint);
+    hipFree(d->d_rho2);
+    hipFree(d->d_rinv);
+    delete d;
+}
+
+}  // namespace
+
+static void c2d(const dcomplex& z, double* out) { memcpy(out, &z, sizeof(double) * 2); }
+
+// The device workspace of eq, with g and the trap refreshed from eq.
+static DevOrbital* dev_orbital(OrbitalEquation eq) {
+    std::lock_guard<std::mutex> lock(g_orb_mu);
+    double d1[2], pdt[2];
+    c2d(eq->d1coef, d1);
+    c2d(eq->prop_dt, pdt);
+    const int imag = pdt[1] != 0.0;
+    DevOrbital*& d = g_orb[eq];
+    if (d && (d->norb != eq->norb || d->grid != eq->grid_size || d->bounds != (int) eq->bounds ||
+              d->imagtime != imag || d->xi != eq->xi || d->xf != eq->xf || d->tstep != eq->tstep ||
+              d->d2coef != eq->d2coef || d->d1[0] != d1[0] || d->d1[1] != d1[1])) {
+        hipDeviceSynchronize();
+        free_dev_orbital(d);
+        d = nullptr;
+    }
+    if (!d) {
+        d = new DevOrbital();
+        if (mctdhb_orbital_create(eq->norb, eq->grid_size, eq->bounds, eq->xi, eq->xf, eq->tstep, eq->g, eq->d2coef,
+                                  d1[0], d1[1], imag, eq->pot_grid, &d->o)) {
+            delete d;
+            d = nullptr;
+            fail_stop("orbital workspace");
+        }
+        d->norb = eq->norb;
+        d->grid = eq->grid_size;
+        d->bounds = eq->bounds;
+        d->imagtime = imag;
+        d->xi = eq->xi;
+        d->xf = eq->xf;
+        d->tstep = eq->tstep;
+        d->d2coef = eq->d2coef;
+        d->d1[0] = d1[0];
+        d->d1[1] = d1[1];
+        const size_t M = eq->norb;
+        stage_alloc(&d->d_orb, M * eq->grid_size);
+        stage_alloc(&d->d_hob, M * M);
+        stage_alloc(&d->d_hint, M * M * M * M);
+        stage_alloc(&d->d_rho2, M * M * M * M);
+        stage_alloc(&d->d_rinv, M * M);
+    } else if (mctdhb_orbital_set_potential(d->o, eq->pot_grid)) {
+        fail_stop("orbital potential");
+    }
+    if (mctdhb_orbital_set_interaction(d->o, eq->g)) fail_stop("orbital interaction");
+    return d;
+}
+
+static void orbitals_h2d(double2* d, Cmatrix orb, int norb, int grid) {
+    for (int k = 0; k < norb; k++) h2d(d + (size_t) k * grid, orb[k], sizeof(dcomplex) * grid);
+}
+
+static void orbitals_d2h(Cmatrix orb, const double2* d, int norb, int grid) {
+    for (int k = 0; k < norb; k++) d2h(orb[k], d + (size_t) k * grid, sizeof(dcomplex) * grid);
+}
+
+static void cmatrix_h2d(double2* d, Cmatrix m, int n) { orbitals_h2d(d, m, n, n); }
+static void cmatrix_d2h(Cmatrix m, const double2* d, int n) { orbitals_d2h(m, d, n, n); }
+
+extern "C" OrbitalEquation get_orbital_equation(char eq_name[], uint16_t norb, uint16_t grid_size,
+                                                BoundaryCondition bounds, IntegratorType integ_type, double xi,
+                                                double xf, double tstep, double tend, double d2coef, dcomplex d1coef,
+                                                void* pot_extra_args, void* inter_extra_args,
+                                                single_particle_pot pot_func, time_dependent_parameter inter_param) {
+    if (grid_size > MAX_GRID_SIZE) {
+        printf("\n\nERROR: Exceeded max grid size %d\n\n", MAX_GRID_SIZE);
+        exit(EXIT_FAILURE);
+    }
+    if (grid_size < MIN_GRID_SIZE) {
+        printf("\n\nERROR: Below min grid size %d\n\n", MIN_GRID_SIZE);
+        exit(EXIT_FAILURE);
+    }
+    if (norb > MAX_ORBITALS) {
+        printf("\n\nERROR: Exceeded max number of orbitals %d\n\n", MAX_ORBITALS);
+        exit(EXIT_FAILURE);
+    }
+    if (!pot_func || !inter_param) {
+        printf("\n\nTYPEERROR: Null pointer given for the trap potential or the interaction parameter\n\n");
+        exit(EXIT_FAILURE);
+    }
+    OrbitalEquation eq = (OrbitalEquation) xmalloc(sizeof(_OrbitalEquation));
+    snprintf(eq->eq_name, STR_BUFF_SIZE, "%s", eq_name ? eq_name : "");
+    eq->t = 0;
+    eq->norb = norb;
+    eq->grid_size = grid_size;
+    eq->bounds = bounds;
+    eq->xi = xi;
+    eq->xf = xf;
+    eq->dx = (xf - xi) / (grid_size - 1);
+    eq->tstep = tstep;
+    eq->tend = tend;
+    const double pdt[2] = {integ_type == IMAGTIME ? 0.0 : tstep, integ_type == IMAGTIME ? -tstep : 0.0};
+    const double tf[2] = {integ_type == IMAGTIME ? 0.0 : 1.0, integ_type == IMAGTIME ? -1.0 : 0.0};
+    memcpy(&eq->prop_dt, pdt, sizeof(pdt));
+    memcpy(&eq->time_fac, tf, sizeof(tf));
+    eq->grid_pts = (Rarray) xmalloc(sizeof(double) * grid_size);
+    for (int i = 0; i < grid_size; i++) eq->grid_pts[i] = xi + i * eq->dx;
+    eq->pot_grid = (Rarray) xmalloc(sizeof(double) * grid_size);
+    pot_func(0, grid_size, eq->grid_pts, pot_extra_args, eq->pot_grid);
+    eq->pot_extra_args = pot_extra_args;
+    eq->inter_extra_args = inter_extra_args;
+    eq->d2coef = d2coef;
+    eq->d1coef = d1coef;
+    eq->g = inter_param(0, inter_extra_args);
+    eq->pot_func = pot_func;
+    eq->inter_param = inter_param;
+    return eq;
+}
+
+extern "C" void destroy_orbital_equation(OrbitalEquation eq) {
+    if (!eq) return;
+    {
+        std::lock_guard<std::mutex> lock(g_orb_mu);
+        auto it = g_orb.find(eq);
+        if (it != g_orb.end()) {
+            hipDeviceSynchronize();
+            free_dev_orbital(it->second);
+            g_orb.erase(it);
+        }
+    }
+    free(eq->grid_pts);
+    free(eq->pot_grid);
+    free(eq);
+}
+
+extern "C" OrbitalWorkspace get_orbital_workspace(OrbitalEquation eq, OrbDerivative der_method) {
+    DevOrbital* d = dev_orbital(eq);
+    const int grid = eq->grid_size, n = grid - 1, M = eq->norb;
+    OrbitalWorkspace w = (OrbitalWorkspace) xmalloc(sizeof(_OrbitalWorkspace));
+    w->norb = M;
+    w->grid_size = grid;
+    w->orb_der_method = der_method;
+    w->impr_ortho = FALSE;
+    w->dvr_mat = nullptr;  // the DVR derivative is not provided
+    w->cn_upper = (Carray) xmalloc(sizeof(dcomplex) * grid);
+    w->cn_lower = (Carray) xmalloc(sizeof(dcomplex) * grid);
+    w->cn_mid = (Carray) xmalloc(sizeof(dcomplex) * grid);
+    w->cn_rhs = nullptr;
+    w->orb_work1 = alloc_cmatrix(M, grid);
+    w->orb_work2 = alloc_cmatrix(M, grid);
+    w->extern_work = nullptr;
+    w->fft_desc = d;
+    w->fft_freq = (Rarray) xmalloc(sizeof(double) * n);
+    w->fft_hder_exp = (Carray) xmalloc(sizeof(dcomplex) * n);
+    // the constant arrays the device workspace was built with
+    const OrbWork* ow = d->o->w;
+    d2h(w->cn_upper, ow->d_cn, sizeof(dcomplex) * grid);
+    d2h(w->cn_lower, ow->d_cn + grid, sizeof(dcomplex) * grid);
+    d2h(w->cn_mid, ow->d_cn + 2 * grid, sizeof(dcomplex) * grid);
+    d2h(w->fft_hder_exp, ow->d_fexp, sizeof(dcomplex) * n);
+    for (int i = 0; i < n; i++)  // calculus.c:272-285
+        w->fft_freq[i] = (i <= (n - 1) / 2) ? (2 * M_PI * i) / (n * eq->dx) : (2 * M_PI * (i - n)) / (n * eq->dx);
+    return w;
+}
+
+extern "C" void destroy_orbital_workspace(OrbitalWorkspace w) {
+    if (!w) return;
+    free(w->cn_upper);
+    free(w->cn_lower);
+    free(w->cn_mid);
+    free_cmatrix(w->norb, w->orb_work1);
+    free_cmatrix(w->norb, w->orb_work2);
+    free(w->fft_freq);
+    free(w->fft_hder_exp);
+    free(w);  // the device side belongs to the orbital equation
+}
+
+extern "C" MCTDHBDataStruct get_mctdhb_struct(IntegratorType integ_type, CoefIntegrator coef_integ_method,
+                                              OrbIntegrator orb_integ_method, OrbDerivative orb_der_method,
+                                              RungeKuttaOrder rk_order, uint16_t lanczos_iter, uint16_t npar,
+                                              uint16_t norb, char eq_name[], BoundaryCondition bounds, double xi,
+                                              double xf, uint16_t grid_size, double tstep, double tend,
+                                              double d2coef, dcomplex d1coef, void* pot_extra_args,
+                                              void* inter_extra_args, single_particle_pot pot_func,
+                                              time_dependent_parameter inter_param) {
+    const uint32_t dim = space_dimension(npar, norb);
+    MCTDHBDataStruct m = (MCTDHBDataStruct) xmalloc(sizeof(_MCTDHBDataStruct));
+    m->integ_type = integ_type;
+    m->coef_integ_method = coef_integ_method;
+    m->orb_integ_method = orb_integ_method;
+    m->orb_der_method = orb_der_method;
+    m->rk_order = rk_order;
+    m->orb_eq = get_orbital_equation(eq_name, norb, grid_size, bounds, integ_type, xi, xf, tstep, tend, d2coef,
+                                     d1coef, pot_extra_args, inter_extra_args, pot_func, inter_param);
+    if (coef_integ_method == LANCZOS && (lanczos_iter > MAX_LANCZOS_ITER || lanczos_iter < 2)) {
+        printf("\n\nERROR: Lanczos iterations must be between %d and %d but %u was requested\n\n", 2,
+               MAX_LANCZOS_ITER, lanczos_iter);
+        exit(EXIT_FAILURE);
+    }
+    m->coef_workspace = get_coef_workspace(coef_integ_method, dim, lanczos_iter);
+    m->multiconfig_space = get_multiconf_struct(npar, norb);
+    m->state = get_manybody_state(npar, norb, grid_size);
+    m->orb_workspace = get_orbital_workspace(m->orb_eq, orb_der_method);
+    return m;
+}
+
+extern "C" void set_mctdhb_integrator(IntegratorType integ_type, CoefIntegrator coef_integ_method,
+                                      OrbIntegrator orb_integ_method, OrbDerivative orb_der_method,
+                                      RungeKuttaOrder rk_order, BoundaryCondition bounds, uint16_t lanczos_iter,
+                                      MCTDHBDataStruct m) {
+    m->integ_type = integ_type;
+    m->coef_integ_method = coef_integ_method;
+    m->orb_integ_method = orb_integ_method;
+    m->orb_der_method = orb_der_method;
+    m->rk_order = rk_order;
+    m->orb_eq->bounds = bounds;
+    if (m->coef_integ_method == LANCZOS) m->coef_workspace->lan_work->iter = lanczos_iter;
+}
+
+extern "C" void destroy_mctdhb_struct(MCTDHBDataStruct m) {
+    if (!m) return;
+    destroy_orbital_workspace(m->orb_workspace);
+    destroy_orbital_equation(m->orb_eq);
+    destroy_coef_workspace(m->coef_workspace);
+    destroy_multiconf_struct(m->multiconfig_space);
+    destroy_manybody_sate(m->state);
+    free(m);
+}
+
+extern "C" void set_regulatization_factor(double reg_fac) {
+    if (reg_fac > 1e-6 || reg_fac < 1e-13) {  // synchronize.h:19-21
+        printf("\n\nERROR: Regularization factor must lies between %.1E and %.1E but %.1E was given\n\n", 1e-13,
+               1e-6, reg_fac);
+        exit(EXIT_FAILURE);
+    }
+    g_reg_factor = reg_fac;
+}
+
+extern "C" double get_current_regularization_factor(void) { return g_reg_factor; }
+
+extern "C" void set_periodic_bounds(uint16_t norb, uint16_t grid_size, Cmatrix orb) {
+    for (int k = 0; k < norb; k++) orb[k][grid_size - 1] = orb[k][0];
+}
+
+extern "C" void advance_linear_crank_nicolson(OrbitalEquation eq, OrbitalWorkspace w, Cmatrix orb) {
+    (void) w;
+    DevOrbital* d = dev_orbital(eq);
+    orbitals_h2d(d->d_orb, orb, eq->norb, eq->grid_size);
+    if (mctdhb_linear_cn_dev(d->o, d->d_orb, nullptr)) fail_stop("advance_linear_crank_nicolson");
+    orbitals_d2h(orb, d->d_orb, eq->norb, eq->grid_size);
+}
+
+extern "C" void advance_linear_fft(OrbitalWorkspace w, Cmatrix orb) {
+    DevOrbital* d = (DevOrbital*) w->fft_desc;
+    orbitals_h2d(d->d_orb, orb, w->norb, w->grid_size);
+    if (mctdhb_linear_fft_dev(d->o, d->d_orb, nullptr)) fail_stop("advance_linear_fft");
+    orbitals_d2h(orb, d->d_orb, w->norb, w->grid_size);
+}
+
+extern "C" void sync_orbital_matrices(OrbitalEquation eq, ManyBodyState psi) {
+    DevOrbital* d = dev_orbital(eq);
+    const int M = psi->norb;
+    orbitals_h2d(d->d_orb, psi->orbitals, M, psi->grid_size);
+    if (mctdhb_orbital_matrices_dev(d->o, d->d_orb, d->d_hob, d->d_hint, nullptr)) fail_stop("sync_orbital_matrices");
+    cmatrix_d2h(psi->hob, d->d_hob, M);
+    d2h(psi->hint, d->d_hint, sizeof(dcomplex) * M * M * M * M);
+}
+
+extern "C" void sync_density_matrices(MultiConfiguration mc, ManyBodyState psi) {
+    mctdhb_space* sp = ensure_dev(mc);
+    const size_t n = sp->S->dim;
+    const int M = psi->norb;
+    stage_alloc(&sp->d_coef, n);
+    stage_alloc(&sp->d_rho1, (size_t) M * M);
+    stage_alloc(&sp->d_rho2, (size_t) M * M * M * M);
+    stage_alloc(&sp->d_out, n > (size_t) M * M ? n : (size_t) M * M);
+    h2d(sp->d_coef, psi->coef, sizeof(dcomplex) * n);
+    if (mctdhb_density_sync_dev(sp, sp->d_coef, sp->d_rho1, sp->d_rho2, sp->d_out, nullptr, g_reg_factor,
+                                MCTDHB_FP64, nullptr))
+        fail_stop("sync_density_matrices");
+    cmatrix_d2h(psi->ob_denmat, sp->d_rho1, M);
+    d2h(psi->tb_denmat, sp->d_rho2, sizeof(dcomplex) * M * M * M * M);
+    cmatrix_d2h(psi->inv_ob_denmat, sp->d_out, M);
+}
+
+extern "C" void sync_equation_params(OrbitalEquation eq) {
+    eq->g = eq->inter_param(eq->t, eq->inter_extra_args);
+    eq->pot_func(eq->t, eq->grid_size, eq->grid_pts, eq->pot_extra_args, eq->pot_grid);
+}
+
+// The orbital split step on the device from the host state; leaves the
+// orbitals, h and g of the new orbitals in the device staging of d.
+static void splitstep_dev_from_host(MCTDHBDataStruct m, DevOrbital* d) {
+    ManyBodyState psi = m->state;
+    const int M = psi->norb;
+    if (m->orb_integ_method != SPLITSTEP) {
+        printf("\n\nERROR: only the SPLITSTEP orbital integrator is provided\n\n");
+        exit(EXIT_FAILURE);
+    }
+    if (mctdhb_orbital_set_rk_order(d->o, m->rk_order)) fail_stop("Runge-Kutta order");
+    orbitals_h2d(d->d_orb, psi->orbitals, M, psi->grid_size);
+    h2d(d->d_rho2, psi->tb_denmat, sizeof(dcomplex) * M * M * M * M);
+    cmatrix_h2d(d->d_rinv, psi->inv_ob_denmat, M);
+    // orbital_integration.c:375-381: finite differences use Crank-Nicolson,
+    // the other derivative methods the FFT
+    const int use_fft = m->orb_der_method != FINITEDIFF;
+    if (mctdhb_splitstep_dev(d->o, use_fft, d->d_orb, d->d_hob, d->d_hint, d->d_rho2, d->d_rinv, nullptr))
+        fail_stop("propagate_splitstep_orb");
+}
+
+extern "C" void propagate_splitstep_orb(MCTDHBDataStruct m, Carray orb_next) {
+    DevOrbital* d = dev_orbital(m->orb_eq);
+    splitstep_dev_from_host(m, d);
+    ManyBodyState psi = m->state;
+    const int M = psi->norb;
+    orbitals_d2h(psi->orbitals, d->d_orb, M, psi->grid_size);
+    cmatrix_d2h(psi->hob, d->d_hob, M);
+    d2h(psi->hint, d->d_hint, sizeof(dcomplex) * M * M * M * M);
+    // orb_next is the reference's row-major scratch; it receives the new orbitals
+    d2h(orb_next, d->d_orb, sizeof(dcomplex) * M * psi->grid_size);
+}
+
+extern "C" void mctdhb_propagate_step(MCTDHBDataStruct m, Carray orb_works, Carray coef_works, uint32_t curr_step) {
+    ManyBodyState psi = m->state;
+    OrbitalEquation eq = m->orb_eq;
+    const int M = psi->norb, grid = psi->grid_size;
+    if (m->coef_integ_method != LANCZOS && M > 1) {
+        printf("\n\nERROR: only the LANCZOS coefficient integrator is provided\n\n");
+        exit(EXIT_FAILURE);
+    }
+    DevOrbital* d = dev_orbital(eq);
+    splitstep_dev_from_host(m, d);
+    // driver.c:176-186: new time, equation parameters, periodic bounds, h and g
+    eq->t = (curr_step + 1) * eq->tstep;
+    if (m->integ_type == REALTIME) {
+        sync_equation_params(eq);
+        if (mctdhb_orbital_set_potential(d->o, eq->pot_grid) || mctdhb_orbital_set_interaction(d->o, eq->g))
+            fail_stop("equation parameters");
+    }
+    if (eq->bounds == PERIODIC_BOUNDS && periodic_bounds(d->o->w, d->d_orb, nullptr)) fail_stop("periodic bounds");
+    if (mctdhb_orbital_matrices_dev(d->o, d->d_orb, d->d_hob, d->d_hint, nullptr)) fail_stop("orbital matrices");
+    orbitals_d2h(psi->orbitals, d->d_orb, M, grid);
+    cmatrix_d2h(psi->hob, d->d_hob, M);
+    d2h(psi->hint, d->d_hint, sizeof(dcomplex) * M * M * M * M);
+    d2h(orb_works, d->d_orb, sizeof(dcomplex) * M * grid);
+    if (M == 1) return;
+    propagate_coef_sil(m, coef_works);
+    sync_density_matrices(m->multiconfig_space, psi);
 }

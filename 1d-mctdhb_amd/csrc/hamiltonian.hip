@@ -18,7 +18,10 @@ unsigned hc_grid_blocks(uint32_t dim) { return div_up(dim, kHcBlockG); }
 
 // ------------------------------------------------------- coefficient block
 
-__global__ void k_hc_coefficients(int M, const double2* __restrict__ hob, const double2* __restrict__ hint,
+// With sum_m n_m = N the last occupation is folded into the single-jump
+// coefficients: A'_kl = A_kl + N B_kl,M-1 and B'_klm = B_klm - B_kl,M-1, so
+// F_kl(n) = A'_kl + sum_{m < M-1} n_m B'_klm.
+__global__ void k_hc_coefficients(int M, int N, const double2* __restrict__ hob, const double2* __restrict__ hint,
                                   double2* __restrict__ cf64, float2* __restrict__ cf32) {
     const int t = threadIdx.x;
     const int M2 = M * M, M3 = M2 * M;
@@ -30,17 +33,19 @@ __global__ void k_hc_coefficients(int M, const double2* __restrict__ hob, const 
     const int total = cf_size(M);
     for (int idx = t; idx < total; idx += blockDim.x) {
         double2 v = make_double2(0, 0);
-        if (idx < cf_off_B(M)) {  // A_kl = h_kl - g_kkkl
+        // B_klm = g_kkkl (m = k), g_klll (m = l), 2 g_kmml (otherwise)
+        auto B = [&](int k, int l, int m) {
+            if (m == k) return G(k, k, k, l);
+            if (m == l) return G(k, l, l, l);
+            return cscale(G(k, m, m, l), 2.0);
+        };
+        if (idx < cf_off_B(M)) {  // A'_kl = h_kl - g_kkkl + N B_kl,M-1
             int k = idx / M, l = idx % M;
-            if (k != l) v = hob[k * M + l] - G(k, k, k, l);
-        } else if (idx < cf_off_d1(M)) {  // B_klm
+            if (k != l) v = hob[k * M + l] - G(k, k, k, l) + cscale(B(k, l, M - 1), (double) N);
+        } else if (idx < cf_off_d1(M)) {  // B'_klm = B_klm - B_kl,M-1
             int r = idx - cf_off_B(M);
             int k = r / M2, l = (r / M) % M, m = r % M;
-            if (k != l) {
-                if (m == k) v = G(k, k, k, l);
-                else if (m == l) v = G(k, l, l, l);
-                else v = cscale(G(k, m, m, l), 2.0);
-            }
+            if (k != l && m < M - 1) v = B(k, l, m) - B(k, l, M - 1);
         } else if (idx < cf_off_d2(M)) {  // d1_k = h_kk - g_kkkk / 2
             int k = idx - cf_off_d1(M);
             v = hob[k * M + k] - cscale(G(k, k, k, k), 0.5);
@@ -67,7 +72,7 @@ __global__ void k_hc_coefficients(int M, const double2* __restrict__ hob, const 
 
 int hc_coefficients(const Space* S, const double2* hob, const double2* hint, double2* cf64, float2* cf32,
                     hipStream_t st) {
-    k_hc_coefficients<<<1, 256, 0, st>>>(S->norb, hob, hint, cf64, cf32);
+    k_hc_coefficients<<<1, 256, 0, st>>>(S->norb, S->npar, hob, hint, cf64, cf32);
     MCTDHB_HIP_CHECK(hipGetLastError());
     return kOk;
 }

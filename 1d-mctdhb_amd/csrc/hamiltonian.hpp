@@ -67,8 +67,9 @@ int hc_build_coef(const Space* S, const double2* hob, const double2* hint, void*
 int hc_apply_cf(const Space* S, const void* coef, const void* cf, void* hc, int prec, int index, hipStream_t st);
 
 // Lanczos form (iteration it): out = s_it H w_it - beta_{it-1} s_{it-1} w_{it-1}
-// (out is complex128) plus per-block partial sums, part[blk * nslot + 2k..2k+1]
-// = sum conj(s_k w_k) out for k <= it and, at it = 0, slot it+1 = |w_0|^2.
+// (out is complex128) plus per-block partial sums, slot-major: slot t of
+// block b at part[t * nblocks + b], slots 2k, 2k+1 = sum conj(s_k w_k) out
+// for k <= it and, at it = 0, slot 2(it+1) = |w_0|^2.
 // All scales come from the device SilState; the kernel is a no-op after a
 // Lanczos breakdown (st->stop).
 struct LanczosHcArgs {

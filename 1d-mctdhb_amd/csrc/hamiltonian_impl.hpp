@@ -13,15 +13,23 @@ constexpr int kHcBlock = 256;
 
 // ----------------------------------------------------------- fused row
 
-template <typename Z> __device__ __forceinline__ double2 load_d2(const Z* p, uint32_t j) { return to_d2(p[j]); }
+// log2 of the coefficient size: gathers use 32-bit byte offsets from a
+// uniform base (one VALU op per gather for the offset, none for a 64-bit
+// address), which needs dim * sizeof(Zin) < 2^32 (checked at launch).
+template <typename Zin> constexpr int coef_shift() { return sizeof(Zin) == 16 ? 4 : 3; }
 
-// Gather c[j] as the compute type.
-template <typename R, typename Zin> __device__ __forceinline__ cplx<R> gather(const Zin* __restrict__ c, uint32_t j);
-template <> __device__ __forceinline__ double2 gather<double, double2>(const double2* __restrict__ c, uint32_t j) {
-    return c[j];
+template <typename R, typename Zin>
+__device__ __forceinline__ cplx<R> gather_b(const Zin* __restrict__ c, uint32_t off) {
+    const Zin v = *(const Zin*) ((const char*) c + off);
+    return mk<R>((R) v.x, (R) v.y);
 }
-template <> __device__ __forceinline__ float2 gather<float, float2>(const float2* __restrict__ c, uint32_t j) {
-    return c[j];
+
+// acc += (G * t) for complex G (uniform) and t.
+template <typename R> __device__ __forceinline__ void cmac(cplx<R>& acc, const cplx<R>& G, const cplx<R>& t) {
+    acc.x += G.x * t.x;
+    acc.y += G.x * t.y;
+    acc.x -= G.y * t.y;
+    acc.y += G.y * t.x;
 }
 
 template <int M, typename R, int IDX, typename Zin>
@@ -29,29 +37,32 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
                                                 const cplx<R>* __restrict__ cf, const uint32_t* __restrict__ Wl,
                                                 int ldw, const R* __restrict__ sql) {
     using Z = cplx<R>;
-    Cfg<M, R, IDX> x;
-    x.load(i, ctx, Wl, ldw, sql);
+    Cfg<M, R, IDX, coef_shift<Zin>()> x;
+    x.template load_impl<true>(i, ctx, Wl, ldw, sql);
     const int* n = x.n;
+    R rn[M];
+#pragma unroll
+    for (int k = 0; k < M; k++) rn[k] = (R) n[k];
 
     // diagonal E(n)
     Z e = mk<R>(0, 0);
 #pragma unroll
     for (int k = 0; k < M; k++) {
-        const R nk = (R) n[k];
         const Z d1 = cf[cf_off_d1(M) + k], d2 = cf[cf_off_d2(M) + k];
-        e.x += nk * (d1.x + nk * d2.x);
-        e.y += nk * (d1.y + nk * d2.y);
+        e.x += rn[k] * (d1.x + rn[k] * d2.x);
+        e.y += rn[k] * (d1.y + rn[k] * d2.y);
 #pragma unroll
         for (int s = k + 1; s < M; s++) {
             const Z d3 = cf[cf_off_d3(M) + k * M + s];
-            const R ns = nk * (R) n[s];
+            const R ns = rn[k] * rn[s];
             e.x += ns * d3.x;
             e.y += ns * d3.y;
         }
     }
-    Z acc = cmul(e, gather<R, Zin>(c, i));
+    Z acc = cmul(e, gather_b<R, Zin>(c, x.is));
 
-    // single jumps k -> l
+    // single jumps k -> l: F_kl = A'_kl + sum_{m < M-1} n_m B'_klm (the last
+    // occupation folded in through sum_m n_m = N when the block is built)
 #pragma unroll
     for (int k = 0; k < M; k++) {
         if (n[k] >= 1) {
@@ -60,21 +71,30 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
                 if (l == k) continue;
                 Z F = cf[cf_off_A(M) + k * M + l];
 #pragma unroll
-                for (int m = 0; m < M; m++) {
+                for (int m = 0; m < M - 1; m++) {
                     const Z B = cf[cf_off_B(M) + (k * M + l) * M + m];
-                    F.x += (R) n[m] * B.x;
-                    F.y += (R) n[m] * B.y;
+                    F.x += rn[m] * B.x;
+                    F.y += rn[m] * B.y;
                 }
-                const R b = x.s0[k] * x.s1[l];
-                acc = cfma(cscale(F, b), gather<R, Zin>(c, x.single(k, l)), acc);
+                const Z t = cscale(gather_b<R, Zin>(c, x.single(k, l)), x.s0[k] * x.s1[l]);
+                cmac<R>(acc, F, t);
             }
         }
     }
-    // two particles out of the same orbital k
+    // creation factors of the two added particles, shared by every double
+    // jump into (q, l): sqrt((n_q+1)(n_l+1)), or sqrt((n_q+1)(n_q+2)) at q = l
+    R P[M][M];
+#pragma unroll
+    for (int q = 0; q < M; q++)
+#pragma unroll
+        for (int l = q; l < M; l++) P[q][l] = x.s1[q] * (q == l ? x.s2[q] : x.s1[l]);
+
+    // two particles out of the same orbital k, summed per k and scaled once
+    // by the annihilation factor sqrt(n_k (n_k - 1))
 #pragma unroll
     for (int k = 0; k < M; k++) {
         if (n[k] >= 2) {
-            const R a = x.sm[k] * x.s0[k];
+            Z g = mk<R>(0, 0);
 #pragma unroll
             for (int q = 0; q < M; q++) {
                 if (q == k) continue;
@@ -82,19 +102,21 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
                 for (int l = q; l < M; l++) {
                     if (l == k) continue;
                     const Z G = cf[cf_off_ot(M) + (k * M + q) * M + l];
-                    const R b = a * x.s1[q] * (q == l ? x.s2[q] : x.s1[l]);
-                    acc = cfma(cscale(G, b), gather<R, Zin>(c, x.same(k, q, l)), acc);
+                    cmac<R>(g, G, cscale(gather_b<R, Zin>(c, x.same(k, q, l)), P[q][l]));
                 }
             }
+            const R a = x.sm[k] * x.s0[k];
+            acc.x += a * g.x;
+            acc.y += a * g.y;
         }
     }
-    // one particle out of each of k < s
+    // one particle out of each of k < s, scaled once by sqrt(n_k n_s)
 #pragma unroll
     for (int k = 0; k < M; k++) {
 #pragma unroll
         for (int s = k + 1; s < M; s++) {
             if (n[k] >= 1 && n[s] >= 1) {
-                const R a = x.s0[k] * x.s0[s];
+                Z g = mk<R>(0, 0);
 #pragma unroll
                 for (int q = 0; q < M; q++) {
                     if (q == k || q == s) continue;
@@ -102,10 +124,12 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
                     for (int l = q; l < M; l++) {
                         if (l == k || l == s) continue;
                         const Z G = cf[cf_off_tt(M) + ((k * M + s) * M + q) * M + l];
-                        const R b = a * x.s1[q] * (q == l ? x.s2[q] : x.s1[l]);
-                        acc = cfma(cscale(G, b), gather<R, Zin>(c, x.diff(k, s, q, l)), acc);
+                        cmac<R>(g, G, cscale(gather_b<R, Zin>(c, x.diff(k, s, q, l)), P[q][l]));
                     }
                 }
+                const R a = x.s0[k] * x.s0[s];
+                acc.x += a * g.x;
+                acc.y += a * g.y;
             }
         }
     }
@@ -170,7 +194,7 @@ template <typename Zw> struct EpiLanczos {
 template <int M, typename R, int IDX, typename Zin, typename Epi>
 __global__ __launch_bounds__(kHcBlock) void k_hc_fused(RowCtx ctx, const Zin* __restrict__ c,
                                                        const cplx<R>* __restrict__ cf,
-                                                       const uint32_t* __restrict__ Wg, int ldw, int use_sq,
+                                                       const uint32_t* __restrict__ Wg, int ldw,
                                                        int stop_check, const SilState* st, Epi epi) {
     extern __shared__ __align__(16) unsigned char smem[];
     if (stop_check && st->stop) return;
@@ -181,15 +205,13 @@ __global__ __launch_bounds__(kHcBlock) void k_hc_fused(RowCtx ctx, const Zin* __
     if constexpr (IDX == kIdxFly) {
         for (int t = threadIdx.x; t < wsz; t += blockDim.x) Wl[t] = Wg[t];
     }
-    if (use_sq) {
-        for (int t = threadIdx.x; t < ctx.npar + 3; t += blockDim.x) sql[t] = sqrt((R) t);
-    }
+    for (int t = threadIdx.x; t < ctx.npar + 3; t += blockDim.x) sql[t] = sqrt((R) t);
     __syncthreads();
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t i = b * kHcBlock + threadIdx.x;
     const bool valid = i < ctx.dim;
     double2 v = make_double2(0, 0);
-    if (valid) v = to_d2(hc_row_fused<M, R, IDX, Zin>(i, ctx, c, cf, Wl, ldw, use_sq ? sql : nullptr));
+    if (valid) v = to_d2(hc_row_fused<M, R, IDX, Zin>(i, ctx, c, cf, Wl, ldw, sql));
     epi(i, valid, v, red);
 }
 
@@ -197,37 +219,35 @@ static inline RowCtx row_ctx(const Space* S) {
     return make_row_ctx(S->dim, S->npar, S->d_ht, S->d_map, S->d_mapot, S->d_maptt, S->d_strideot, S->d_stridett);
 }
 
-static inline bool sq_in_lds(const Space* S) { return S->npar + 3 <= 2048; }
-
 // Layout matches k_hc_fused: 64 B reduction scratch, the W table region
 // (reserved in both index modes so the sqrt LUT offset is the same), sqrt LUT.
-static inline size_t fused_smem(const Space* S, bool fly, bool use_sq, size_t rsize) {
-    (void) fly;
+static inline size_t fused_smem(const Space* S, size_t rsize) {
     size_t wsz = (size_t) (S->norb - 1) * S->ldw * 4;
-    size_t s = 64 + ((wsz + 15) & ~(size_t) 15);
-    if (use_sq) s += rsize * (S->npar + 3);
-    return s;
+    return 64 + ((wsz + 15) & ~(size_t) 15) + rsize * (S->npar + 3);
 }
 
 template <int M, typename R, typename Zin, typename Epi>
 static inline int launch_fused(const Space* S, const Zin* c, const cplx<R>* cf, int index, int stop_check,
                         const SilState* st, Epi epi, hipStream_t stream) {
     const bool fly = index == kIndexOnTheFly;
-    const bool use_sq = sq_in_lds(S);
-    const size_t smem = fused_smem(S, fly, use_sq, sizeof(R));
+    const size_t smem = fused_smem(S, sizeof(R));
     if (smem > 64 * 1024) {
-        set_error("on-the-fly index tables exceed the LDS budget for this (N, M); use table indexing");
+        set_error("rank and square-root tables exceed the LDS budget of the fused H.C kernel for this (N, M)");
+        return kInvalidArgument;
+    }
+    if ((uint64_t) S->dim * sizeof(Zin) >= (1ull << 32)) {
+        set_error("coefficient vector above 4 GiB: the fused H.C kernel needs 32-bit byte offsets");
         return kInvalidArgument;
     }
     const unsigned nb = hc_grid_blocks(S->dim);
     RowCtx ctx = row_ctx(S);
     if (fly) {
         k_hc_fused<M, R, kIdxFly, Zin, Epi><<<nb, kHcBlock, smem, stream>>>(
-            ctx, c, cf, S->d_W, S->ldw, use_sq, stop_check, st, epi);
+            ctx, c, cf, S->d_W, S->ldw, stop_check, st, epi);
     } else {
         if constexpr (std::is_same<R, double>::value)
             k_hc_fused<M, R, kIdxTables, Zin, Epi><<<nb, kHcBlock, smem, stream>>>(
-                ctx, c, cf, S->d_W, S->ldw, use_sq, stop_check, st, epi);
+                ctx, c, cf, S->d_W, S->ldw, stop_check, st, epi);
     }
     MCTDHB_HIP_CHECK(hipGetLastError());
     return kOk;

@@ -425,6 +425,70 @@ __global__ void k_rk4_combine(double2* __restrict__ y, const double2* __restrict
     }
 }
 
+// out = y0 + dt (a_0 k_0 + a_1 k_1 + ...), the stage sums of an explicit
+// tableau in ascending stage order.
+struct RkSum {
+    const double2* k[6];
+    double a[6];
+    int ns;
+};
+
+// (y0 and out may be the same array: each element is read before it is written.)
+__global__ void k_rk_sum(const double2* y0, RkSum s, double dt, double2* out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double2 acc = cscale(s.k[0][i], s.a[0]);
+    for (int j = 1; j < s.ns; j++) acc = acc + cscale(s.k[j][i], s.a[j]);
+    out[i] = y0[i] + cscale(acc, dt);
+}
+
+// Heun's 2nd-order and Butcher's 6-stage 5th-order tableaus (J. C. Butcher,
+// J. Austral. Math. Soc. 4 (1964) 179-194), row-major a[stage][j].
+constexpr double kRk2A[2 * 2] = {0, 0, 1, 0};
+constexpr double kRk2B[2] = {0.5, 0.5};
+constexpr double kRk5A[6 * 6] = {
+    0,        0,       0,        0,         0,       0,  //
+    0.25,     0,       0,        0,         0,       0,  //
+    0.125,    0.125,   0,        0,         0,       0,  //
+    0,        -0.5,    1,        0,         0,       0,  //
+    3.0 / 16, 0,       0,        9.0 / 16,  0,       0,  //
+    -3.0 / 7, 2.0 / 7, 12.0 / 7, -12.0 / 7, 8.0 / 7, 0,
+};
+constexpr double kRk5B[6] = {7.0 / 90, 0, 32.0 / 90, 12.0 / 90, 32.0 / 90, 7.0 / 90};
+
+static int rk_tableau(OrbWork* w, int ns, const double* A, const double* b, double2* orb, const double2* hob,
+                      const double2* hint, const double2* rho2, const double2* rinv, hipStream_t st) {
+    const OrbEq& q = w->eq;
+    const int total = q.norb * q.grid_size;
+    const unsigned nb = div_up(total, 256);
+    double2* k = w->d_stage;
+    double2* y = w->d_stage + 6 * (size_t) total;
+    int rc;
+    for (int s = 0; s < ns; s++) {
+        const double2* in = orb;
+        if (s > 0) {
+            RkSum r{};
+            for (int j = 0; j < s; j++) {
+                r.k[j] = k + (size_t) j * total;
+                r.a[j] = A[s * ns + j];
+            }
+            r.ns = s;
+            k_rk_sum<<<nb, 256, 0, st>>>(orb, r, q.tstep, y, total);
+            in = y;
+        }
+        if ((rc = nonlinear_rhs(w, in, hob, hint, rho2, rinv, k + (size_t) s * total, st))) return rc;
+    }
+    RkSum r{};
+    for (int j = 0; j < ns; j++) {
+        r.k[j] = k + (size_t) j * total;
+        r.a[j] = b[j];
+    }
+    r.ns = ns;
+    k_rk_sum<<<nb, 256, 0, st>>>(orb, r, q.tstep, orb, total);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
 // --------------------------------------------------------- Gram-Schmidt
 
 // One block: renormalise F_0, then for i >= 1 and j < i apply the
@@ -511,17 +575,24 @@ int splitstep(OrbWork* w, int use_fft, double2* orb, double2* hob, double2* hint
     if (rc) return rc;
     rc = orbital_matrices(w, orb, hob, hint, st);
     if (rc) return rc;
-    double2 *k1 = w->d_stage, *k2 = k1 + total, *k3 = k2 + total, *k4 = k3 + total, *y = k4 + total;
-    const double dt = q.tstep;
-    const unsigned nb = div_up(total, 256);
-    if ((rc = nonlinear_rhs(w, orb, hob, hint, rho2, rinv, k1, st))) return rc;
-    k_axpy_stage<<<nb, 256, 0, st>>>(orb, k1, 0.5 * dt, y, total);
-    if ((rc = nonlinear_rhs(w, y, hob, hint, rho2, rinv, k2, st))) return rc;
-    k_axpy_stage<<<nb, 256, 0, st>>>(orb, k2, 0.5 * dt, y, total);
-    if ((rc = nonlinear_rhs(w, y, hob, hint, rho2, rinv, k3, st))) return rc;
-    k_axpy_stage<<<nb, 256, 0, st>>>(orb, k3, dt, y, total);
-    if ((rc = nonlinear_rhs(w, y, hob, hint, rho2, rinv, k4, st))) return rc;
-    k_rk4_combine<<<nb, 256, 0, st>>>(orb, k1, k2, k3, k4, dt, total);
+    if (w->rk_order == 2) {
+        rc = rk_tableau(w, 2, kRk2A, kRk2B, orb, hob, hint, rho2, rinv, st);
+    } else if (w->rk_order == 5) {
+        rc = rk_tableau(w, 6, kRk5A, kRk5B, orb, hob, hint, rho2, rinv, st);
+    } else {
+        double2 *k1 = w->d_stage, *k2 = k1 + total, *k3 = k2 + total, *k4 = k3 + total, *y = k4 + total;
+        const double dt = q.tstep;
+        const unsigned nb = div_up(total, 256);
+        if ((rc = nonlinear_rhs(w, orb, hob, hint, rho2, rinv, k1, st))) return rc;
+        k_axpy_stage<<<nb, 256, 0, st>>>(orb, k1, 0.5 * dt, y, total);
+        if ((rc = nonlinear_rhs(w, y, hob, hint, rho2, rinv, k2, st))) return rc;
+        k_axpy_stage<<<nb, 256, 0, st>>>(orb, k2, 0.5 * dt, y, total);
+        if ((rc = nonlinear_rhs(w, y, hob, hint, rho2, rinv, k3, st))) return rc;
+        k_axpy_stage<<<nb, 256, 0, st>>>(orb, k3, dt, y, total);
+        if ((rc = nonlinear_rhs(w, y, hob, hint, rho2, rinv, k4, st))) return rc;
+        k_rk4_combine<<<nb, 256, 0, st>>>(orb, k1, k2, k3, k4, dt, total);
+    }
+    if (rc) return rc;
     rc = use_fft ? linear_fft(w, orb, st) : linear_cn(w, orb, st);
     if (rc) return rc;
     if (w->imagtime) rc = orthonormalize(w, orb, st);
@@ -747,7 +818,7 @@ int orbwork_create(const OrbEq& eq, int imagtime, const double* pot_host, OrbWor
     ok = ok && hipMalloc((void**) &w->d_fexp, sizeof(double2) * n) == hipSuccess;
     ok = ok && hipMalloc((void**) &w->d_tw, sizeof(double2) * n) == hipSuccess;
     ok = ok && hipMalloc((void**) &w->d_cn, sizeof(double2) * 3 * grid) == hipSuccess;
-    ok = ok && hipMalloc((void**) &w->d_stage, sizeof(double2) * 5 * (size_t) M * grid) == hipSuccess;
+    ok = ok && hipMalloc((void**) &w->d_stage, sizeof(double2) * 7 * (size_t) M * grid) == hipSuccess;
     ok = ok && hipMalloc((void**) &w->d_red, sizeof(double) * 1024) == hipSuccess;
     if (!ok) {
         orbwork_destroy(w);

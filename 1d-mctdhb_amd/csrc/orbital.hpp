@@ -20,12 +20,13 @@ struct OrbEq {
 struct OrbWork {
     OrbEq eq;
     int imagtime = 0;
+    int rk_order = 4;  // nonlinear sub-step: 2 (Heun), 4 (classical), 5 (Butcher, 6 stages)
     int nfft = 0, log2n = -1;  // FFT length grid_size - 1 (log2n < 0: not a power of two)
     double* d_pot = nullptr;     // grid_size
     double2* d_fexp = nullptr;   // nfft: exp(-i dt/2 (i a1 k - a2 k^2))
     double2* d_tw = nullptr;     // nfft / 2 twiddles exp(-2 pi i t / nfft)
     double2* d_cn = nullptr;     // 3 * grid_size: upper | lower | mid
-    double2* d_stage = nullptr;  // 5 * norb * grid_size: k1..k4, y
+    double2* d_stage = nullptr;  // 7 * norb * grid_size: up to six stages and y
     double* d_red = nullptr;     // reduction scratch
 };
 

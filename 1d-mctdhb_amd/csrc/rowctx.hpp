@@ -20,24 +20,37 @@ struct RowCtx {
 
 enum : int { kIdxTables = 0, kIdxFly = 1 };
 
-template <int M, typename R, int IDX> struct Cfg {
+// SHIFT scales every index the context returns by 2^SHIFT, so the H.C
+// kernel gets byte offsets (coefficient size 2^SHIFT) with no extra
+// arithmetic per gather; SHIFT = 0 returns plain configuration indices.
+template <int M, typename R, int IDX, int SHIFT = 0> struct Cfg {
     static constexpr int MD = M > 1 ? M - 1 : 1;
-    uint32_t i;
+    uint32_t i;   // configuration index
+    uint32_t is;  // i << SHIFT
     int n[M];
     R sm[M], s0[M], s1[M], s2[M];  // sqrt(n-1), sqrt(n), sqrt(n+1), sqrt(n+2)
     uint32_t Dm2[MD], Dm1[MD], Dp1[MD], Dp2[MD];
     uint32_t sot, stt;
     const RowCtx* ctx;
 
+    // sql: table of sqrt(0..N+2) in LDS, or null to compute the roots
     __device__ __forceinline__ void load(uint32_t idx, const RowCtx& c, const uint32_t* __restrict__ Wl, int ldw,
                                          const R* __restrict__ sql) {
+        if (sql) load_impl<true>(idx, c, Wl, ldw, sql);
+        else load_impl<false>(idx, c, Wl, ldw, sql);
+    }
+
+    template <bool LUT>
+    __device__ __forceinline__ void load_impl(uint32_t idx, const RowCtx& c, const uint32_t* __restrict__ Wl,
+                                              int ldw, const R* __restrict__ sql) {
         i = idx;
+        is = idx << SHIFT;
         ctx = &c;
 #pragma unroll
         for (int k = 0; k < M; k++) n[k] = c.ht[(size_t) i * M + k];
 #pragma unroll
         for (int k = 0; k < M; k++) {
-            if (sql) {
+            if constexpr (LUT) {
                 sm[k] = n[k] >= 1 ? sql[n[k] - 1] : R(0);
                 s0[k] = sql[n[k]];
                 s1[k] = sql[n[k] + 1];
@@ -56,10 +69,10 @@ template <int M, typename R, int IDX> struct Cfg {
                 P += n[m];
                 const uint32_t* Wm = Wl + m * ldw + kWOff + P;
                 const uint32_t w0 = Wm[0];
-                Dm2[m] = Wm[-2] - w0;
-                Dm1[m] = Wm[-1] - w0;
-                Dp1[m] = Wm[1] - w0;
-                Dp2[m] = Wm[2] - w0;
+                Dm2[m] = (Wm[-2] - w0) << SHIFT;
+                Dm1[m] = (Wm[-1] - w0) << SHIFT;
+                Dp1[m] = (Wm[1] - w0) << SHIFT;
+                Dp2[m] = (Wm[2] - w0) << SHIFT;
             }
         } else {
             sot = c.sot[i];
@@ -67,9 +80,10 @@ template <int M, typename R, int IDX> struct Cfg {
         }
     }
 
-    // rank of the configuration reached by removing r1, r2 and adding a1, a2
+    // rank (<< SHIFT) of the configuration reached by removing r1, r2 and
+    // adding a1, a2
     __device__ __forceinline__ uint32_t fly(int r1, int r2, int a1, int a2) const {
-        uint32_t j = i;
+        uint32_t j = is;
 #pragma unroll
         for (int m = 0; m < M - 1; m++) {
             const int d = jump_delta(m, r1, r2, a1, a2);
@@ -104,17 +118,17 @@ template <int M, typename R, int IDX> struct Cfg {
     // i - e_k + e_l
     __device__ __forceinline__ uint32_t single(int k, int l) const {
         if constexpr (IDX == kIdxFly) return fly(k, -1, l, -1);
-        else return ctx->map[i + (uint32_t) k * ctx->dim + (uint32_t) l * M * ctx->dim];
+        else return ctx->map[i + (uint32_t) k * ctx->dim + (uint32_t) l * M * ctx->dim] << SHIFT;
     }
     // i - 2 e_k + e_q + e_l
     __device__ __forceinline__ uint32_t same(int k, int q, int l) const {
         if constexpr (IDX == kIdxFly) return fly(k, k, q, l);
-        else return ctx->mapot[sot + ot_chunk(k) * (M * M) + q + l * M];
+        else return ctx->mapot[sot + ot_chunk(k) * (M * M) + q + l * M] << SHIFT;
     }
     // i - e_k - e_s + e_q + e_l (k < s)
     __device__ __forceinline__ uint32_t diff(int k, int s, int q, int l) const {
         if constexpr (IDX == kIdxFly) return fly(k, s, q, l);
-        else return ctx->maptt[stt + tt_chunk(k, s) * (M * M) + q + l * M];
+        else return ctx->maptt[stt + tt_chunk(k, s) * (M * M) + q + l * M] << SHIFT;
     }
 };
 

@@ -51,7 +51,7 @@ class _OperatorMappings(C.Structure):
 
 
 class _MultiConfiguration(C.Structure):
-    """reference _MultiConfiguration (mctdhb_types.h:161-169) + appended dev."""
+    """reference _MultiConfiguration (mctdhb_types.h:162-169) + appended dev."""
     _fields_ = [("npar", C.c_uint16), ("norb", C.c_uint16), ("dim", C.c_uint32),
                 ("hash_table", C.POINTER(C.c_uint16)), ("subspaces_dim", C.POINTER(C.c_uint32)),
                 ("op_maps", C.POINTER(_OperatorMappings)), ("dev", C.c_void_p)]
@@ -69,6 +69,15 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise MctdhbError(f"{LIB_PATH} is not built; run mctdhb_amd.build() (hipcc, gfx950)")
+    # torch ships its own libamdhip64 with the same soname (libamdhip64.so.7)
+    # but finds it by another name and path.  Loaded first, it is the one HIP
+    # runtime of the process: this library's dependency then resolves to it
+    # by soname.  Loaded after this library, torch would bring up a second
+    # runtime, which cannot open the GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     L.mctdhb_last_error.restype = C.c_char_p
     L.mctdhb_device_count.argtypes = [C.POINTER(_i)]
@@ -95,6 +104,8 @@ def lib():
                                         C.c_double, C.c_double, _i, _vp, _cpp]
     L.mctdhb_orbital_destroy.argtypes = [_vp]
     L.mctdhb_orbital_set_potential.argtypes = [_vp, _vp]
+    L.mctdhb_orbital_set_interaction.argtypes = [_vp, C.c_double]
+    L.mctdhb_orbital_set_rk_order.argtypes = [_vp, _i]
     L.mctdhb_linear_fft_dev.argtypes = [_vp, _vp, _vp]
     L.mctdhb_linear_cn_dev.argtypes = [_vp, _vp, _vp]
     L.mctdhb_orbital_matrices_dev.argtypes = [_vp, _vp, _vp, _vp, _vp]
@@ -241,13 +252,15 @@ class DeviceOrbital:
     _OrbitalWorkspace, mctdhb_types.h:136-149, :199-212) on the GPU."""
 
     def __init__(self, norb, grid_size, xi, xf, tstep, g, pot, imagtime=True, bounds=1001, d2coef=-0.5,
-                 d1coef=0j):
+                 d1coef=0j, rk_order=4):
         pot = np.ascontiguousarray(pot, np.float64)
         h = _vp()
         check(lib().mctdhb_orbital_create(norb, grid_size, bounds, xi, xf, tstep, g, d2coef,
                                           float(np.real(d1coef)), float(np.imag(d1coef)), int(bool(imagtime)),
                                           pot.ctypes.data, C.byref(h)))
         self._h, self.norb, self.grid_size, self.imagtime = h, norb, grid_size, imagtime
+        if rk_order != 4:
+            check(lib().mctdhb_orbital_set_rk_order(self._h, rk_order))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -259,6 +272,9 @@ class DeviceOrbital:
     @property
     def handle(self):
         return self._h
+
+    def set_interaction(self, g):
+        check(lib().mctdhb_orbital_set_interaction(self._h, float(g)))
 
     def set_potential(self, pot):
         pot = np.ascontiguousarray(pot, np.float64)

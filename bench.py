@@ -78,6 +78,32 @@ def cpu_baseline(npar, norb, lm, dt, steps, seed):
                       f"reference's table-driven H.C; {threads} OpenMP threads; table build excluded"}
 
 
+COPY_PROBE_BYTES = 512 << 20
+
+
+def probe_hc(args, space, C0, H, G, stream):
+    """Five launches of the mixed on-the-fly H.C kernel, then five of a
+    plain device copy of COPY_PROBE_BYTES (read and written once each), so a
+    FETCH_SIZE / WRITE_SIZE counter run can be scaled by what the same
+    counter reports for a copy of known size."""
+    import torch
+    import mctdhb_amd as amd
+
+    cf = torch.empty(space.coef_bytes(amd.MIXED), dtype=torch.uint8, device="cuda")
+    space.hc_coefficients(H, G, cf, amd.MIXED, stream=stream)
+    src = C0.to(torch.complex64)
+    out = torch.empty_like(src)
+    for _ in range(5):
+        space.apply_hamiltonian_cf(src, cf, out, amd.MIXED, amd.INDEX_FLY, stream=stream)
+    a = torch.empty(COPY_PROBE_BYTES // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "hc_mixed_fly", "hc_launches": 5, "copy_launches": 5,
+                      "copy_bytes_read": COPY_PROBE_BYTES, "copy_bytes_written": COPY_PROBE_BYTES}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +117,9 @@ def main():
     ap.add_argument("--hc-reps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe-hc", action="store_true",
+                    help="only the mixed H.C kernel (5 launches) and a plain copy kernel of known size, "
+                         "for hardware-counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -121,6 +150,10 @@ def main():
 
     def ev():
         return torch.cuda.Event(enable_timing=True)
+
+    if args.probe_hc:
+        probe_hc(args, space, C0, H, G, stream)
+        return
 
     # ---- H.C kernel alone (coefficient block prebuilt), three forms
     hc_ms = {}

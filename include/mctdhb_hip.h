@@ -60,6 +60,29 @@ void               destroy_lanczos_workspace(WorkspaceLanczos lan_work);        
 CoefWorkspace      get_coef_workspace(CoefIntegrator coef_integ_method, uint32_t space_dim,
                                       uint16_t lan_it);                          /* :181-198 */
 void               destroy_coef_workspace(CoefWorkspace coef_work);              /* :408-421 */
+OrbitalEquation    get_orbital_equation(char eq_name[], uint16_t norb, uint16_t grid_size,
+                                        BoundaryCondition bounds, IntegratorType integ_type, double xi,
+                                        double xf, double tstep, double tend, double d2coef, dcomplex d1coef,
+                                        void* pot_extra_args, void* inter_extra_args,
+                                        single_particle_pot pot_func,
+                                        time_dependent_parameter inter_param); /* :30-108 */
+void               destroy_orbital_equation(OrbitalEquation orbeq);             /* :339-345 */
+/* fft_desc holds the device workspace; dvr_mat is NULL (DVR not provided) */
+OrbitalWorkspace   get_orbital_workspace(OrbitalEquation eq_desc, OrbDerivative der_method); /* :200-244 */
+void               destroy_orbital_workspace(OrbitalWorkspace orb_work);        /* :391-406 */
+MCTDHBDataStruct   get_mctdhb_struct(IntegratorType integ_type, CoefIntegrator coef_integ_method,
+                                     OrbIntegrator orb_integ_method, OrbDerivative orb_der_method,
+                                     RungeKuttaOrder rk_order, uint16_t lanczos_iter, uint16_t npar,
+                                     uint16_t norb, char eq_name[], BoundaryCondition bounds, double xi,
+                                     double xf, uint16_t grid_size, double tstep, double tend, double d2coef,
+                                     dcomplex d1coef, void* pot_extra_args, void* inter_extra_args,
+                                     single_particle_pot pot_func,
+                                     time_dependent_parameter inter_param);  /* :246-314 */
+void               set_mctdhb_integrator(IntegratorType integ_type, CoefIntegrator coef_integ_method,
+                                         OrbIntegrator orb_integ_method, OrbDerivative orb_der_method,
+                                         RungeKuttaOrder rk_order, BoundaryCondition bounds,
+                                         uint16_t lanczos_iter, MCTDHBDataStruct mctdhb); /* :316-331 */
+void               destroy_mctdhb_struct(MCTDHBDataStruct mctdhb);              /* :422-432 */
 
 /* include/configurational/hamiltonian.h:7-13 */
 void apply_hamiltonian(MultiConfiguration multiconf, Carray coef, Cmatrix hob, Carray hint,
@@ -72,6 +95,32 @@ void set_twobody_dm(MultiConfiguration multiconf, Carray coef, Carray rho);     
 /* include/linalg/multiconfig_lanczos.h:31-39 */
 int lanczos(MultiConfiguration multiconf, Cmatrix hob_mat, Carray hint_mat, int iter, Carray diag,
             Carray offd, Cmatrix lvec);                                          /* multiconfig_lanczos.c:11-90 */
+
+/* include/linalg/multiconfig_lanczos.h:57-63 */
+double lowest_state_lanczos(uint16_t niter, MultiConfiguration multiconf, Cmatrix Ho, Carray Hint,
+                            Carray C);                                           /* multiconfig_lanczos.c:92-172 */
+
+/* include/integrator/split_linear_orbitals.h:59-62 */
+void advance_linear_crank_nicolson(OrbitalEquation eq_desc, OrbitalWorkspace orb_work,
+                                   Cmatrix orb);                                 /* split_linear_orbitals.c:335-373 */
+void advance_linear_fft(OrbitalWorkspace orb_work, Cmatrix orb);                 /* split_linear_orbitals.c:375-402 */
+
+/* include/integrator/synchronize.h:36-57 */
+void   set_regulatization_factor(double reg_fac);                                /* synchronize.c:11-26 (reference spelling) */
+double get_current_regularization_factor(void);                                  /* synchronize.c:28-32 */
+void   sync_orbital_matrices(OrbitalEquation eq_desc, ManyBodyState psi);        /* synchronize.c:34-39 */
+void   sync_density_matrices(MultiConfiguration mc_space, ManyBodyState psi);    /* synchronize.c:41-56 */
+void   sync_equation_params(OrbitalEquation eq_desc);                            /* synchronize.c:58-67 */
+
+/* include/integrator/orbital_integration.h:8, :37.  Orbital integrator
+ * SPLITSTEP only; rk_order RK2 / RK4 / RK5.  orb_next is scratch. */
+void set_periodic_bounds(uint16_t norb, uint16_t grid_size, Cmatrix orb);       /* orbital_integration.c:13-20 */
+void propagate_splitstep_orb(MCTDHBDataStruct mctdhb, Carray orb_next);          /* orbital_integration.c:351-434 */
+
+/* include/integrator/driver.h:66-71.  Coefficient integrator LANCZOS only;
+ * orb_works and coef_works are scratch. */
+void mctdhb_propagate_step(MCTDHBDataStruct mctdhb, Carray orb_works, Carray coef_works,
+                           uint32_t curr_step);                                  /* driver.c:158-202 */
 
 /* include/integrator/coefficients_integration.h:25-26 */
 void propagate_coef_sil(MCTDHBDataStruct mctdhb, Carray cnext);                  /* coefficients_integration.c:59-160 */
@@ -175,6 +224,12 @@ int mctdhb_orbital_create(int norb, int grid_size, int bounds, double xi, double
                           mctdhb_orbital** out);
 int mctdhb_orbital_destroy(mctdhb_orbital* w);
 int mctdhb_orbital_set_potential(mctdhb_orbital* w, const double* pot);
+/* Contact interaction g for later launches (synchronize.c:59-68 refreshes it
+ * from the time-dependent parameter in real time). */
+int mctdhb_orbital_set_interaction(mctdhb_orbital* w, double g);
+/* Order of the nonlinear Runge-Kutta sub-step: 2, 4 (default) or 5
+ * (RungeKuttaOrder, mctdhb_types.h:80-85; orbital_integration.c:384-416). */
+int mctdhb_orbital_set_rk_order(mctdhb_orbital* w, int order);
 /* Device arrays: orb norb x grid_size, hob M x M, hint M^4, rho2 M^4,
  * rinv M x M (inverse one-body density matrix), all complex128. */
 int mctdhb_linear_fft_dev(mctdhb_orbital* w, void* orb, void* stream);       /* split_linear_orbitals.c:375-402 */

@@ -50,7 +50,7 @@ class OrbEq(C.Structure):
         ("d1coef_re", C.c_double), ("d1coef_im", C.c_double),
         ("prop_dt_re", C.c_double), ("prop_dt_im", C.c_double),
         ("time_fac_re", C.c_double), ("time_fac_im", C.c_double),
-        ("pot", C.POINTER(C.c_double)),
+        ("pot", C.POINTER(C.c_double)), ("rk_order", C.c_int),
     ]
 
 
@@ -114,6 +114,7 @@ def lib():
         L.orc_herm_inverse.restype = C.c_int
         L.orc_regularization.argtypes = [C.c_int, C.c_double, _cp]
         L.orc_sync_density.argtypes = [sp, _cp, _cp, _cp, _cp, C.c_double]
+        L.orc_rk_nonlinear.argtypes = [oe, _cp, _cp, _cp, _cp, _cp]
         L.orc_splitstep_orb.argtypes = [oe, C.c_int, C.c_int, _cp, _cp, _cp, _cp, _cp]
         L.orc_propagate_step.argtypes = [sp, oe, C.c_int, C.c_int, C.c_int, _cp, _cp, _cp,
                                          _cp, _cp, _cp, _cp, _cp]
@@ -237,7 +238,7 @@ def tridiag_eig(d, e):
 
 
 def make_orbeq(norb, grid_size, xi, xf, tstep, g, pot, imagtime=True, bounds=1001,
-               d2coef=-0.5, d1coef=0j):
+               d2coef=-0.5, d1coef=0j, rk_order=4):
     """Fill an ``OrbEq`` as reference get_orbital_equation does
     (types_definition.c:30-108: dx = (xf - xi)/(grid_size - 1); imaginary
     time uses prop_dt = -i dt and time_fac = -i)."""
@@ -255,6 +256,7 @@ def make_orbeq(norb, grid_size, xi, xf, tstep, g, pot, imagtime=True, bounds=100
         eq.prop_dt_re, eq.prop_dt_im = tstep, 0.0
         eq.time_fac_re, eq.time_fac_im = 1.0, 0.0
     eq.pot = pot.ctypes.data_as(C.POINTER(C.c_double))
+    eq.rk_order = rk_order
     eq._pot_keep = pot  # keep the buffer alive with the struct
     return eq
 

@@ -84,6 +84,7 @@ typedef struct
     double   prop_dt_re, prop_dt_im;   /* imag time: (0,-dt); real time: (dt,0) */
     double   time_fac_re, time_fac_im; /* imag time: (0,-1);  real time: (1,0)  */
     const double* pot;                 /* grid_size */
+    int      rk_order;                 /* nonlinear sub-step: 2, 4 (also 0) or 5 */
 } orc_orbeq;
 
 ocplx orc_csimps(uint32_t n, double h, const ocplx* f);
@@ -114,6 +115,8 @@ int   orc_herm_inverse(int n, const ocplx* A, ocplx* Ainv);
 void  orc_regularization(int n, double x, ocplx* A);
 void  orc_sync_density(const orc_space* S, const ocplx* coef, ocplx* rho1, ocplx* rho2,
                        ocplx* rinv, double reg_factor);
+void  orc_rk_nonlinear(const orc_orbeq* eq, ocplx* orb, const ocplx* hob, const ocplx* hint,
+                       const ocplx* rho2, const ocplx* rinv);
 void  orc_splitstep_orb(const orc_orbeq* eq, int imagtime, int use_fft, ocplx* orb,
                         ocplx* hob, ocplx* hint, const ocplx* rho2, const ocplx* rinv);
 void  orc_propagate_step(const orc_space* S, orc_orbeq* eq, int imagtime, int use_fft,

@@ -2,7 +2,7 @@
  * orbital.c (oracle) - the orbital split-step and everything it calls:
  * Simpson quadrature, finite differences, Gram-Schmidt, h_kl and g_ksql
  * integrals, Crank-Nicolson and FFT linear half-steps, the nonlinear
- * right-hand side, a classical RK4 nonlinear step, the M x M Hermitian
+ * right-hand side, the RK2/RK4/RK5 nonlinear step, the M x M Hermitian
  * eigen/inverse pieces of the density sync, and one full MCTDHB step.
  * TEST INFRASTRUCTURE ONLY.
  *
@@ -14,7 +14,8 @@
  *    transform for odd lengths and radix-2 decimation for powers of two,
  *    with DFTI's 1/sqrt(n) forward and backward scale;
  *  - odelib cplx_rungekutta{2,4,5} (orbital_integration.c:384-416) ->
- *    the classical 4-stage RK4 tableau;
+ *    Heun's 2nd-order, the classical 4th-order and Butcher's 6-stage
+ *    5th-order tableaus (only their convergence orders are checked);
  *  - LAPACKE zheev / zhesv (lapack_interface.c:10-124) -> cyclic complex
  *    Jacobi rotations and Gauss-Jordan elimination with partial pivoting.
  */
@@ -849,6 +850,70 @@ rk4_nonlinear(const orc_orbeq* eq, ocplx* orb, const ocplx* hob, const ocplx* hi
     free(y);
 }
 
+/* The 2nd- and 5th-order explicit tableaus: Heun's trapezoidal method and
+ * Butcher's six-stage fifth-order method (J. C. Butcher, J. Austral. Math.
+ * Soc. 4 (1964) 179-194).  odelib's own tableaus are not available. */
+static const double RK2_A[2 * 2] = {0, 0, 1, 0};
+static const double RK2_B[2] = {0.5, 0.5};
+static const double RK5_A[6 * 6] = {
+    0,          0,         0,          0,           0,         0,
+    0.25,       0,         0,          0,           0,         0,
+    0.125,      0.125,     0,          0,           0,         0,
+    0,          -0.5,      1,          0,           0,         0,
+    3.0 / 16,   0,         0,          9.0 / 16,    0,         0,
+    -3.0 / 7,   2.0 / 7,   12.0 / 7,   -12.0 / 7,   8.0 / 7,   0,
+};
+static const double RK5_B[6] = {7.0 / 90, 0, 32.0 / 90, 12.0 / 90, 32.0 / 90, 7.0 / 90};
+
+/* One explicit RK step with an s-stage tableau; stage sums run over j in
+ * ascending order: y = y0 + dt * (a_0 k_0 + a_1 k_1 + ...). */
+static void
+tableau_nonlinear(const orc_orbeq* eq, int s, const double* A, const double* b, ocplx* orb,
+                  const ocplx* hob, const ocplx* hint, const ocplx* rho2, const ocplx* rinv)
+{
+    const size_t n = (size_t) eq->norb * eq->grid_size;
+    const double dt = eq->tstep;
+    ocplx* k = malloc(sizeof(ocplx) * n * s);
+    ocplx* y = malloc(sizeof(ocplx) * n);
+    for (int st = 0; st < s; st++)
+    {
+        if (st == 0)
+        {
+            orc_nonlinear_rhs(eq, orb, hob, hint, rho2, rinv, k);
+            continue;
+        }
+        for (size_t i = 0; i < n; i++)
+        {
+            ocplx acc = A[st * s] * k[i];
+            for (int j = 1; j < st; j++) acc += A[st * s + j] * k[j * n + i];
+            y[i] = orb[i] + dt * acc;
+        }
+        orc_nonlinear_rhs(eq, y, hob, hint, rho2, rinv, k + st * n);
+    }
+    for (size_t i = 0; i < n; i++)
+    {
+        ocplx acc = b[0] * k[i];
+        for (int j = 1; j < s; j++) acc += b[j] * k[j * n + i];
+        orb[i] = orb[i] + dt * acc;
+    }
+    free(k);
+    free(y);
+}
+
+/* The nonlinear sub-step of order eq->rk_order (reference
+ * orbital_integration.c:384-416 switches on RK2 / RK4 / RK5). */
+void
+orc_rk_nonlinear(const orc_orbeq* eq, ocplx* orb, const ocplx* hob, const ocplx* hint,
+                 const ocplx* rho2, const ocplx* rinv)
+{
+    if (eq->rk_order == 2)
+        tableau_nonlinear(eq, 2, RK2_A, RK2_B, orb, hob, hint, rho2, rinv);
+    else if (eq->rk_order == 5)
+        tableau_nonlinear(eq, 6, RK5_A, RK5_B, orb, hob, hint, rho2, rinv);
+    else
+        rk4_nonlinear(eq, orb, hob, hint, rho2, rinv);
+}
+
 static void
 periodic_bounds(const orc_orbeq* eq, ocplx* orb)
 {
@@ -870,7 +935,7 @@ orc_splitstep_orb(const orc_orbeq* eq, int imagtime, int use_fft, ocplx* orb, oc
         orc_advance_linear_cn(eq, orb);
     orc_orbital_hob(eq, orb, hob);
     orc_orbital_hint(eq, orb, hint);
-    rk4_nonlinear(eq, orb, hob, hint, rho2, rinv);
+    orc_rk_nonlinear(eq, orb, hob, hint, rho2, rinv);
     if (use_fft)
         orc_advance_linear_fft(eq, orb);
     else

@@ -10,7 +10,9 @@ _hip = None
 def hip():
     global _hip
     if _hip is None:
-        _hip = C.CDLL("libamdhip64.so")
+        import torch  # noqa: F401  (torch's HIP runtime first, then bind it by soname)
+
+        _hip = C.CDLL("libamdhip64.so.7")
         _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
         _hip.hipDeviceSynchronize.argtypes = []
     return _hip

@@ -159,12 +159,12 @@ def test_lowest_state_matches_oracle(amd, orc, npar, norb, lm, prec):
     dc = gu.dev(c)
     e = sil.lowest_state(dc, gu.dev(h), gu.dev(g))
     tol = 1e-10 if prec == amd.FP64 else 1e-5
-    assert abs(e - e_ref) <= tol * abs(e_ref)
+    assert abs(e - e_ref) <= tol * abs(e_ref)  # mixed: complex64 Krylov vectors
     assert gu.rel_err(gu.host(dc), ref) < tol
     # the Ritz value is the Rayleigh quotient of the (unit) Ritz vector
     v = gu.host(dc)
     hv = S.apply_hamiltonian(v, h, g)
-    assert abs(np.vdot(v, hv).real / np.vdot(v, v).real - e_ref) < 1e-8 * abs(e_ref)
+    assert abs(np.vdot(v, hv).real / np.vdot(v, v).real - e_ref) < 100 * tol * abs(e_ref)
 
 
 def test_lanczos_breakdown_on_device(amd, orc):

@@ -21,11 +21,13 @@ def amd():
     return mctdhb_amd
 
 
-def _setup(orc, amd, norb, grid, imagtime, bounds=1001, L=16.0, g=1.3, dt=0.01, seed=5, d1coef=0j):
+def _setup(orc, amd, norb, grid, imagtime, bounds=1001, L=16.0, g=1.3, dt=0.01, seed=5, d1coef=0j, rk_order=4):
     x = np.linspace(-L / 2, L / 2, grid)
     pot = 0.5 * x ** 2
-    eq = orc.make_orbeq(norb, grid, x[0], x[-1], dt, g, pot, imagtime=imagtime, bounds=bounds, d1coef=d1coef)
-    w = amd.DeviceOrbital(norb, grid, x[0], x[-1], dt, g, pot, imagtime=imagtime, bounds=bounds, d1coef=d1coef)
+    eq = orc.make_orbeq(norb, grid, x[0], x[-1], dt, g, pot, imagtime=imagtime, bounds=bounds, d1coef=d1coef,
+                        rk_order=rk_order)
+    w = amd.DeviceOrbital(norb, grid, x[0], x[-1], dt, g, pot, imagtime=imagtime, bounds=bounds, d1coef=d1coef,
+                          rk_order=rk_order)
     orb = np.ascontiguousarray(inputs.hermite_gauss(norb, x, seed=seed, noise=0.02).ravel())
     if bounds == 1001:
         orb.reshape(norb, grid)[:, -1] = orb.reshape(norb, grid)[:, 0]
@@ -130,12 +132,13 @@ def test_density_sync_regularises_a_condensate(orc, amd):
     assert gu.rel_err(gu.host(di), rinv.ravel()) < 1e-8
 
 
-@pytest.mark.parametrize("use_fft", [True, False])
-@pytest.mark.parametrize("imagtime", [False, True])
-def test_splitstep(orc, amd, use_fft, imagtime):
-    print(f"splitstep fft={use_fft} imag={imagtime}", flush=True)
+@pytest.mark.parametrize("use_fft,imagtime,rk", [(True, False, 4), (True, True, 4), (False, False, 4),
+                                                 (False, True, 4), (True, False, 2), (True, True, 5),
+                                                 (False, False, 5)])
+def test_splitstep(orc, amd, use_fft, imagtime, rk):
+    print(f"splitstep fft={use_fft} imag={imagtime} rk={rk}", flush=True)
     npar, norb = 6, 3
-    x, eq, w, orb = _setup(orc, amd, norb, 129, imagtime)
+    x, eq, w, orb = _setup(orc, amd, norb, 129, imagtime, rk_order=rk)
     S, c, r1, r2, rinv = _densities(orc, npar, norb)
     hob = np.zeros(norb * norb, complex)
     hint = np.zeros(norb ** 4, complex)

@@ -314,3 +314,39 @@ def test_ground_state_energy_known_answer(orc, npar, norb, g, energy, condensate
     e, occ = _ground_state(orc, npar, norb, g)
     assert abs(e - energy) < 3e-4, e
     assert abs(occ - condensate) < 0.01
+
+
+@pytest.mark.parametrize("order,expect,h0", [(2, 8.0, 0.1), (4, 32.0, 0.1), (5, 64.0, 0.025)])
+def test_rk_nonlinear_local_error_order(orc, order, expect, h0):
+    # one explicit step of order p has local error C h^(p+1): halving h
+    # divides it by 2^(p+1) once h is in the asymptotic range.  The exact
+    # step is approximated by 128 RK4 sub-steps.
+    npar, norb = 5, 3
+    x = np.linspace(-6, 6, 97)
+    orb0 = np.ascontiguousarray(inputs.hermite_gauss(norb, x, seed=3, noise=0.02).ravel())
+    S = orc.Space(npar, norb)
+    r1, r2, rinv = S.sync_density(inputs.coefficients(S.dim, 6))
+    rinv = np.ascontiguousarray(rinv.ravel())
+
+    def matrices(eq):
+        hob = np.zeros(norb * norb, complex)
+        hint = np.zeros(norb ** 4, complex)
+        orc.lib().orc_orbital_hob(eq, orb0, hob)
+        orc.lib().orc_orbital_hint(eq, orb0, hint)
+        return hob, hint
+
+    def step(h, rk, substeps=1):
+        eq = orc.make_orbeq(norb, x.size, x[0], x[-1], h / substeps, 1.3, 0.5 * x ** 2, imagtime=False,
+                            rk_order=rk)
+        hob, hint = matrices(eq)
+        f = orb0.copy()
+        for _ in range(substeps):
+            orc.lib().orc_rk_nonlinear(eq, f, hob, hint, r2, rinv)
+        return f
+
+    errs = []
+    for h in (h0, h0 / 2):
+        exact = step(h, 4, substeps=128)
+        errs.append(np.max(np.abs(step(h, order) - exact)))
+    ratio = errs[0] / errs[1]
+    assert 0.75 * expect < ratio < 1.35 * expect, (errs, ratio)
