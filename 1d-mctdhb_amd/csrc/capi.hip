@@ -772,15 +772,18 @@ extern "C" void set_twobody_dm(MultiConfiguration mc, Carray coef, Carray rho) {
     d2h(rho, sp->d_rho2, sizeof(dcomplex) * M * M * M * M);
 }
 
-static mctdhb_sil* host_sil(mctdhb_space* sp, int lm) {
+// The cached FP64 Krylov workspace of the host entry points; max_lm is the
+// caller's bound (MAX_LANCZOS_ITER for the SIL step, the device capacity for
+// the ground-state diagonalisation).
+static mctdhb_sil* host_sil(mctdhb_space* sp, int lm, int max_lm = MAX_LANCZOS_ITER) {
     if (sp->sil && sp->sil->L->lm != lm) {
         mctdhb_sil_destroy(sp->sil);
         sp->sil = nullptr;
     }
     if (!sp->sil) {
-        if (lm < MIN_LANCZOS_ITER || lm > MAX_LANCZOS_ITER) {
+        if (lm < MIN_LANCZOS_ITER || lm > max_lm) {
             printf("\n\nERROR: Lanczos iterations must be between %d and %d but %d was requested\n\n",
-                   MIN_LANCZOS_ITER, MAX_LANCZOS_ITER, lm);
+                   MIN_LANCZOS_ITER, max_lm, lm);
             exit(EXIT_FAILURE);
         }
         if (mctdhb_sil_create(sp, lm, MCTDHB_FP64, MCTDHB_INDEX_FLY, &sp->sil)) fail_stop("Lanczos workspace");
@@ -792,7 +795,7 @@ extern "C" int lanczos(MultiConfiguration mc, Cmatrix hob, Carray hint, int lm, 
                        Cmatrix lvec) {
     mctdhb_space* sp = ensure_dev(mc);
     const size_t n = sp->S->dim;
-    mctdhb_sil* s = host_sil(sp, lm);
+    mctdhb_sil* s = host_sil(sp, lm, kMaxKrylov);
     stage_alloc(&sp->d_coef, n);
     stage_hob_hint(sp, hob, hint);
     if (sp->d_lvec) {
@@ -822,7 +825,7 @@ extern "C" int lanczos(MultiConfiguration mc, Cmatrix hob, Carray hint, int lm, 
 extern "C" double lowest_state_lanczos(uint16_t niter, MultiConfiguration mc, Cmatrix Ho, Carray Hint, Carray C) {
     mctdhb_space* sp = ensure_dev(mc);
     const size_t n = sp->S->dim;
-    mctdhb_sil* s = host_sil(sp, niter);
+    mctdhb_sil* s = host_sil(sp, niter, kMaxKrylov);
     stage_alloc(&sp->d_coef, n);
     stage_hob_hint(sp, Ho, Hint);
     h2d(sp->d_coef, C, sizeof(dcomplex) * n);

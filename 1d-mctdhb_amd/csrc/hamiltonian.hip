@@ -7,7 +7,7 @@
 // targets come either from the reference tables or from the closed-form rank
 // deltas of fock.hpp (W table staged in LDS).  See hamiltonian.hpp for the
 // rule grouping.
-#include "hamiltonian.hpp"
+#include "hamiltonian_impl.hpp"
 #include "rowctx.hpp"
 
 namespace mctdhb {
@@ -216,50 +216,6 @@ template <typename Zout> struct EpiPlainG {
     }
 };
 
-struct EpiLanczosG {
-    LanczosHcArgs a;
-    __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 row, double* lds) const {
-        const SilState* st = a.st;
-        const int it = a.it;
-        double2 o = make_double2(0, 0);
-        if (valid) {
-            o = cscale(row, st->s[it]);
-            if (it > 0) {
-                const double bp = st->beta[it - 1] * st->s[it - 1];
-                o = o - cscale(to_d2(((const double2*) a.w[it - 1])[i]), bp);
-            }
-            a.out[i] = o;
-        }
-        // slot-major partials: slot t of block b at part[t * gridDim.x + b]
-        double* part = a.part + blockIdx.x;
-        const size_t ps = gridDim.x;
-        for (int k = 0; k <= it; k++) {
-            double v[2] = {0, 0};
-            if (valid) {
-                const double2 wk = to_d2(((const double2*) a.w[k])[i]);
-                const double2 p = cconjmul(wk, o);
-                v[0] = p.x * st->s[k];
-                v[1] = p.y * st->s[k];
-            }
-            block_sum<2>(v, lds);
-            if (threadIdx.x == 0) {
-                part[(2 * k) * ps] = v[0];
-                part[(2 * k + 1) * ps] = v[1];
-            }
-        }
-        double v1[1] = {0};
-        if (valid && it == 0) {
-            const double2 w0 = to_d2(((const double2*) a.w[0])[i]);
-            v1[0] = w0.x * w0.x + w0.y * w0.y;
-        }
-        block_sum<1>(v1, lds);
-        if (threadIdx.x == 0) {
-            part[(2 * (it + 1)) * ps] = v1[0];
-            part[(2 * (it + 1) + 1) * ps] = 0;
-        }
-    }
-};
-
 template <typename Epi>
 __global__ __launch_bounds__(kHcBlockG) void k_hc_generic(int M, RowCtx ctx, const double2* __restrict__ c,
                                                          const double2* __restrict__ hob,
@@ -397,7 +353,7 @@ int hc_lanczos(const Space* S, const double2* cf64, const float2* cf32, const La
         // generic path is FP64: hob/hint are carried in cf64 as [hob | hint]
         const double2* hob = cf64;
         const double2* hint = cf64 + S->norb * S->norb;
-        EpiLanczosG epi{a};
+        EpiLanczos<double2> epi{a};
         k_hc_generic<<<hc_grid_blocks(S->dim), kHcBlockG, 0, st>>>(S->norb, row_ctx_g(S),
                                                                    (const double2*) a.w[a.it], hob, hint, 1,
                                                                    a.st, epi);

@@ -73,8 +73,9 @@ int hc_apply_cf(const Space* S, const void* coef, const void* cf, void* hc, int 
 // All scales come from the device SilState; the kernel is a no-op after a
 // Lanczos breakdown (st->stop).
 struct LanczosHcArgs {
-    const void* w[kMaxLanczos];
+    const void* w[kMaxKrylov];
     int it;
+    int last;  // final iteration: only alpha_it (slots 2it, 2it+1), out not stored
     const SilState* st;
     double2* out;
     double* part;

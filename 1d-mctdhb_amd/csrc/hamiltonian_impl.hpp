@@ -145,6 +145,8 @@ template <typename Zout> struct EpiPlain {
     }
 };
 
+// The Lanczos epilogue.  On the last iteration (a.last) only alpha_it is
+// needed: out is not stored and only the slot pair of k = it is written.
 template <typename Zw> struct EpiLanczos {
     LanczosHcArgs a;
     __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 row, double* lds) const {
@@ -157,12 +159,12 @@ template <typename Zw> struct EpiLanczos {
                 const double bp = st->beta[it - 1] * st->s[it - 1];
                 o = o - cscale(to_d2(((const Zw*) a.w[it - 1])[i]), bp);
             }
-            a.out[i] = o;
+            if (!a.last) a.out[i] = o;
         }
         // slot-major partials: slot t of block b at part[t * gridDim.x + b]
         double* part = a.part + blockIdx.x;
         const size_t ps = gridDim.x;
-        for (int k = 0; k <= it; k++) {
+        for (int k = a.last ? it : 0; k <= it; k++) {
             double v[2] = {0, 0};
             if (valid) {
                 const double2 wk = to_d2(((const Zw*) a.w[k])[i]);
@@ -176,6 +178,7 @@ template <typename Zw> struct EpiLanczos {
                 part[(2 * k + 1) * ps] = v[1];
             }
         }
+        if (a.last) return;
         double v1[1] = {0};
         if (valid && it == 0) {
             const double2 w0 = to_d2(((const Zw*) a.w[0])[i]);

@@ -30,7 +30,7 @@ constexpr double kBreakdownTol = 1e-14;  // reference multiconfig_lanczos.c:34
 
 __global__ void k_sil_init(SilState* st, int lm) {
     if (threadIdx.x != 0) return;
-    for (int k = 0; k <= kMaxLanczos; k++) {
+    for (int k = 0; k <= kMaxKrylov; k++) {
         st->s[k] = 0;
         st->beta[k] = 0;
         st->alpha[k] = make_double2(0, 0);
@@ -43,6 +43,7 @@ __global__ void k_sil_init(SilState* st, int lm) {
     st->nrm2 = 0;
     st->eground = 0;
     st->iter_done = lm;
+    st->arrive = 0;
     st->stop = 0;
     st->eig_fail = 0;
 }
@@ -63,11 +64,12 @@ __global__ __launch_bounds__(256) void k_reduce_slots(const double* __restrict__
 
 // After the H.C pass of iteration it: alpha_it and the reorthogonalisation
 // coefficients (on w_k) for the next streaming pass.
-__global__ void k_sil_after_hc(SilState* st, const double* __restrict__ tot, int it) {
-    if (threadIdx.x != 0 || st->stop) return;
+// On the last iteration only alpha is formed.
+__device__ void sil_after_hc(SilState* st, const volatile double* tot, int it, int last) {
     if (it == 0) st->gram[0] = make_double2(tot[2 * (it + 1)], 0);  // <v_0, v_0> = |C|^2
     const double2 alpha = make_double2(tot[2 * it], tot[2 * it + 1]);
     st->alpha[it] = alpha;
+    if (last) return;
     for (int k = 0; k <= it; k++) {
         const double2 d = make_double2(tot[2 * k], tot[2 * k + 1]);
         double2 ortho = d - cmul(alpha, st->gram[k]);
@@ -79,8 +81,7 @@ __global__ void k_sil_after_hc(SilState* st, const double* __restrict__ tot, int
 // After the streaming pass that produced w_{it+1}: beta_it, the breakdown
 // test of the reference (beta / max beta < 1e-14 ends the recursion with
 // it + 1 vectors) and the Gram column <v_k, v_{it+1}>.
-__global__ void k_sil_after_reorth(SilState* st, const double* __restrict__ tot, int it) {
-    if (threadIdx.x != 0 || st->stop) return;
+__device__ void sil_after_reorth(SilState* st, const volatile double* tot, int it) {
     const double nrm2 = tot[2 * (it + 1)];
     const double beta = sqrt(nrm2);
     st->beta[it] = beta;
@@ -95,6 +96,34 @@ __global__ void k_sil_after_reorth(SilState* st, const double* __restrict__ tot,
     for (int k = 0; k <= it; k++)
         st->gram[k] = cscale(make_double2(tot[2 * k], tot[2 * k + 1]), st->s[k] * sn);
     st->gram[it + 1] = make_double2(nrm2 * sn * sn, 0);
+}
+
+// k_reduce_slots over slots slot0 + blockIdx.x, then (in the last block to
+// finish) the scalar update OP of iteration it: 0 = sil_after_hc, 1 =
+// sil_after_reorth.  Blocks count themselves in st->arrive (a vector
+// atomic); the last one resets it.
+template <int OP>
+__global__ __launch_bounds__(256) void k_reduce_scalars(const double* __restrict__ part, int nblocks, double* tot,
+                                                        SilState* st, int it, int slot0, int final_it) {
+    __shared__ double red[4];
+    __shared__ int last;
+    if (st->stop) return;  // set only by a previous kernel: uniform over the grid
+    const int t = slot0 + blockIdx.x;
+    double v[1] = {0};
+    const double* p = part + (size_t) t * nblocks;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) v[0] += p[b];
+    block_sum<1>(v, red);
+    if (threadIdx.x == 0) {
+        tot[t] = v[0];
+        __threadfence();
+        last = atomicAdd(&st->arrive, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last || threadIdx.x != 0) return;
+    __threadfence();
+    st->arrive = 0;
+    if (OP == 0) sil_after_hc(st, tot, it, final_it);
+    else sil_after_reorth(st, tot, it);
 }
 
 // exp(-i T prop_dt) e_1 from the eigen-decomposition of the real tridiagonal
@@ -154,8 +183,8 @@ __device__ int tridiag_eig_dev(int n, double* d, double* e, double* z) {
 __global__ void k_sil_tridiag(SilState* st, double dt_re, double dt_im) {
     // LDS rather than per-thread arrays: a lone thread's dynamically
     // indexed arrays would live in scratch memory
-    __shared__ double d[kMaxLanczos + 1], e[kMaxLanczos + 1], z[(kMaxLanczos + 1) * (kMaxLanczos + 1)];
-    __shared__ double2 aux[kMaxLanczos + 1];
+    __shared__ double d[kMaxKrylov + 1], e[kMaxKrylov + 1], z[(kMaxKrylov + 1) * (kMaxKrylov + 1)];
+    __shared__ double2 aux[kMaxKrylov + 1];
     if (threadIdx.x != 0) return;
     const int n = st->iter_done;
     for (int k = 0; k < n; k++) {
@@ -183,7 +212,7 @@ __global__ void k_sil_tridiag(SilState* st, double dt_re, double dt_im) {
 // 131-169): eigenvector of the smallest Ritz value (the first index found
 // when several tie, with the reference's 1e15 sentinel).
 __global__ void k_sil_ground(SilState* st) {
-    __shared__ double d[kMaxLanczos + 1], e[kMaxLanczos + 1], z[(kMaxLanczos + 1) * (kMaxLanczos + 1)];
+    __shared__ double d[kMaxKrylov + 1], e[kMaxKrylov + 1], z[(kMaxKrylov + 1) * (kMaxKrylov + 1)];
     if (threadIdx.x != 0) return;
     const int n = st->iter_done;
     for (int k = 0; k < n; k++) {
@@ -216,7 +245,7 @@ __global__ __launch_bounds__(kStreamBlock) void k_to_complex64(const double2* __
 }
 
 struct WPtrs {
-    const void* w[kMaxLanczos];
+    const void* w[kMaxKrylov];
 };
 
 // w_{it+1} = out - sum_{k<=it} cre_k w_k, with partials of <w_k, w_{it+1}>
@@ -260,20 +289,100 @@ __global__ __launch_bounds__(kStreamBlock) void k_sil_reorth(const double2* __re
     }
 }
 
-// C' = sum_j y_j w_j (w_0 is C itself in FP64), in place, plus |C'|^2 partials.
+// Krylov dimensions past the register-resident kernels: the update pass
+// w_{it+1} = out - sum_k cre_k w_k with the partials of |w_{it+1}|^2 (slot
+// 2 nk, nk = it + 1), then the Gram column in chunks of eight vectors.
 template <typename Zw>
+__global__ __launch_bounds__(kStreamBlock) void k_sil_reorth_any(const double2* __restrict__ out, WPtrs wp,
+                                                                 Zw* __restrict__ wnew, uint32_t n,
+                                                                 const SilState* __restrict__ st, int nk,
+                                                                 double* __restrict__ part) {
+    __shared__ double red[kStreamBlock / 64];
+    if (st->stop) return;
+    double acc[1] = {0};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double2 w = out[i];
+        for (int k = 0; k < nk; k++) w = w - cmul(st->cre[k], to_d2(((const Zw*) wp.w[k])[i]));
+        const Zw ws = from_d2<Zw>(w);
+        wnew[i] = ws;
+        const double2 wr = to_d2(ws);
+        acc[0] += wr.x * wr.x + wr.y * wr.y;
+    }
+    block_sum<1>(acc, red);
+    if (threadIdx.x == 0) part[(size_t) (2 * nk) * gridDim.x + blockIdx.x] = acc[0];
+}
+
+template <typename Zw>
+__global__ __launch_bounds__(kStreamBlock) void k_multidot8(WPtrs wp, const Zw* __restrict__ w, uint32_t n,
+                                                            const SilState* __restrict__ st, int k0, int nk,
+                                                            double* __restrict__ part) {
+    __shared__ double red[16 * (kStreamBlock / 64)];
+    if (st->stop) return;
+    double acc[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) acc[t] = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double2 wr = to_d2(w[i]);
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            if (k0 + j < nk) {
+                const double2 p = cconjmul(to_d2(((const Zw*) wp.w[k0 + j])[i]), wr);
+                acc[2 * j] += p.x;
+                acc[2 * j + 1] += p.y;
+            }
+        }
+    }
+    block_sum<16>(acc, red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            if (k0 + j < nk) {
+                part[(size_t) (2 * (k0 + j)) * gridDim.x + blockIdx.x] = acc[2 * j];
+                part[(size_t) (2 * (k0 + j) + 1) * gridDim.x + blockIdx.x] = acc[2 * j + 1];
+            }
+        }
+    }
+}
+
+// C' = sum_j y_j w_j for any Krylov dimension (coefficients read per vector).
+template <typename Zw>
+__global__ __launch_bounds__(kStreamBlock) void k_sil_recombine_any(double2* __restrict__ C, WPtrs wp, uint32_t n,
+                                                                    const SilState* __restrict__ st,
+                                                                    double* __restrict__ part) {
+    __shared__ double red[kStreamBlock / 64];
+    const int nd = st->iter_done;
+    double acc[1] = {0};
+    if (!st->eig_fail) {
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+            double2 v = cmul(st->y[0], C[i]);
+            for (int k = 1; k < nd; k++) v = v + cmul(st->y[k], to_d2(((const Zw*) wp.w[k])[i]));
+            C[i] = v;
+            acc[0] += v.x * v.x + v.y * v.y;
+        }
+    }
+    block_sum<1>(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// C' = sum_j y_j w_j (w_0 is C itself in FP64), in place, plus |C'|^2
+// partials.  NK is the Krylov dimension; vectors past a breakdown
+// (k >= iter_done) are skipped.
+template <typename Zw, int NK>
 __global__ __launch_bounds__(kStreamBlock) void k_sil_recombine(double2* __restrict__ C, WPtrs wp, uint32_t n,
                                                                 const SilState* __restrict__ st,
                                                                 double* __restrict__ part) {
     __shared__ double red[kStreamBlock / 64];
     const int nd = st->iter_done;
-    double2 y[kMaxLanczos];
-    for (int k = 0; k < nd; k++) y[k] = st->y[k];
+    double2 y[NK];
+#pragma unroll
+    for (int k = 0; k < NK; k++) y[k] = st->y[k];
     double acc[1] = {0};
     if (!st->eig_fail) {
         for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
             double2 v = cmul(y[0], C[i]);
-            for (int k = 1; k < nd; k++) v = v + cmul(y[k], to_d2(((const Zw*) wp.w[k])[i]));
+#pragma unroll
+            for (int k = 1; k < NK; k++)
+                if (k < nd) v = v + cmul(y[k], to_d2(((const Zw*) wp.w[k])[i]));
             C[i] = v;
             acc[0] += v.x * v.x + v.y * v.y;
         }
@@ -302,8 +411,8 @@ __global__ __launch_bounds__(kStreamBlock) void k_export_vector(const Zw* __rest
 
 int sil_create(const Space* S, int lm, int prec, int index, Sil** out) {
     *out = nullptr;
-    if (lm < 2 || lm > kMaxLanczos) {
-        set_error("Lanczos iterations must be between 2 and 15");
+    if (lm < 2 || lm > kMaxKrylov) {
+        set_error("Lanczos iterations must be between 2 and 64");
         return kInvalidArgument;
     }
     if (prec != kPrecFP64 && prec != kPrecMixed) {
@@ -330,14 +439,14 @@ int sil_create(const Space* S, int lm, int prec, int index, Sil** out) {
     const int M = S->norb;
     const size_t cfn = generic ? (size_t) M * M + (size_t) M * M * M * M : (size_t) cf_size(M);
     const unsigned nbh = hc_grid_blocks(S->dim);
-    L->part_cap = std::max<size_t>((size_t) nbh * lanczos_hc_slots(kMaxLanczos),
-                                   (size_t) kStreamGrid * (2 * kMaxLanczos + 2));
+    L->part_cap = std::max<size_t>((size_t) nbh * lanczos_hc_slots(kMaxKrylov),
+                                   (size_t) kStreamGrid * (2 * kMaxKrylov + 2));
     bool ok = hipMalloc((void**) &L->d_st, sizeof(SilState)) == hipSuccess;
     ok = ok && hipMalloc((void**) &L->cf64, sizeof(double2) * cfn) == hipSuccess;
     if (prec == kPrecMixed) ok = ok && hipMalloc((void**) &L->cf32, sizeof(float2) * cfn) == hipSuccess;
     ok = ok && hipMalloc((void**) &L->out, sizeof(double2) * n) == hipSuccess;
     ok = ok && hipMalloc((void**) &L->part, sizeof(double) * L->part_cap) == hipSuccess;
-    ok = ok && hipMalloc((void**) &L->tot, sizeof(double) * 64) == hipSuccess;
+    ok = ok && hipMalloc((void**) &L->tot, sizeof(double) * (2 * kMaxKrylov + 4)) == hipSuccess;
     for (int k = 1; k < lm && ok; k++) ok = hipMalloc(&L->w[k], zw * n) == hipSuccess;
     if (prec == kPrecMixed) ok = ok && hipMalloc(&L->w0f, sizeof(float2) * n) == hipSuccess;
     if (!ok) {
@@ -357,7 +466,7 @@ void sil_destroy(Sil* L) {
     hipFree(L->out);
     hipFree(L->part);
     hipFree(L->tot);
-    for (int k = 1; k < kMaxLanczos; k++) hipFree(L->w[k]);
+    for (int k = 1; k < kMaxKrylov; k++) hipFree(L->w[k]);
     hipFree(L->w0f);
     delete L;
 }
@@ -377,7 +486,25 @@ static void launch_reorth(Sil* L, int it, hipStream_t st) {
     const int nslot = 2 * NK + 1;
     k_sil_reorth<Zw, NK><<<kStreamGrid, kStreamBlock, 0, st>>>(L->out, wp, (Zw*) L->w[it + 1], L->S->dim, L->d_st,
                                                                L->part, nslot);
-    k_reduce_slots<<<nslot, 256, 0, st>>>(L->part, kStreamGrid, nslot, L->tot, L->d_st);
+    k_reduce_scalars<1><<<nslot, 256, 0, st>>>(L->part, kStreamGrid, L->tot, L->d_st, it, 0, 0);
+}
+
+template <typename Zw>
+static void recombine_dispatch(Sil* L, double2* C, hipStream_t st) {
+    WPtrs wp{};
+    for (int k = 0; k < L->lm; k++) wp.w[k] = L->w[k];
+    switch (L->lm) {
+#define CASE(K)                                                                                        \
+    case K:                                                                                            \
+        k_sil_recombine<Zw, K><<<kStreamGrid, kStreamBlock, 0, st>>>(C, wp, L->S->dim, L->d_st, L->part); \
+        break;
+        CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
+        CASE(14) CASE(15)
+#undef CASE
+        default:
+            k_sil_recombine_any<Zw><<<kStreamGrid, kStreamBlock, 0, st>>>(C, wp, L->S->dim, L->d_st, L->part);
+            break;
+    }
 }
 
 template <typename Zw>
@@ -388,7 +515,18 @@ static void reorth_dispatch(Sil* L, int it, hipStream_t st) {
         CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12)
         CASE(13) CASE(14)
 #undef CASE
-        default: break;
+        default: {
+            const int nk = it + 1;
+            WPtrs wp{};
+            for (int k = 0; k < nk; k++) wp.w[k] = L->w[k];
+            Zw* wnew = (Zw*) L->w[it + 1];
+            k_sil_reorth_any<Zw><<<kStreamGrid, kStreamBlock, 0, st>>>(L->out, wp, wnew, L->S->dim, L->d_st, nk,
+                                                                       L->part);
+            for (int k0 = 0; k0 < nk; k0 += 8)
+                k_multidot8<Zw><<<kStreamGrid, kStreamBlock, 0, st>>>(wp, wnew, L->S->dim, L->d_st, k0, nk, L->part);
+            k_reduce_scalars<1><<<2 * nk + 1, 256, 0, st>>>(L->part, kStreamGrid, L->tot, L->d_st, it, 0, 0);
+            break;
+        }
     }
 }
 
@@ -421,15 +559,16 @@ static int sil_run_lanczos(Sil* L, const double2* C, const double2* hob, const d
         a.st = L->d_st;
         a.out = L->out;
         a.part = L->part;
+        a.last = it == L->lm - 1;
         rc = hc_lanczos(S, L->cf64, L->cf32, a, L->prec, L->index, st);
         if (rc) return rc;
-        const int nslot = lanczos_hc_slots(it);
-        k_reduce_slots<<<nslot, 256, 0, st>>>(L->part, nbh, nslot, L->tot, L->d_st);
-        k_sil_after_hc<<<1, 64, 0, st>>>(L->d_st, L->tot, it);
+        if (a.last)
+            k_reduce_scalars<0><<<2, 256, 0, st>>>(L->part, nbh, L->tot, L->d_st, it, 2 * it, 1);
+        else
+            k_reduce_scalars<0><<<lanczos_hc_slots(it), 256, 0, st>>>(L->part, nbh, L->tot, L->d_st, it, 0, 0);
         if (it == L->lm - 1) break;
         if (L->prec == kPrecFP64) reorth_dispatch<double2>(L, it, st);
         else reorth_dispatch<float2>(L, it, st);
-        k_sil_after_reorth<<<1, 64, 0, st>>>(L->d_st, L->tot, it);
     }
     MCTDHB_HIP_CHECK(hipGetLastError());
     return kOk;
@@ -440,12 +579,8 @@ int sil_step(Sil* L, double2* C, const double2* hob, const double2* hint, double
     int rc = sil_run_lanczos(L, C, hob, hint, st);
     if (rc) return rc;
     k_sil_tridiag<<<1, 64, 0, st>>>(L->d_st, dt_re, dt_im);
-    WPtrs wp{};
-    for (int k = 0; k < L->lm; k++) wp.w[k] = L->w[k];
-    if (L->prec == kPrecFP64)
-        k_sil_recombine<double2><<<kStreamGrid, kStreamBlock, 0, st>>>(C, wp, L->S->dim, L->d_st, L->part);
-    else
-        k_sil_recombine<float2><<<kStreamGrid, kStreamBlock, 0, st>>>(C, wp, L->S->dim, L->d_st, L->part);
+    if (L->prec == kPrecFP64) recombine_dispatch<double2>(L, C, st);
+    else recombine_dispatch<float2>(L, C, st);
     if (imagtime) {
         k_reduce_slots<<<1, 256, 0, st>>>(L->part, kStreamGrid, 1, L->tot + 48, nullptr);
         k_scale_by_inv_norm<<<kStreamGrid, kStreamBlock, 0, st>>>(C, L->S->dim, L->tot + 48);
@@ -458,12 +593,8 @@ int sil_ground(Sil* L, double2* C, const double2* hob, const double2* hint, hipS
     int rc = sil_run_lanczos(L, C, hob, hint, st);
     if (rc) return rc;
     k_sil_ground<<<1, 64, 0, st>>>(L->d_st);
-    WPtrs wp{};
-    for (int k = 0; k < L->lm; k++) wp.w[k] = L->w[k];
-    if (L->prec == kPrecFP64)
-        k_sil_recombine<double2><<<kStreamGrid, kStreamBlock, 0, st>>>(C, wp, L->S->dim, L->d_st, L->part);
-    else
-        k_sil_recombine<float2><<<kStreamGrid, kStreamBlock, 0, st>>>(C, wp, L->S->dim, L->d_st, L->part);
+    if (L->prec == kPrecFP64) recombine_dispatch<double2>(L, C, st);
+    else recombine_dispatch<float2>(L, C, st);
     MCTDHB_HIP_CHECK(hipGetLastError());
     return kOk;
 }

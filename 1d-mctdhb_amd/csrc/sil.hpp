@@ -11,18 +11,24 @@
 
 namespace mctdhb {
 
-constexpr int kMaxLanczos = 15;  // reference MAX_LANCZOS_ITER (mctdhb_types.h:33)
+// Krylov capacity: the imaginary-time finish diagonalises with up to 50
+// Lanczos vectors (reference driver.c:40-45, :300-316).
+constexpr int kMaxKrylov = 64;
+// reference MAX_LANCZOS_ITER (mctdhb_types.h:33): up to this many vectors the
+// streaming kernels keep every coefficient in registers.
+constexpr int kMaxLanczos = 15;
 
 struct SilState {
-    double s[kMaxLanczos + 1];        // scale of w_k
-    double beta[kMaxLanczos + 1];     // offd (real, as carrMod returns)
-    double2 alpha[kMaxLanczos + 1];   // diag (complex, as carrDot returns)
-    double2 cre[kMaxLanczos + 1];     // reorthogonalisation coefficients (on w_k)
-    double2 gram[kMaxLanczos + 1];    // <v_k, v_cur> for the current vector
-    double2 y[kMaxLanczos + 1];       // recombination coefficients (on w_k)
+    double s[kMaxKrylov + 1];        // scale of w_k
+    double beta[kMaxKrylov + 1];     // offd (real, as carrMod returns)
+    double2 alpha[kMaxKrylov + 1];   // diag (complex, as carrDot returns)
+    double2 cre[kMaxKrylov + 1];     // reorthogonalisation coefficients (on w_k)
+    double2 gram[kMaxKrylov + 1];    // <v_k, v_cur> for the current vector
+    double2 y[kMaxKrylov + 1];       // recombination coefficients (on w_k)
     double maxbeta;
     double nrm2;                      // |C'|^2 of the recombined vector
     double eground;                   // lowest Ritz value (ground-state mode)
+    unsigned int arrive;              // blocks done in the current reduce kernel
     int iter_done;
     int stop;                         // breakdown: later kernels are no-ops
     int eig_fail;

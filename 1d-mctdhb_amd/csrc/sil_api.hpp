@@ -13,7 +13,7 @@ struct Sil {
     SilState* d_st = nullptr;
     double2* cf64 = nullptr;  // fused coefficient block, or [hob | hint] for the generic kernel
     float2* cf32 = nullptr;
-    void* w[kMaxLanczos] = {};  // w[0] is C (FP64) or its complex64 copy
+    void* w[kMaxKrylov] = {};  // w[0] is C (FP64) or its complex64 copy
     void* w0f = nullptr;
     double2* out = nullptr;
     double* part = nullptr;

@@ -186,7 +186,8 @@ int    mctdhb_apply_hamiltonian_cf_dev(const mctdhb_space* space, const void* co
 int mctdhb_density_dev(mctdhb_space* space, const void* coef, void* rho1, void* rho2, int prec,
                        int index, void* stream);
 
-/* SIL workspace for lanczos_iter Krylov vectors (2..15). */
+/* SIL workspace for lanczos_iter Krylov vectors (2..64; up to 15 the
+ * streaming kernels keep all coefficients in registers). */
 int mctdhb_sil_create(const mctdhb_space* space, int lanczos_iter, int prec, int index,
                       mctdhb_sil** out);
 int mctdhb_sil_destroy(mctdhb_sil* sil);

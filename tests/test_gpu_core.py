@@ -134,7 +134,7 @@ def test_lanczos_matches_oracle(amd, orc, npar, norb, lm):
 
 
 @pytest.mark.parametrize("npar,norb,lm,imag", [(6, 4, 6, False), (10, 5, 8, True), (3, 9, 5, False),
-                                               (12, 3, 5, True)])
+                                               (12, 3, 5, True), (10, 4, 20, False), (10, 4, 18, True)])
 def test_sil_step_matches_oracle(amd, orc, npar, norb, lm, imag):
     print(f"sil N={npar} M={norb} lm={lm} imag={imag}", flush=True)
     S = orc.Space(npar, norb)
@@ -148,7 +148,8 @@ def test_sil_step_matches_oracle(amd, orc, npar, norb, lm, imag):
     assert gu.rel_err(gu.host(dc), ref) < 1e-10
 
 
-@pytest.mark.parametrize("npar,norb,lm,prec", [(6, 4, 8, 0), (10, 5, 12, 0), (10, 5, 12, 1), (3, 9, 6, 0)])
+@pytest.mark.parametrize("npar,norb,lm,prec", [(6, 4, 8, 0), (10, 5, 12, 0), (10, 5, 12, 1), (3, 9, 6, 0),
+                                               (12, 5, 40, 0), (12, 5, 24, 1)])
 def test_lowest_state_matches_oracle(amd, orc, npar, norb, lm, prec):
     print(f"lowest N={npar} M={norb} lm={lm} prec={prec}", flush=True)
     S = orc.Space(npar, norb)
