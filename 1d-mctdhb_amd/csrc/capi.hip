@@ -16,6 +16,7 @@
 
 #include "density.hpp"
 #include "hamiltonian.hpp"
+#include "monitor.hpp"
 #include "mctdhb_hip.h"
 #include "orbital.hpp"
 #include "sil_api.hpp"
@@ -240,6 +241,30 @@ extern "C" int mctdhb_lowest_state_dev(mctdhb_sil* s, void* coef, const void* ho
     }
     *energy = h.eground;
     return kOk;
+}
+
+extern "C" int mctdhb_eig_residual_dev(mctdhb_space* sp, const void* coef, const void* hob, const void* hint,
+                                       double e, double* result, void* stream) {
+    if (!sp || !coef || !hob || !hint || !result) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    hipStream_t st = (hipStream_t) stream;
+    double2* hc = nullptr;
+    double* part = nullptr;
+    MCTDHB_HIP_CHECK(hipMallocAsync((void**) &hc, sizeof(double2) * sp->S->dim, st));
+    MCTDHB_HIP_CHECK(hipMallocAsync((void**) &part, sizeof(double) * eig_residual_part_doubles(), st));
+    int rc = eig_residual_dev(sp->S, (const double2*) coef, (const double2*) hob, (const double2*) hint, e, hc, part,
+                              part + eig_residual_part_doubles() - 1, st);
+    if (!rc && hipMemcpyAsync(result, part + eig_residual_part_doubles() - 1, sizeof(double), hipMemcpyDeviceToHost,
+                              st) != hipSuccess) {
+        set_error("device to host copy failed");
+        rc = kDeviceError;
+    }
+    hipFreeAsync(hc, st);
+    hipFreeAsync(part, st);
+    if (!rc) MCTDHB_HIP_CHECK(hipStreamSynchronize(st));
+    return rc;
 }
 
 extern "C" int mctdhb_sil_scalars(const mctdhb_sil* s, int* iter_done, double* diag, double* offd, void* stream) {
@@ -820,6 +845,17 @@ extern "C" int lanczos(MultiConfiguration mc, Cmatrix hob, Carray hint, int lm, 
         }
     }
     return done;
+}
+
+extern "C" double eig_residual(MultiConfiguration mc, Carray c, Cmatrix hob, Carray hint, double e) {
+    mctdhb_space* sp = ensure_dev(mc);
+    const size_t n = sp->S->dim;
+    stage_alloc(&sp->d_coef, n);
+    stage_hob_hint(sp, hob, hint);
+    h2d(sp->d_coef, c, sizeof(dcomplex) * n);
+    double r = 0;
+    if (mctdhb_eig_residual_dev(sp, sp->d_coef, sp->d_hob, sp->d_hint, e, &r, nullptr)) fail_stop("eig_residual");
+    return r;
 }
 
 extern "C" double lowest_state_lanczos(uint16_t niter, MultiConfiguration mc, Cmatrix Ho, Carray Hint, Carray C) {

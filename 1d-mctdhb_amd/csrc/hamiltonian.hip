@@ -277,7 +277,7 @@ static int check_mode(const Space* S, int prec, int index) {
         return kNotBuilt;
     }
     if (use_generic(S) && prec != kPrecFP64) {
-        set_error("mixed precision H.C covers 2 <= norb <= 8");
+        set_error("mixed precision H.C covers 2 <= norb <= 7");
         return kInvalidArgument;
     }
     return kOk;

@@ -102,16 +102,28 @@ __device__ void sil_after_reorth(SilState* st, const volatile double* tot, int i
 // finish) the scalar update OP of iteration it: 0 = sil_after_hc, 1 =
 // sil_after_reorth.  Blocks count themselves in st->arrive (a vector
 // atomic); the last one resets it.
+constexpr int kReduceBlock = 1024;
+
 template <int OP>
-__global__ __launch_bounds__(256) void k_reduce_scalars(const double* __restrict__ part, int nblocks, double* tot,
-                                                        SilState* st, int it, int slot0, int final_it) {
-    __shared__ double red[4];
+__global__ __launch_bounds__(kReduceBlock) void k_reduce_scalars(const double* __restrict__ part, int nblocks,
+                                                                 double* tot, SilState* st, int it, int slot0,
+                                                                 int final_it) {
+    __shared__ double red[kReduceBlock / 64];
     __shared__ int last;
     if (st->stop) return;  // set only by a previous kernel: uniform over the grid
     const int t = slot0 + blockIdx.x;
-    double v[1] = {0};
     const double* p = part + (size_t) t * nblocks;
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) v[0] += p[b];
+    // four independent partial sums keep four loads in flight per thread
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    int b = threadIdx.x;
+    for (; b + 3 * kReduceBlock < nblocks; b += 4 * kReduceBlock) {
+        a0 += p[b];
+        a1 += p[b + kReduceBlock];
+        a2 += p[b + 2 * kReduceBlock];
+        a3 += p[b + 3 * kReduceBlock];
+    }
+    for (; b < nblocks; b += kReduceBlock) a0 += p[b];
+    double v[1] = {(a0 + a1) + (a2 + a3)};
     block_sum<1>(v, red);
     if (threadIdx.x == 0) {
         tot[t] = v[0];
@@ -126,113 +138,135 @@ __global__ __launch_bounds__(256) void k_reduce_scalars(const double* __restrict
     else sil_after_reorth(st, tot, it);
 }
 
-// exp(-i T prop_dt) e_1 from the eigen-decomposition of the real tridiagonal
-// T = (Re alpha, beta), as reference coefficients_integration.c:104-145.
-// Implicit-shift QL with Givens rotations (the EISPACK tql2 method), one
-// thread; n <= 15.
-__device__ int tridiag_eig_dev(int n, double* d, double* e, double* z) {
-    for (int r = 0; r < n; r++)
-        for (int c = 0; c < n; c++) z[r * n + c] = (r == c) ? 1.0 : 0.0;
-    e[n - 1] = 0.0;
+// ------------------------------------------------ tridiagonal eigensolver
+//
+// Implicit-shift QL with Givens rotations (the EISPACK tql2 method) on one
+// wave: lane r keeps d[r] and e[r] in registers and owns row r of the
+// eigenvector matrix (in LDS), so each rotation updates all rows at once;
+// the scalar recurrence runs redundantly in every lane on values broadcast
+// with readlane.  n <= 64.
+
+__device__ __forceinline__ double rlane(double x, int i) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), i);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), i);
+    return __hiloint2double(hi, lo);
+}
+
+// On return lane r holds eigenvalue d (unsorted) and zr[0..n) is row r of Z
+// (eigenvector j in column j).  Returns non-zero if an eigenvalue needed
+// more than 60 sweeps.
+__device__ int tridiag_eig_wave(int n, double& d, double& e, double* zr) {
+    const int lane = threadIdx.x & 63;
+    for (int c = 0; c < n; c++) zr[c] = (lane == c) ? 1.0 : 0.0;
+    if (lane == n - 1) e = 0.0;
     for (int l = 0; l < n; l++) {
-        int sweeps = 0, m;
+        int sweeps = 0;
         for (;;) {
-            for (m = l; m < n - 1; m++) {
-                const double scale = fabs(d[m]) + fabs(d[m + 1]);
-                if (fabs(e[m]) <= 2.220446049250313e-16 * scale) break;
+            int m = l;
+            for (; m < n - 1; m++) {
+                const double scale = fabs(rlane(d, m)) + fabs(rlane(d, m + 1));
+                if (fabs(rlane(e, m)) <= 2.220446049250313e-16 * scale) break;
             }
             if (m == l) break;
             if (++sweeps > 60) return 1;
-            const double theta = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            const double dl = rlane(d, l), el = rlane(e, l);
+            const double theta = (rlane(d, l + 1) - dl) / (2.0 * el);
             const double rho = hypot(theta, 1.0);
-            double g = d[m] - d[l] + e[l] / (theta + (theta >= 0 ? rho : -rho));
+            double g = rlane(d, m) - dl + el / (theta + (theta >= 0 ? rho : -rho));
             double sn = 1.0, cs = 1.0, shift = 0.0;
             bool early = false;
             for (int i = m - 1; i >= l; i--) {
-                const double f = sn * e[i], b = cs * e[i];
+                const double ei = rlane(e, i);
+                const double f = sn * ei, b = cs * ei;
                 double r = hypot(f, g);
-                e[i + 1] = r;
+                if (lane == i + 1) e = r;
                 if (r == 0.0) {
-                    d[i + 1] -= shift;
-                    e[m] = 0.0;
+                    if (lane == i + 1) d -= shift;
+                    if (lane == m) e = 0.0;
                     early = true;
                     break;
                 }
                 sn = f / r;
                 cs = g / r;
-                g = d[i + 1] - shift;
-                r = (d[i] - g) * sn + 2.0 * cs * b;
+                g = rlane(d, i + 1) - shift;
+                r = (rlane(d, i) - g) * sn + 2.0 * cs * b;
                 shift = sn * r;
-                d[i + 1] = g + shift;
+                if (lane == i + 1) d = g + shift;
                 g = cs * r - b;
-                for (int row = 0; row < n; row++) {
-                    const double zi1 = z[row * n + i + 1], zi = z[row * n + i];
-                    z[row * n + i + 1] = sn * zi + cs * zi1;
-                    z[row * n + i] = cs * zi - sn * zi1;
+                if (lane < n) {
+                    const double zi1 = zr[i + 1], zi = zr[i];
+                    zr[i + 1] = sn * zi + cs * zi1;
+                    zr[i] = cs * zi - sn * zi1;
                 }
             }
             if (early) continue;
-            d[l] -= shift;
-            e[l] = g;
-            e[m] = 0.0;
+            if (lane == l) {
+                d -= shift;
+                e = g;
+            }
+            if (lane == m) e = 0.0;
         }
     }
     return 0;
 }
 
-__global__ void k_sil_tridiag(SilState* st, double dt_re, double dt_im) {
-    // LDS rather than per-thread arrays: a lone thread's dynamically
-    // indexed arrays would live in scratch memory
-    __shared__ double d[kMaxKrylov + 1], e[kMaxKrylov + 1], z[(kMaxKrylov + 1) * (kMaxKrylov + 1)];
-    __shared__ double2 aux[kMaxKrylov + 1];
-    if (threadIdx.x != 0) return;
+// exp(-i T prop_dt) e_1 from the eigen-decomposition of the real tridiagonal
+// T = (Re alpha, beta), as reference coefficients_integration.c:104-145:
+// y = Z exp(-i D prop_dt) Z^T e_1.  One wave.
+__global__ __launch_bounds__(64) void k_sil_tridiag(SilState* st, double dt_re, double dt_im) {
+    __shared__ double z[kMaxKrylov * (kMaxKrylov + 1)];
+    const int lane = threadIdx.x;
     const int n = st->iter_done;
-    for (int k = 0; k < n; k++) {
-        d[k] = st->alpha[k].x;
-        e[k] = st->beta[k];
-    }
-    if (tridiag_eig_dev(n, d, e, z)) {
-        st->eig_fail = 1;
+    double d = lane < n ? st->alpha[lane].x : 0.0;
+    double e = lane < n ? st->beta[lane] : 0.0;
+    double* zr = z + lane * (kMaxKrylov + 1);
+    if (tridiag_eig_wave(n, d, e, zr)) {
+        if (lane == 0) st->eig_fail = 1;
         return;
     }
-    // y = Z exp(-i D prop_dt) Z^T e_1; exp(-i d dt) for complex dt = a + ib
-    // is exp(d b) (cos(d a) - i sin(d a)).
-    for (int k = 0; k < n; k++) {
-        const double mag = exp(d[k] * dt_im), ph = -d[k] * dt_re;
-        aux[k] = make_double2(z[0 * n + k] * mag * cos(ph), z[0 * n + k] * mag * sin(ph));
+    // lane k: aux_k = Z[0][k] exp(-i d_k prop_dt); exp(-i d dt) for complex
+    // dt = a + ib is exp(d b) (cos(d a) - i sin(d a))
+    double ax = 0, ay = 0;
+    if (lane < n) {
+        const double z0k = z[lane];  // row 0, column lane
+        const double mag = exp(d * dt_im), ph = -d * dt_re;
+        ax = z0k * mag * cos(ph);
+        ay = z0k * mag * sin(ph);
     }
-    for (int k = 0; k < n; k++) {
-        double2 y = make_double2(0, 0);
-        for (int j = 0; j < n; j++) y = y + cscale(aux[j], z[k * n + j]);
-        st->y[k] = cscale(y, st->s[k]);
+    double yx = 0, yy = 0;
+    for (int j = 0; j < n; j++) {
+        const double zkj = zr[j];
+        yx += rlane(ax, j) * zkj;
+        yy += rlane(ay, j) * zkj;
     }
+    if (lane < n) st->y[lane] = make_double2(yx * st->s[lane], yy * st->s[lane]);
 }
 
 // Ground-state mode (reference lowest_state_lanczos, multiconfig_lanczos.c:
 // 131-169): eigenvector of the smallest Ritz value (the first index found
-// when several tie, with the reference's 1e15 sentinel).
-__global__ void k_sil_ground(SilState* st) {
-    __shared__ double d[kMaxKrylov + 1], e[kMaxKrylov + 1], z[(kMaxKrylov + 1) * (kMaxKrylov + 1)];
-    if (threadIdx.x != 0) return;
+// when several tie, with the reference's 1e15 sentinel).  One wave.
+__global__ __launch_bounds__(64) void k_sil_ground(SilState* st) {
+    __shared__ double z[kMaxKrylov * (kMaxKrylov + 1)];
+    const int lane = threadIdx.x;
     const int n = st->iter_done;
-    for (int k = 0; k < n; k++) {
-        d[k] = st->alpha[k].x;
-        e[k] = st->beta[k];
-    }
-    if (tridiag_eig_dev(n, d, e, z)) {
-        st->eig_fail = 1;
+    double d = lane < n ? st->alpha[lane].x : 0.0;
+    double e = lane < n ? st->beta[lane] : 0.0;
+    double* zr = z + lane * (kMaxKrylov + 1);
+    if (tridiag_eig_wave(n, d, e, zr)) {
+        if (lane == 0) st->eig_fail = 1;
         return;
     }
     double lowest = 1e15;
     int j = 0;
     for (int k = 0; k < n; k++) {
-        if (lowest > d[k]) {
-            lowest = d[k];
+        const double dk = rlane(d, k);
+        if (lowest > dk) {
+            lowest = dk;
             j = k;
         }
     }
-    st->eground = lowest;
-    for (int k = 0; k < n; k++) st->y[k] = make_double2(z[k * n + j] * st->s[k], 0.0);
+    if (lane == 0) st->eground = lowest;
+    if (lane < n) st->y[lane] = make_double2(zr[j] * st->s[lane], 0.0);
 }
 
 // ---------------------------------------------------------- streaming
@@ -486,7 +520,7 @@ static void launch_reorth(Sil* L, int it, hipStream_t st) {
     const int nslot = 2 * NK + 1;
     k_sil_reorth<Zw, NK><<<kStreamGrid, kStreamBlock, 0, st>>>(L->out, wp, (Zw*) L->w[it + 1], L->S->dim, L->d_st,
                                                                L->part, nslot);
-    k_reduce_scalars<1><<<nslot, 256, 0, st>>>(L->part, kStreamGrid, L->tot, L->d_st, it, 0, 0);
+    k_reduce_scalars<1><<<nslot, kReduceBlock, 0, st>>>(L->part, kStreamGrid, L->tot, L->d_st, it, 0, 0);
 }
 
 template <typename Zw>
@@ -524,7 +558,7 @@ static void reorth_dispatch(Sil* L, int it, hipStream_t st) {
                                                                        L->part);
             for (int k0 = 0; k0 < nk; k0 += 8)
                 k_multidot8<Zw><<<kStreamGrid, kStreamBlock, 0, st>>>(wp, wnew, L->S->dim, L->d_st, k0, nk, L->part);
-            k_reduce_scalars<1><<<2 * nk + 1, 256, 0, st>>>(L->part, kStreamGrid, L->tot, L->d_st, it, 0, 0);
+            k_reduce_scalars<1><<<2 * nk + 1, kReduceBlock, 0, st>>>(L->part, kStreamGrid, L->tot, L->d_st, it, 0, 0);
             break;
         }
     }
@@ -563,9 +597,9 @@ static int sil_run_lanczos(Sil* L, const double2* C, const double2* hob, const d
         rc = hc_lanczos(S, L->cf64, L->cf32, a, L->prec, L->index, st);
         if (rc) return rc;
         if (a.last)
-            k_reduce_scalars<0><<<2, 256, 0, st>>>(L->part, nbh, L->tot, L->d_st, it, 2 * it, 1);
+            k_reduce_scalars<0><<<2, kReduceBlock, 0, st>>>(L->part, nbh, L->tot, L->d_st, it, 2 * it, 1);
         else
-            k_reduce_scalars<0><<<lanczos_hc_slots(it), 256, 0, st>>>(L->part, nbh, L->tot, L->d_st, it, 0, 0);
+            k_reduce_scalars<0><<<lanczos_hc_slots(it), kReduceBlock, 0, st>>>(L->part, nbh, L->tot, L->d_st, it, 0, 0);
         if (it == L->lm - 1) break;
         if (L->prec == kPrecFP64) reorth_dispatch<double2>(L, it, st);
         else reorth_dispatch<float2>(L, it, st);

@@ -100,6 +100,9 @@ def lib():
     L.mctdhb_lanczos_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
     L.mctdhb_sil_scalars.argtypes = [_vp, C.POINTER(_i), _vp, _vp, _vp]
     L.mctdhb_lowest_state_dev.argtypes = [_vp, _vp, _vp, _vp, C.POINTER(C.c_double), _vp]
+    L.mctdhb_eig_residual_dev.argtypes = [_vp, _vp, _vp, _vp, C.c_double, C.POINTER(C.c_double), _vp]
+    L.eig_residual.argtypes = [C.POINTER(_MultiConfiguration), _vp, C.POINTER(_vp), _vp, C.c_double]
+    L.eig_residual.restype = C.c_double
     L.mctdhb_orbital_create.argtypes = [_i, _i, _i, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                         C.c_double, C.c_double, _i, _vp, _cpp]
     L.mctdhb_orbital_destroy.argtypes = [_vp]
@@ -206,6 +209,14 @@ class DeviceSpace:
     def density(self, coef, rho1, rho2=None, prec=FP64, index=INDEX_FLY, stream=None):
         check(lib().mctdhb_density_dev(self._h, _ptr(coef), _ptr(rho1), _ptr(rho2), prec, index,
                                        _stream(stream)))
+
+    def eig_residual(self, coef, hob, hint, e, stream=None) -> float:
+        """max_i |e c_i - (H c)_i| / N (reference eig_residual,
+        integrator_monitor.c:9-28); synchronises."""
+        r = C.c_double()
+        check(lib().mctdhb_eig_residual_dev(self._h, _ptr(coef), _ptr(hob), _ptr(hint), float(e), C.byref(r),
+                                            _stream(stream)))
+        return r.value
 
 
 class DeviceSil:
@@ -382,3 +393,9 @@ class Multiconf:
         hint = np.ascontiguousarray(hint, np.complex128)
         e = lib().lowest_state_lanczos(niter, self._p, self._rows(hob), hint.ctypes.data, c.ctypes.data)
         return e, c
+
+    def eig_residual(self, coef, hob, hint, e) -> float:
+        coef = np.ascontiguousarray(coef, np.complex128)
+        hob = np.ascontiguousarray(hob, np.complex128).reshape(self.norb, self.norb)
+        hint = np.ascontiguousarray(hint, np.complex128)
+        return lib().eig_residual(self._p, coef.ctypes.data, self._rows(hob), hint.ctypes.data, float(e))

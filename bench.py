@@ -247,9 +247,11 @@ def main():
     ab_f64 = algorithmic_bytes(N, M, 16, lm)
 
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "hc_mixed_fly_traffic_N100_M5.json")
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("bytes_per_launch")
+    tpath = os.path.join(ROOT, "profiles", "hc_mixed_fly_traffic_N100_M5.md")
+    if (N, M) == (100, 5) and os.path.exists(tpath):
+        for ln in open(tpath):
+            if ln.startswith("bytes_per_launch:"):
+                traffic = int(ln.split(":")[1])
 
     roofline = {
         "bound": "hbm",

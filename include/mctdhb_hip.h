@@ -125,7 +125,9 @@ void mctdhb_propagate_step(MCTDHBDataStruct mctdhb, Carray orb_works, Carray coe
 /* include/integrator/coefficients_integration.h:25-26 */
 void propagate_coef_sil(MCTDHBDataStruct mctdhb, Carray cnext);                  /* coefficients_integration.c:59-160 */
 
-/* include/assistant/integrator_monitor.h:53 */
+/* include/assistant/integrator_monitor.h */
+double   eig_residual(MultiConfiguration multiconf, Carray c, Cmatrix hob, Carray hint,
+                      double e);                                                 /* integrator_monitor.c:9-28 */
 dcomplex total_energy(ManyBodyState psi);                                        /* integrator_monitor.c:55-88 */
 
 /* ================================================= 2. device-resident */
@@ -205,6 +207,10 @@ int mctdhb_lanczos_dev(mctdhb_sil* sil, const void* coef, const void* hob, const
  * NULL; synchronises) <- that Ritz value.  multiconfig_lanczos.c:93-172 */
 int mctdhb_lowest_state_dev(mctdhb_sil* sil, void* coef, const void* hob, const void* hint, double* energy,
                             void* stream);
+/* *result (host; synchronises) <- max_i |e c_i - (H c)_i| / N for coef
+ * (complex128, device) with the FP64 H.C.  integrator_monitor.c:9-28 */
+int mctdhb_eig_residual_dev(mctdhb_space* space, const void* coef, const void* hob, const void* hint, double e,
+                            double* result, void* stream);
 /* Host copy of the recursion scalars of the last call (synchronises):
  * diag (2*lanczos_iter doubles, complex), offd (lanczos_iter doubles). */
 int mctdhb_sil_scalars(const mctdhb_sil* sil, int* iter_done, double* diag, double* offd,

@@ -155,7 +155,7 @@ def test_lowest_state_matches_oracle(amd, orc, npar, norb, lm, prec):
     S = orc.Space(npar, norb)
     c, h, g = _problem(npar, norb, 37)
     e_ref, ref = S.lowest_state_lanczos(c, h, g, lm)
-    D = amd.DeviceSpace(npar, norb, tables=False)
+    D = amd.DeviceSpace(npar, norb, tables=norb > 7)  # M > 7 runs the table-driven generic H.C
     sil = amd.DeviceSil(D, lm, prec=prec, index=amd.INDEX_FLY)
     dc = gu.dev(c)
     e = sil.lowest_state(dc, gu.dev(h), gu.dev(g))
@@ -166,6 +166,25 @@ def test_lowest_state_matches_oracle(amd, orc, npar, norb, lm, prec):
     v = gu.host(dc)
     hv = S.apply_hamiltonian(v, h, g)
     assert abs(np.vdot(v, hv).real / np.vdot(v, v).real - e_ref) < 100 * tol * abs(e_ref)
+
+
+@pytest.mark.parametrize("npar,norb", [(8, 4), (20, 5), (3, 9)])
+def test_eig_residual_matches_definition(amd, orc, npar, norb):
+    print(f"eig_residual N={npar} M={norb}", flush=True)
+    S = orc.Space(npar, norb)
+    c, h, g = _problem(npar, norb, 41)
+    hc = S.apply_hamiltonian(c, h, g)
+    e = np.vdot(c, hc).real
+    ref = np.max(np.abs(e * c - hc)) / npar
+    D = amd.DeviceSpace(npar, norb, tables=True)
+    r = D.eig_residual(gu.dev(c), gu.dev(h), gu.dev(g), e)
+    assert abs(r - ref) <= 1e-12 * max(ref, 1.0)
+    mc = amd.Multiconf(npar, norb)
+    assert abs(mc.eig_residual(c, h, g, e) - ref) <= 1e-12 * max(ref, 1.0)
+    if S.dim <= 400:  # an eigenvector has a vanishing residual
+        w, v = np.linalg.eigh(np.array([S.apply_hamiltonian(np.eye(S.dim)[k], h, g) for k in range(S.dim)]).T)
+        r0 = D.eig_residual(gu.dev(np.ascontiguousarray(v[:, 0])), gu.dev(h), gu.dev(g), w[0])
+        assert r0 < 1e-10
 
 
 def test_lanczos_breakdown_on_device(amd, orc):
