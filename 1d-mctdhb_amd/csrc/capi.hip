@@ -161,6 +161,29 @@ extern "C" int mctdhb_apply_hamiltonian_cf_dev(const mctdhb_space* sp, const voi
     return hc_apply_cf(sp->S, coef, cf, hc, prec, index, (hipStream_t) stream);
 }
 
+extern "C" int mctdhb_hc_tiling(mctdhb_space* sp, int set, int* tiling) {
+    if (!sp) {
+        set_error("null space");
+        return kInvalidArgument;
+    }
+    if (set < -2 || set > 1) {
+        set_error("tiling must be -2 (query), -1 (time both again), 0 (one configuration per lane) or 1 (pairs)");
+        return kInvalidArgument;
+    }
+    if (set == 1 && !pair_possible(sp->S)) {
+        set_error("this space has no pair work list (needs 2 <= norb <= 7 and dim >= 2)");
+        return kInvalidArgument;
+    }
+    if (set >= 0) sp->S->hc_tiling = set;
+    if (set == -1) {
+        sp->S->hc_tiling = -1;
+        const int rc = hc_tune_tiling(sp->S, nullptr);
+        if (rc) return rc;
+    }
+    if (tiling) *tiling = sp->S->hc_tiling;
+    return kOk;
+}
+
 extern "C" int mctdhb_density_dev(mctdhb_space* sp, const void* coef, void* rho1, void* rho2, int prec, int index,
                                   void* stream) {
     if (!sp || !coef || !rho1) {

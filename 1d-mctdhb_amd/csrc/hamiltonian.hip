@@ -16,6 +16,74 @@ constexpr int kHcBlockG = 256;
 
 unsigned hc_grid_blocks(uint32_t dim) { return div_up(dim, kHcBlockG); }
 
+const float2* hc_padded_copy(const Space* S, const float2* c, hipStream_t st) {
+    const size_t bytes = sizeof(float2) * ((size_t) S->dim + 2);
+    if (!S->d_pad) {
+        if (hipMalloc(&S->d_pad, bytes) != hipSuccess) return nullptr;
+        if (hipMemset(S->d_pad, 0, bytes) != hipSuccess) return nullptr;
+    }
+    float2* p = (float2*) S->d_pad + 1;
+    if (c && hipMemcpyAsync(p, c, sizeof(float2) * S->dim, hipMemcpyDeviceToDevice, st) != hipSuccess) return nullptr;
+    return p;
+}
+
+int hc_tune_tiling(const Space* S, hipStream_t st) {
+    if (S->hc_tiling >= 0) return kOk;
+    if (!pair_possible(S)) {
+        S->hc_tiling = 0;
+        return kOk;
+    }
+    const int M = S->norb;
+    float2 *cf = nullptr, *out = nullptr;
+    const size_t cfb = sizeof(float2) * cf_size(M);
+    if (!hc_padded_copy(S, nullptr, st)) {
+        set_error("padded staging of the coefficient vector failed");
+        return kDeviceError;
+    }
+    MCTDHB_HIP_CHECK(hipMallocAsync((void**) &cf, cfb, st));
+    MCTDHB_HIP_CHECK(hipMallocAsync((void**) &out, sizeof(float2) * S->dim, st));
+    MCTDHB_HIP_CHECK(hipMemsetAsync(cf, 0, cfb, st));
+    const float2* in = (const float2*) S->d_pad + 1;
+    hipEvent_t e0, e1;
+    MCTDHB_HIP_CHECK(hipEventCreate(&e0));
+    MCTDHB_HIP_CHECK(hipEventCreate(&e1));
+    float best[2] = {1e30f, 1e30f};
+    int rc = kOk;
+    for (int rep = 0; rep < 3 && !rc; rep++) {
+        for (int t = 0; t < 2 && !rc; t++) {
+            hipEventRecord(e0, st);
+            switch (M) {
+                case 2: rc = HcFused<2>::plain_tiled(S, in, cf, out, t, st); break;
+                case 3: rc = HcFused<3>::plain_tiled(S, in, cf, out, t, st); break;
+                case 4: rc = HcFused<4>::plain_tiled(S, in, cf, out, t, st); break;
+                case 5: rc = HcFused<5>::plain_tiled(S, in, cf, out, t, st); break;
+                case 6: rc = HcFused<6>::plain_tiled(S, in, cf, out, t, st); break;
+                case 7: rc = HcFused<7>::plain_tiled(S, in, cf, out, t, st); break;
+                default: break;
+            }
+            hipEventRecord(e1, st);
+            hipEventSynchronize(e1);
+            float ms = 0;
+            hipEventElapsedTime(&ms, e0, e1);
+            if (rep > 0 && ms < best[t]) best[t] = ms;  // the first round warms up
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFreeAsync(cf, st);
+    hipFreeAsync(out, st);
+    MCTDHB_HIP_CHECK(hipStreamSynchronize(st));
+    if (rc) return rc;
+    S->hc_tiling = best[1] < best[0] ? 1 : 0;
+    return kOk;
+}
+
+unsigned hc_lanczos_blocks(const Space* S, int prec) {
+    const bool fused = S->norb >= 2 && S->norb <= kFusedMaxOrb;
+    if (fused && use_pair(S, prec)) return div_up((uint64_t) S->n_pairs + S->n_singles, kHcBlock);
+    return hc_grid_blocks(S->dim);
+}
+
 // ------------------------------------------------------- coefficient block
 
 // With sum_m n_m = N the last occupation is folded into the single-jump
@@ -294,6 +362,7 @@ int hc_apply(const Space* S, const void* coef, const double2* hob, const double2
         MCTDHB_HIP_CHECK(hipGetLastError());
         return kOk;
     }
+    if (prec == kPrecMixed && (rc = hc_tune_tiling(S, st))) return rc;
     const int M = S->norb;
     double2* cf64 = nullptr;
     float2* cf32 = nullptr;
@@ -341,6 +410,7 @@ int hc_apply_cf(const Space* S, const void* coef, const void* cf, void* hc, int 
         MCTDHB_HIP_CHECK(hipGetLastError());
         return kOk;
     }
+    if (prec == kPrecMixed && (rc = hc_tune_tiling(S, st))) return rc;
     const void* blk = prec == kPrecFP64 ? cf : (const void*) ((const double2*) cf + cf_size(M));
     return dispatch_fused_plain(S, coef, blk, hc, prec, index, st);
 }

@@ -85,11 +85,26 @@ int hc_lanczos(const Space* S, const double2* cf64, const float2* cf32, const La
                int index, hipStream_t st);
 
 unsigned hc_grid_blocks(uint32_t dim);
+// The mixed form has two tilings: one configuration per lane, or two
+// configurations of one run per lane (the pair work list of the space).
+inline bool pair_possible(const Space* S) { return S->d_slots && S->dim >= 2 && S->norb >= 2 && S->norb <= 7; }
+inline bool use_pair(const Space* S, int prec) { return prec == kPrecMixed && pair_possible(S) && S->hc_tiling == 1; }
+// Times both mixed tilings once on this space (zero data: the kernels'
+// cost does not depend on the values) and records the faster in
+// S->hc_tiling.  Synchronises the stream; no-op once done.
+int hc_tune_tiling(const Space* S, hipStream_t st);
+// Blocks of the H.C launch of the Lanczos form (one partial-sum row each).
+unsigned hc_lanczos_blocks(const Space* S, int prec);
+// The paired kernel reads c[-1] and c[dim] (zero padding).  Copy of c into
+// the space's padded staging (allocated on first use); returns element 0.
+const float2* hc_padded_copy(const Space* S, const float2* c, hipStream_t st);
 
 // Per-orbital-count entry points of the fused kernel (hc_m<M>.hip).  The
 // mixed form always uses on-the-fly indices.
 template <int M> struct HcFused {
     static int plain(const Space* S, const void* c, const void* cf, void* out, int prec, int index, hipStream_t st);
+    // mixed form with an explicit tiling (0 per lane, 1 pairs; c padded for 1)
+    static int plain_tiled(const Space* S, const void* c, const void* cf, void* out, int tiling, hipStream_t st);
     static int lanczos(const Space* S, const void* cf, const LanczosHcArgs& a, int prec, int index, hipStream_t st);
 };
 

@@ -256,24 +256,4 @@ static inline int launch_fused(const Space* S, const Zin* c, const cplx<R>* cf, 
     return kOk;
 }
 
-template <int M>
-int HcFused<M>::plain(const Space* S, const void* c, const void* cf, void* out, int prec, int index,
-                      hipStream_t st) {
-    if (prec == kPrecFP64)
-        return launch_fused<M, double, double2>(S, (const double2*) c, (const double2*) cf, index, 0, nullptr,
-                                                EpiPlain<double2>{(double2*) out}, st);
-    return launch_fused<M, float, float2>(S, (const float2*) c, (const float2*) cf, kIndexOnTheFly, 0, nullptr,
-                                          EpiPlain<float2>{(float2*) out}, st);
-}
-
-template <int M>
-int HcFused<M>::lanczos(const Space* S, const void* cf, const LanczosHcArgs& a, int prec, int index,
-                        hipStream_t st) {
-    if (prec == kPrecFP64)
-        return launch_fused<M, double, double2>(S, (const double2*) a.w[a.it], (const double2*) cf, index, 1, a.st,
-                                                EpiLanczos<double2>{a}, st);
-    return launch_fused<M, float, float2>(S, (const float2*) a.w[a.it], (const float2*) cf, kIndexOnTheFly, 1, a.st,
-                                          EpiLanczos<float2>{a}, st);
-}
-
 }  // namespace mctdhb

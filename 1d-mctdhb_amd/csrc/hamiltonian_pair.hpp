@@ -1,0 +1,419 @@
+// hamiltonian_pair.hpp - the mixed-precision H.C kernel on configuration
+// pairs.
+//
+// The gathers of the fused kernel are bound by the vector-memory address
+// path (each 64-lane gather instruction costs the texture addresser about
+// 16 cycles however wide it is).  Here a lane takes two configurations
+// i0, i0 + 1 of one run (the work list Space::d_slots): they share every
+// rank delta, so every jump target of i0 + 1 is the target of i0 plus one
+// and a single 16-byte load fetches the complex64 coefficients of both.
+// Occupations of i0 + 1 are those of i0 with one particle moved from orbital
+// 0 to orbital 1; only jumps out of orbitals 0 and 1 can be allowed for one
+// of the two and not the other, and the bosonic factors (zero for a jump out
+// of an empty orbital) discard that half.  Configurations left single at the
+// end of odd-length runs take the one-configuration path.
+#pragma once
+
+#include "hamiltonian_impl.hpp"
+
+namespace mctdhb {
+
+struct PairCtx {
+    const uint32_t* slots;
+    uint32_t n_pairs, n_slots;
+};
+
+// Gathers are raw buffer loads (one 16-byte load per pair that the compiler
+// cannot split, with the hardware range check as a safety net).  The
+// coefficient vector carries one zero element before and after it
+// (c[-1], c[dim]), so the window (j0, j0 + 1) of a pair, j0 in [-1, dim - 1],
+// is always readable and no gather needs a bounds test.
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+
+// Resource over c[-1 .. dim]; byte offset 8 (j + 1) addresses c[j].
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t padded_rsrc(const float2* c, uint32_t dim) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*) (c - 1), (short) 0, (int) ((dim + 2) * 8u), 0x00020000);
+}
+// Resource over c[0 .. dim).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t coef_rsrc(const float2* c, uint32_t dim) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*) c, (short) 0, (int) (dim * 8u), 0x00020000);
+}
+__device__ __forceinline__ float4 ld16(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
+}
+__device__ __forceinline__ float2 ld8(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const u32x2_t v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
+}
+
+__device__ __forceinline__ float2 lo2(const float4& v) { return make_float2(v.x, v.y); }
+__device__ __forceinline__ float2 hi2(const float4& v) { return make_float2(v.z, v.w); }
+
+template <int M>
+__device__ __forceinline__ void hc_pair_fused(uint32_t i0, const RowCtx& ctx, __amdgpu_buffer_rsrc_t c,
+                                              const float2* __restrict__ cf, const uint32_t* __restrict__ Wl,
+                                              int ldw, const float* __restrict__ sql, float2& out0,
+                                              float2& out1) {
+    using Z = float2;
+    // occupations of i0; those of i0 + 1 follow
+    int na[M], nb[M];
+#pragma unroll
+    for (int k = 0; k < M; k++) na[k] = ctx.ht[(size_t) i0 * M + k];
+#pragma unroll
+    for (int k = 0; k < M; k++) nb[k] = na[k];
+    nb[0] -= 1;
+    nb[1] += 1;
+    float ra[M], rb[M];
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        ra[k] = (float) na[k];
+        rb[k] = (float) nb[k];
+    }
+    // square roots sqrt(n-1), sqrt(n), sqrt(n+1), sqrt(n+2) (LDS table)
+    float am[M], a0[M], a1[M], a2[M], bm[M], b0[M], b1[M], b2[M];
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        am[k] = na[k] >= 1 ? sql[na[k] - 1] : 0.f;
+        a0[k] = sql[na[k]];
+        a1[k] = sql[na[k] + 1];
+        a2[k] = sql[na[k] + 2];
+        if (k < 2) {
+            bm[k] = nb[k] >= 1 ? sql[nb[k] - 1] : 0.f;
+            b0[k] = sql[nb[k]];
+            b1[k] = sql[nb[k] + 1];
+            b2[k] = sql[nb[k] + 2];
+        } else {
+            bm[k] = am[k];
+            b0[k] = a0[k];
+            b1[k] = a1[k];
+            b2[k] = a2[k];
+        }
+    }
+    // byte-offset rank deltas of i0 (shared by i0 + 1)
+    constexpr int MD = M > 1 ? M - 1 : 1;
+    uint32_t Dm2[MD], Dm1[MD], Dp1[MD], Dp2[MD];
+    {
+        int P = 0;
+#pragma unroll
+        for (int m = 0; m < M - 1; m++) {
+            P += na[m];
+            const uint32_t* Wm = Wl + m * ldw + kWOff + P;
+            const uint32_t w0 = Wm[0];
+            Dm2[m] = (Wm[-2] - w0) << 3;
+            Dm1[m] = (Wm[-1] - w0) << 3;
+            Dp1[m] = (Wm[1] - w0) << 3;
+            Dp2[m] = (Wm[2] - w0) << 3;
+        }
+    }
+    const uint32_t is = (i0 + 1) << 3;  // byte offset of c[i0] in the padded resource
+    auto fly = [&](int r1, int r2, int q1, int q2) {
+        uint32_t j = is;
+#pragma unroll
+        for (int m = 0; m < M - 1; m++) {
+            const int d = jump_delta(m, r1, r2, q1, q2);
+            if (d == -2) j -= Dm2[m];
+            else if (d == -1) j -= Dm1[m];
+            else if (d == 1) j -= Dp1[m];
+            else if (d == 2) j -= Dp2[m];
+        }
+        return j;
+    };
+
+    // diagonal
+    Z ea = mk<float>(0, 0), eb = mk<float>(0, 0);
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        const Z d1 = cf[cf_off_d1(M) + k], d2 = cf[cf_off_d2(M) + k];
+        ea.x += ra[k] * (d1.x + ra[k] * d2.x);
+        ea.y += ra[k] * (d1.y + ra[k] * d2.y);
+        eb.x += rb[k] * (d1.x + rb[k] * d2.x);
+        eb.y += rb[k] * (d1.y + rb[k] * d2.y);
+#pragma unroll
+        for (int s = k + 1; s < M; s++) {
+            const Z d3 = cf[cf_off_d3(M) + k * M + s];
+            const float nsa = ra[k] * ra[s], nsb = rb[k] * rb[s];
+            ea.x += nsa * d3.x;
+            ea.y += nsa * d3.y;
+            eb.x += nsb * d3.x;
+            eb.y += nsb * d3.y;
+        }
+    }
+    const float4 cc = ld16(c, is);
+    Z acc0 = cmul(ea, lo2(cc)), acc1 = cmul(eb, hi2(cc));
+
+    // single jumps k -> l; F_kl(i0 + 1) = F_kl(i0) - B'_kl0 + B'_kl1
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        const bool va = na[k] >= 1, vb = nb[k] >= 1;
+        if (va || vb) {
+#pragma unroll
+            for (int l = 0; l < M; l++) {
+                if (l == k) continue;
+                Z F = cf[cf_off_A(M) + k * M + l];
+#pragma unroll
+                for (int m = 0; m < M - 1; m++) {
+                    const Z B = cf[cf_off_B(M) + (k * M + l) * M + m];
+                    F.x += ra[m] * B.x;
+                    F.y += ra[m] * B.y;
+                }
+                Z Fb = F;
+                {
+                    const Z B0 = cf[cf_off_B(M) + (k * M + l) * M + 0];
+                    Fb.x -= B0.x;
+                    Fb.y -= B0.y;
+                    if (M > 2) {
+                        const Z B1 = cf[cf_off_B(M) + (k * M + l) * M + 1];
+                        Fb.x += B1.x;
+                        Fb.y += B1.y;
+                    }
+                }
+                const uint32_t off = fly(k, -1, l, -1);
+                const float4 v = ld16(c, off);
+                cmac<float>(acc0, F, cscale(lo2(v), a0[k] * a1[l]));
+                cmac<float>(acc1, Fb, cscale(hi2(v), b0[k] * b1[l]));
+            }
+        }
+    }
+    // creation factors of the two added particles (formed at each use: the
+    // vector ALU has room, the register file does not)
+    auto Pa = [&](int q, int l) { return a1[q] * (q == l ? a2[q] : a1[l]); };
+    auto Pb = [&](int q, int l) { return b1[q] * (q == l ? b2[q] : b1[l]); };
+
+    // two particles out of orbital k
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        const bool va = na[k] >= 2, vb = nb[k] >= 2;
+        if (va || vb) {
+            Z ga = mk<float>(0, 0), gb = mk<float>(0, 0);
+#pragma unroll
+            for (int q = 0; q < M; q++) {
+                if (q == k) continue;
+#pragma unroll
+                for (int l = q; l < M; l++) {
+                    if (l == k) continue;
+                    const Z G = cf[cf_off_ot(M) + (k * M + q) * M + l];
+                    const uint32_t off = fly(k, k, q, l);
+                    const float4 v = ld16(c, off);
+                    cmac<float>(ga, G, cscale(lo2(v), Pa(q, l)));
+                    cmac<float>(gb, G, cscale(hi2(v), Pb(q, l)));
+                }
+            }
+            const float fa = am[k] * a0[k], fb = bm[k] * b0[k];
+            acc0.x += fa * ga.x;
+            acc0.y += fa * ga.y;
+            acc1.x += fb * gb.x;
+            acc1.y += fb * gb.y;
+        }
+    }
+    // one particle out of each of k < s
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+#pragma unroll
+        for (int s = k + 1; s < M; s++) {
+            const bool va = na[k] >= 1 && na[s] >= 1, vb = nb[k] >= 1 && nb[s] >= 1;
+            if (va || vb) {
+                Z ga = mk<float>(0, 0), gb = mk<float>(0, 0);
+#pragma unroll
+                for (int q = 0; q < M; q++) {
+                    if (q == k || q == s) continue;
+#pragma unroll
+                    for (int l = q; l < M; l++) {
+                        if (l == k || l == s) continue;
+                        const Z G = cf[cf_off_tt(M) + ((k * M + s) * M + q) * M + l];
+                        const uint32_t off = fly(k, s, q, l);
+                        const float4 v = ld16(c, off);
+                        cmac<float>(ga, G, cscale(lo2(v), Pa(q, l)));
+                        cmac<float>(gb, G, cscale(hi2(v), Pb(q, l)));
+                    }
+                }
+                const float fa = a0[k] * a0[s], fb = b0[k] * b0[s];
+                acc0.x += fa * ga.x;
+                acc0.y += fa * ga.y;
+                acc1.x += fb * gb.x;
+                acc1.y += fb * gb.y;
+            }
+        }
+    }
+    out0 = acc0;
+    out1 = acc1;
+}
+
+// ------------------------------------------------------------- epilogues
+// Called by every thread of the block (the Lanczos form reduces over it);
+// has1 marks a lane that holds a pair, has0 a lane with any configuration.
+
+template <typename Zout> struct EpiPlainPair {
+    Zout* out;
+    __device__ __forceinline__ void operator()(uint32_t i0, bool has0, bool has1, double2 v0, double2 v1,
+                                               double* lds) const {
+        if (has0) out[i0] = from_d2<Zout>(v0);
+        if (has1) out[i0 + 1] = from_d2<Zout>(v1);
+    }
+};
+
+// The Lanczos epilogue on a pair (complex64 Krylov vectors; the two
+// entries of a vector are read with one 16-byte load).
+struct EpiLanczosPair {
+    LanczosHcArgs a;
+    uint32_t dim;
+    __device__ __forceinline__ void operator()(uint32_t i0, bool has0, bool has1, double2 v0, double2 v1,
+                                               double* lds) const {
+        const SilState* st = a.st;
+        const int it = a.it;
+        // entries i0 and i0 + 1 of a complex64 vector (the second zero
+        // unless this lane holds a pair)
+        auto pair_of = [&](const void* w, double2& x0, double2& x1) {
+            const __amdgpu_buffer_rsrc_t r = coef_rsrc((const float2*) w, dim);
+            if (has1) {
+                const float4 q = ld16(r, i0 << 3);
+                x0 = make_double2(q.x, q.y);
+                x1 = make_double2(q.z, q.w);
+            } else {
+                const float2 q = ld8(r, i0 << 3);
+                x0 = make_double2(q.x, q.y);
+                x1 = make_double2(0, 0);
+            }
+        };
+        double2 o0 = make_double2(0, 0), o1 = make_double2(0, 0);
+        const double s_it = st->s[it];
+        if (has0) {
+            o0 = cscale(v0, s_it);
+            if (has1) o1 = cscale(v1, s_it);
+            if (it > 0) {
+                const double bp = st->beta[it - 1] * st->s[it - 1];
+                double2 x0, x1;
+                pair_of(a.w[it - 1], x0, x1);
+                o0 = o0 - cscale(x0, bp);
+                o1 = o1 - cscale(x1, bp);
+            }
+            if (!a.last) {
+                a.out[i0] = o0;
+                if (has1) a.out[i0 + 1] = o1;
+            }
+        }
+        double* part = a.part + blockIdx.x;
+        const size_t ps = gridDim.x;
+        for (int k = a.last ? it : 0; k <= it; k++) {
+            double v[2] = {0, 0};
+            if (has0) {
+                double2 x0, x1;
+                pair_of(a.w[k], x0, x1);
+                const double2 p = cconjmul(x0, o0) + cconjmul(x1, o1);
+                v[0] = p.x * st->s[k];
+                v[1] = p.y * st->s[k];
+            }
+            block_sum<2>(v, lds);
+            if (threadIdx.x == 0) {
+                part[(2 * k) * ps] = v[0];
+                part[(2 * k + 1) * ps] = v[1];
+            }
+        }
+        if (a.last) return;
+        double v1s[1] = {0};
+        if (has0 && it == 0) {
+            double2 x0, x1;
+            pair_of(a.w[0], x0, x1);
+            v1s[0] = x0.x * x0.x + x0.y * x0.y + x1.x * x1.x + x1.y * x1.y;
+        }
+        block_sum<1>(v1s, lds);
+        if (threadIdx.x == 0) {
+            part[(2 * (it + 1)) * ps] = v1s[0];
+            part[(2 * (it + 1) + 1) * ps] = 0;
+        }
+    }
+};
+
+// One lane per slot: pairs through hc_pair_fused, singles through the
+// one-configuration row.
+template <int M, typename Epi>
+__global__ __launch_bounds__(kHcBlock) void k_hc_pair(RowCtx ctx, PairCtx pc, const float2* __restrict__ c,
+                                                      const float2* __restrict__ cf, const uint32_t* __restrict__ Wg,
+                                                      int ldw, int stop_check, const SilState* st, Epi epi) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    if (stop_check && st->stop) return;
+    double* red = (double*) smem;
+    uint32_t* Wl = (uint32_t*) (smem + 64);
+    const int wsz = (M - 1) * ldw;
+    float* sql = (float*) (smem + 64 + ((wsz * 4 + 15) & ~15));
+    for (int t = threadIdx.x; t < wsz; t += blockDim.x) Wl[t] = Wg[t];
+    for (int t = threadIdx.x; t < ctx.npar + 3; t += blockDim.x) sql[t] = sqrtf((float) t);
+    __syncthreads();
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t slot = b * kHcBlock + threadIdx.x;
+    const bool has0 = slot < pc.n_slots;
+    const bool has1 = slot < pc.n_pairs;
+    const uint32_t i0 = has0 ? pc.slots[slot] : 0;
+    float2 r0 = make_float2(0, 0), r1 = make_float2(0, 0);
+    if (has1) {
+        hc_pair_fused<M>(i0, ctx, padded_rsrc(c, ctx.dim), cf, Wl, ldw, sql, r0, r1);
+    } else if (has0) {
+        r0 = hc_row_fused<M, float, kIdxFly, float2>(i0, ctx, c, cf, Wl, ldw, sql);
+    }
+    epi(i0, has0, has1, to_d2(r0), to_d2(r1), red);
+}
+
+template <int M, typename Epi>
+static inline int launch_pair(const Space* S, const float2* c, const float2* cf, int stop_check, const SilState* st,
+                              Epi epi, hipStream_t stream) {
+    const size_t smem = fused_smem(S, sizeof(float));
+    if (smem > 64 * 1024) {
+        set_error("rank and square-root tables exceed the LDS budget of the fused H.C kernel for this (N, M)");
+        return kInvalidArgument;
+    }
+    if ((uint64_t) S->dim * sizeof(float2) >= (1ull << 32)) {
+        set_error("coefficient vector above 4 GiB: the fused H.C kernel needs 32-bit byte offsets");
+        return kInvalidArgument;
+    }
+    PairCtx pc;
+    pc.slots = S->d_slots;
+    pc.n_pairs = S->n_pairs;
+    pc.n_slots = S->n_pairs + S->n_singles;
+    k_hc_pair<M, Epi><<<hc_lanczos_blocks(S, kPrecMixed), kHcBlock, smem, stream>>>(row_ctx(S), pc, c, cf, S->d_W,
+                                                                                   S->ldw, stop_check, st, epi);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
+template <int M>
+int HcFused<M>::plain(const Space* S, const void* c, const void* cf, void* out, int prec, int index,
+                      hipStream_t st) {
+    if (prec == kPrecFP64)
+        return launch_fused<M, double, double2>(S, (const double2*) c, (const double2*) cf, index, 0, nullptr,
+                                                EpiPlain<double2>{(double2*) out}, st);
+    if (use_pair(S, prec)) {
+        const float2* cp = hc_padded_copy(S, (const float2*) c, st);
+        if (!cp) {
+            set_error("padded staging of the coefficient vector failed");
+            return kDeviceError;
+        }
+        return launch_pair<M>(S, cp, (const float2*) cf, 0, nullptr, EpiPlainPair<float2>{(float2*) out}, st);
+    }
+    return launch_fused<M, float, float2>(S, (const float2*) c, (const float2*) cf, kIndexOnTheFly, 0, nullptr,
+                                          EpiPlain<float2>{(float2*) out}, st);
+}
+
+template <int M>
+int HcFused<M>::plain_tiled(const Space* S, const void* c, const void* cf, void* out, int tiling, hipStream_t st) {
+    if (tiling == 1)
+        return launch_pair<M>(S, (const float2*) c, (const float2*) cf, 0, nullptr, EpiPlainPair<float2>{(float2*) out},
+                              st);
+    return launch_fused<M, float, float2>(S, (const float2*) c, (const float2*) cf, kIndexOnTheFly, 0, nullptr,
+                                          EpiPlain<float2>{(float2*) out}, st);
+}
+
+template <int M>
+int HcFused<M>::lanczos(const Space* S, const void* cf, const LanczosHcArgs& a, int prec, int index,
+                        hipStream_t st) {
+    if (prec == kPrecFP64)
+        return launch_fused<M, double, double2>(S, (const double2*) a.w[a.it], (const double2*) cf, index, 1, a.st,
+                                                EpiLanczos<double2>{a}, st);
+    if (use_pair(S, prec))
+        return launch_pair<M>(S, (const float2*) a.w[a.it], (const float2*) cf, 1, a.st, EpiLanczosPair{a, S->dim},
+                              st);
+    return launch_fused<M, float, float2>(S, (const float2*) a.w[a.it], (const float2*) cf, kIndexOnTheFly, 1, a.st,
+                                          EpiLanczos<float2>{a}, st);
+}
+
+}  // namespace mctdhb

@@ -1,5 +1,5 @@
 // hc_m4.hip - fused H.C kernels instantiated for M = 4 orbitals.
-#include "hamiltonian_impl.hpp"
+#include "hamiltonian_pair.hpp"
 
 namespace mctdhb {
 template struct HcFused<4>;

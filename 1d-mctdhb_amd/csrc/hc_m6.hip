@@ -1,5 +1,5 @@
 // hc_m6.hip - fused H.C kernels instantiated for M = 6 orbitals.
-#include "hamiltonian_impl.hpp"
+#include "hamiltonian_pair.hpp"
 
 namespace mctdhb {
 template struct HcFused<6>;

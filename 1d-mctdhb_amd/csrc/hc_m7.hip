@@ -1,5 +1,5 @@
 // hc_m7.hip - fused H.C kernels instantiated for M = 7 orbitals.
-#include "hamiltonian_impl.hpp"
+#include "hamiltonian_pair.hpp"
 
 namespace mctdhb {
 template struct HcFused<7>;

@@ -462,6 +462,10 @@ int sil_create(const Space* S, int lm, int prec, int index, Sil** out) {
         set_error("this SIL form needs the jump tables; build the space with tables");
         return kNotBuilt;
     }
+    if (prec == kPrecMixed) {
+        const int rc = hc_tune_tiling(S, 0);
+        if (rc) return rc;
+    }
     Sil* L = new Sil();
     L->S = S;
     L->lm = lm;
@@ -472,7 +476,7 @@ int sil_create(const Space* S, int lm, int prec, int index, Sil** out) {
     const size_t zw = prec == kPrecFP64 ? sizeof(double2) : sizeof(float2);
     const int M = S->norb;
     const size_t cfn = generic ? (size_t) M * M + (size_t) M * M * M * M : (size_t) cf_size(M);
-    const unsigned nbh = hc_grid_blocks(S->dim);
+    const unsigned nbh = std::max(hc_grid_blocks(S->dim), hc_lanczos_blocks(S, prec));
     L->part_cap = std::max<size_t>((size_t) nbh * lanczos_hc_slots(kMaxKrylov),
                                    (size_t) kStreamGrid * (2 * kMaxKrylov + 2));
     bool ok = hipMalloc((void**) &L->d_st, sizeof(SilState)) == hipSuccess;
@@ -481,8 +485,14 @@ int sil_create(const Space* S, int lm, int prec, int index, Sil** out) {
     ok = ok && hipMalloc((void**) &L->out, sizeof(double2) * n) == hipSuccess;
     ok = ok && hipMalloc((void**) &L->part, sizeof(double) * L->part_cap) == hipSuccess;
     ok = ok && hipMalloc((void**) &L->tot, sizeof(double) * (2 * kMaxKrylov + 4)) == hipSuccess;
-    for (int k = 1; k < lm && ok; k++) ok = hipMalloc(&L->w[k], zw * n) == hipSuccess;
-    if (prec == kPrecMixed) ok = ok && hipMalloc(&L->w0f, sizeof(float2) * n) == hipSuccess;
+    auto padded = [&](void** base, void** vec, size_t esz) {
+        if (hipMalloc(base, esz * (n + 2)) != hipSuccess) return false;
+        if (hipMemset(*base, 0, esz * (n + 2)) != hipSuccess) return false;
+        *vec = (char*) *base + esz;
+        return true;
+    };
+    for (int k = 1; k < lm && ok; k++) ok = padded(&L->wbase[k], &L->w[k], zw);
+    if (prec == kPrecMixed) ok = ok && padded(&L->w0fbase, &L->w0f, sizeof(float2));
     if (!ok) {
         sil_destroy(L);
         set_error("device allocation failed for the Lanczos workspace");
@@ -500,8 +510,8 @@ void sil_destroy(Sil* L) {
     hipFree(L->out);
     hipFree(L->part);
     hipFree(L->tot);
-    for (int k = 1; k < kMaxKrylov; k++) hipFree(L->w[k]);
-    hipFree(L->w0f);
+    for (int k = 1; k < kMaxKrylov; k++) hipFree(L->wbase[k]);
+    hipFree(L->w0fbase);
     delete L;
 }
 
@@ -585,7 +595,8 @@ static int sil_run_lanczos(Sil* L, const double2* C, const double2* hob, const d
     } else {
         L->w[0] = (void*) C;
     }
-    const unsigned nbh = hc_grid_blocks(n);
+    if (L->prec == kPrecMixed && (rc = hc_tune_tiling(S, st))) return rc;
+    const unsigned nbh = hc_lanczos_blocks(S, L->prec);
     for (int it = 0; it < L->lm; it++) {
         LanczosHcArgs a{};
         for (int k = 0; k <= it; k++) a.w[k] = L->w[k];

@@ -13,8 +13,13 @@ struct Sil {
     SilState* d_st = nullptr;
     double2* cf64 = nullptr;  // fused coefficient block, or [hob | hint] for the generic kernel
     float2* cf32 = nullptr;
-    void* w[kMaxKrylov] = {};  // w[0] is C (FP64) or its complex64 copy
+    // w[0] is C (FP64) or its complex64 copy w0f.  Every vector this
+    // workspace allocates has one zero element before and after it (the
+    // paired H.C kernel reads w[-1] and w[dim]); wbase[] are the allocations.
+    void* w[kMaxKrylov] = {};
+    void* wbase[kMaxKrylov] = {};
     void* w0f = nullptr;
+    void* w0fbase = nullptr;
     double2* out = nullptr;
     double* part = nullptr;
     size_t part_cap = 0;

@@ -175,6 +175,29 @@ __global__ void k_scan_apply(const uint32_t* __restrict__ cnt, uint32_t n, const
     }
 }
 
+// Pair slots.  Within a run (fixed P_1 .. P_{M-2}) the rank grows by one as
+// P_0 = n_0 falls by one, i.e. as n_1 grows: run position = n_1.  A
+// configuration at an even position starts a pair when the run goes on
+// (n_0 >= 1) and is single otherwise.
+__global__ void k_slot_flags(int M, uint32_t dim, const uint16_t* __restrict__ ht, uint32_t* __restrict__ pf,
+                             uint32_t* __restrict__ sf) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= dim) return;
+    const uint32_t n0 = ht[(size_t) i * M], n1 = ht[(size_t) i * M + 1];
+    const bool even = (n1 & 1u) == 0;
+    pf[i] = even && n0 >= 1;
+    sf[i] = even && n0 == 0;
+}
+
+__global__ void k_slot_scatter(uint32_t dim, const uint32_t* __restrict__ pf, const uint32_t* __restrict__ sf,
+                               const uint32_t* __restrict__ pp, const uint32_t* __restrict__ sp, uint32_t npairs,
+                               uint32_t* __restrict__ slots) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= dim) return;
+    if (pf[i]) slots[pp[i]] = i;
+    if (sf[i]) slots[npairs + sp[i]] = i;
+}
+
 // ------------------------------------------------------------ host driver
 
 static int scan_strides(const uint32_t* d_cnt, uint32_t n, uint32_t m2, uint32_t* d_stride, uint64_t* total,
@@ -189,6 +212,48 @@ static int scan_strides(const uint32_t* d_cnt, uint32_t n, uint32_t m2, uint32_t
     MCTDHB_HIP_CHECK(hipStreamSynchronize(st));
     MCTDHB_HIP_CHECK(hipFreeAsync(d_sums, st));
     return kOk;
+}
+
+static int build_pair_slots(Space* S, hipStream_t st) {
+    const uint32_t dim = S->dim;
+    uint32_t *pf = nullptr, *sf = nullptr, *pp = nullptr, *sp = nullptr;
+    const size_t b = sizeof(uint32_t) * dim;
+    if (hipMalloc((void**) &pf, b) != hipSuccess || hipMalloc((void**) &sf, b) != hipSuccess ||
+        hipMalloc((void**) &pp, b) != hipSuccess || hipMalloc((void**) &sp, b) != hipSuccess) {
+        hipFree(pf);
+        hipFree(sf);
+        hipFree(pp);
+        set_error("device allocation failed (pair slots)");
+        return kOutOfMemory;
+    }
+    const int tb = 256;
+    k_slot_flags<<<div_up(dim, tb), tb, 0, st>>>(S->norb, dim, S->d_ht, pf, sf);
+    uint64_t npairs = 0, nsingles = 0;
+    int rc = scan_strides(pf, dim, 1, pp, &npairs, st);
+    if (!rc) rc = scan_strides(sf, dim, 1, sp, &nsingles, st);
+    if (!rc && 2 * npairs + nsingles != dim) {
+        set_error("internal error: pair slots do not cover the space");
+        rc = kNumerical;
+    }
+    if (!rc && hipMalloc((void**) &S->d_slots, sizeof(uint32_t) * (npairs + nsingles + 1)) != hipSuccess) {
+        set_error("device allocation failed (pair slots)");
+        rc = kOutOfMemory;
+    }
+    if (!rc) {
+        S->n_pairs = (uint32_t) npairs;
+        S->n_singles = (uint32_t) nsingles;
+        k_slot_scatter<<<div_up(dim, tb), tb, 0, st>>>(dim, pf, sf, pp, sp, S->n_pairs, S->d_slots);
+        if (hipGetLastError() != hipSuccess) {
+            set_error("pair slot scatter failed to launch");
+            rc = kDeviceError;
+        }
+    }
+    hipStreamSynchronize(st);
+    hipFree(pf);
+    hipFree(sf);
+    hipFree(pp);
+    hipFree(sp);
+    return rc;
 }
 
 int space_create(int npar, int norb, int flags, Space** out) {
@@ -295,6 +360,13 @@ int space_create(int npar, int norb, int flags, Space** out) {
         MCTDHB_HIP_CHECK(hipGetLastError());
         S->tables = true;
     }
+    if (norb >= 2) {
+        rc = build_pair_slots(S, st);
+        if (rc) {
+            space_destroy(S);
+            return rc;
+        }
+    }
 #undef ALLOC
     MCTDHB_HIP_CHECK(hipDeviceSynchronize());
     *out = S;
@@ -311,6 +383,8 @@ void space_destroy(Space* S) {
     hipFree(S->d_maptt);
     hipFree(S->d_strideot);
     hipFree(S->d_stridett);
+    hipFree(S->d_slots);
+    hipFree(S->d_pad);
     delete S;
 }
 

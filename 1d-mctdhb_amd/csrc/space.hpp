@@ -28,6 +28,19 @@ struct Space {
     uint32_t* d_stridett = nullptr;
     uint64_t mapot_len = 0, maptt_len = 0;
     bool tables = false;
+    // Work list of the paired H.C kernel (2 <= M): slots[0, n_pairs) are the
+    // configurations i that start a pair (i, i + 1) of one run (fixed
+    // n_2 .. n_{M-1}, ranks consecutive as one particle moves from orbital 0
+    // to orbital 1), slots[n_pairs, n_pairs + n_singles) the configurations
+    // left alone at the end of odd-length runs; all in rank order.
+    uint32_t* d_slots = nullptr;
+    uint32_t n_pairs = 0, n_singles = 0;
+    // complex64 staging with one zero element on each side (dim + 2), for
+    // callers of the paired kernel whose vectors are not padded
+    mutable void* d_pad = nullptr;
+    // tiling of the mixed H.C kernel picked by timing both on this space:
+    // -1 not yet timed, 0 one configuration per lane, 1 configuration pairs
+    mutable int hc_tiling = -1;
 };
 
 int space_dimension_exact(int npar, int norb, uint32_t* dim);

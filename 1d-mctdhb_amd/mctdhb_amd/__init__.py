@@ -92,6 +92,7 @@ def lib():
     L.mctdhb_hc_coefficients_dev.argtypes = [_vp, _vp, _vp, _vp, _i, _vp]
     L.mctdhb_apply_hamiltonian_cf_dev.argtypes = [_vp, _vp, _vp, _vp, _i, _i, _vp]
     L.mctdhb_density_dev.argtypes = [_vp, _vp, _vp, _vp, _i, _i, _vp]
+    L.mctdhb_hc_tiling.argtypes = [_vp, _i, C.POINTER(_i)]
     L.mctdhb_sil_create.argtypes = [_vp, _i, _i, _i, _cpp]
     L.mctdhb_sil_destroy.argtypes = [_vp]
     L.mctdhb_sil_device_bytes.argtypes = [_vp, _i, _i]
@@ -209,6 +210,13 @@ class DeviceSpace:
     def density(self, coef, rho1, rho2=None, prec=FP64, index=INDEX_FLY, stream=None):
         check(lib().mctdhb_density_dev(self._h, _ptr(coef), _ptr(rho1), _ptr(rho2), prec, index,
                                        _stream(stream)))
+
+    def hc_tiling(self, set: int = -2) -> int:
+        """Tiling of the mixed H.C kernel (0 per lane, 1 configuration
+        pairs); set -1 to time both again, 0/1 to force."""
+        t = _i()
+        check(lib().mctdhb_hc_tiling(self._h, set, C.byref(t)))
+        return t.value
 
     def eig_residual(self, coef, hob, hint, e, stream=None) -> float:
         """max_i |e c_i - (H c)_i| / N (reference eig_residual,

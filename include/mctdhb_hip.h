@@ -183,6 +183,14 @@ int    mctdhb_hc_coefficients_dev(const mctdhb_space* space, const void* hob, co
 int    mctdhb_apply_hamiltonian_cf_dev(const mctdhb_space* space, const void* coef, const void* cf, void* hc,
                                        int prec, int index, void* stream);
 
+/* Tiling of the mixed H.C kernel on this space: 0 one configuration per
+ * lane, 1 two configurations of one run per lane (one 16-byte gather serves
+ * both).  By default both are timed at the first mixed call and the faster
+ * kept.  set: -2 query only, -1 time both again, 0 or 1 force; *tiling
+ * (may be NULL) receives the current choice (-1: not yet timed).  Callers
+ * that change it must not have a mixed SIL workspace open on the space. */
+int mctdhb_hc_tiling(mctdhb_space* space, int set, int* tiling);
+
 /* rho1 (M x M, <a+_k a_l>) and rho2 (M^4, <a+_k a+_s a_q a_l>), complex128;
  * rho2 may be NULL.  coef complex128 (FP64) or complex64 (MIXED). */
 int mctdhb_density_dev(mctdhb_space* space, const void* coef, void* rho1, void* rho2, int prec,

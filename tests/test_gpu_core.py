@@ -73,13 +73,18 @@ def test_hc_fp64_matches_oracle(amd, orc, npar, norb):
         assert np.array_equal(got_f, got_t)
 
 
-@pytest.mark.parametrize("npar,norb", [(9, 2), (10, 4), (10, 5), (5, 7), (40, 4)])
-def test_hc_mixed_matches_oracle(amd, orc, npar, norb):
-    print(f"hc mixed N={npar} M={norb}", flush=True)
+# small spaces put many configurations next to the first and last ranks and
+# at the ends of odd-length runs (the single slots of the paired kernel)
+@pytest.mark.parametrize("npar,norb", [(9, 2), (10, 4), (10, 5), (5, 7), (40, 4), (1, 2), (2, 2), (1, 3),
+                                       (2, 3), (3, 4), (2, 7), (1, 5), (7, 6), (60, 3)])
+@pytest.mark.parametrize("tiling", [0, 1])
+def test_hc_mixed_matches_oracle(amd, orc, npar, norb, tiling):
+    print(f"hc mixed N={npar} M={norb} tiling={tiling}", flush=True)
     S = orc.Space(npar, norb)
     c, h, g = _problem(npar, norb)
     ref = S.apply_hamiltonian(c, h, g)
     D = amd.DeviceSpace(npar, norb, tables=False)
+    assert D.hc_tiling(tiling) == tiling
     out = gu.empty(S.dim, "complex64")
     D.apply_hamiltonian(gu.dev(c, np.complex64), gu.dev(h), gu.dev(g), out, prec=amd.MIXED,
                         index=amd.INDEX_FLY)
@@ -185,6 +190,24 @@ def test_eig_residual_matches_definition(amd, orc, npar, norb):
         w, v = np.linalg.eigh(np.array([S.apply_hamiltonian(np.eye(S.dim)[k], h, g) for k in range(S.dim)]).T)
         r0 = D.eig_residual(gu.dev(np.ascontiguousarray(v[:, 0])), gu.dev(h), gu.dev(g), w[0])
         assert r0 < 1e-10
+
+
+@pytest.mark.parametrize("npar,norb,lm,imag", [(10, 5, 8, False), (9, 2, 5, True), (2, 3, 3, False),
+                                               (6, 7, 6, True), (30, 4, 18, False)])
+@pytest.mark.parametrize("tiling", [0, 1])
+def test_sil_step_mixed_matches_oracle(amd, orc, npar, norb, lm, imag, tiling):
+    print(f"sil mixed N={npar} M={norb} lm={lm} imag={imag} tiling={tiling}", flush=True)
+    S = orc.Space(npar, norb)
+    c, h, g = _problem(npar, norb, 43)
+    dt = -0.01j if imag else 0.01
+    _, ref = S.sil_step(c, h, g, lm, dt, imag)
+    D = amd.DeviceSpace(npar, norb, tables=False)
+    D.hc_tiling(tiling)
+    sil = amd.DeviceSil(D, lm, prec=amd.MIXED, index=amd.INDEX_FLY)
+    dc = gu.dev(c)
+    sil.step(dc, gu.dev(h), gu.dev(g), dt, imag)
+    # complex64 Krylov vectors and H.C, FP64 scalars and recombination
+    assert gu.rel_err(gu.host(dc), ref) < 2e-5
 
 
 def test_lanczos_breakdown_on_device(amd, orc):
