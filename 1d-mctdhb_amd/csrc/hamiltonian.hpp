@@ -95,6 +95,12 @@ inline bool use_pair(const Space* S, int prec) { return prec == kPrecMixed && pa
 int hc_tune_tiling(const Space* S, hipStream_t st);
 // Blocks of the H.C launch of the Lanczos form (one partial-sum row each).
 unsigned hc_lanczos_blocks(const Space* S, int prec);
+// Whether the last Lanczos iteration runs the half-row Rayleigh form (the
+// one-configuration-per-lane fused kernel): slot 2 it then holds
+// <w_it, H w_it> instead of <v_it, out>.
+inline bool hc_lanczos_half(const Space* S, int prec) {
+    return S->norb >= 2 && S->norb <= kFusedMaxOrb && !use_pair(S, prec);
+}
 // The paired kernel reads c[-1] and c[dim] (zero padding).  Copy of c into
 // the space's padded staging (allocated on first use); returns element 0.
 const float2* hc_padded_copy(const Space* S, const float2* c, hipStream_t st);

@@ -32,7 +32,14 @@ template <typename R> __device__ __forceinline__ void cmac(cplx<R>& acc, const c
     acc.y += G.y * t.x;
 }
 
-template <int M, typename R, int IDX, typename Zin>
+// Row i of H.C.  With HALF, the row's share of the Rayleigh quotient
+// <c, H c> instead: H is Hermitian and the jump types come in inverse pairs
+// (remove R, add A) / (remove A, add R), so sum_i conj(c_i) (H c)_i =
+// sum_i E_i |c_i|^2 + 2 Re sum_i conj(c_i) sum_{T in half} H(i, T(i)) c_T(i),
+// keeping of each pair the type whose sorted removal list comes first
+// (k < l for k -> l, k < q for the double jumps).  The real result is
+// returned in .x.
+template <int M, typename R, int IDX, typename Zin, bool HALF = false>
 __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, const Zin* __restrict__ c,
                                                 const cplx<R>* __restrict__ cf, const uint32_t* __restrict__ Wl,
                                                 int ldw, const R* __restrict__ sql) {
@@ -59,7 +66,8 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
             e.y += ns * d3.y;
         }
     }
-    Z acc = cmul(e, gather_b<R, Zin>(c, x.is));
+    const Z ci = gather_b<R, Zin>(c, x.is);
+    Z acc = HALF ? mk<R>(0, 0) : cmul(e, ci);
 
     // single jumps k -> l: F_kl = A'_kl + sum_{m < M-1} n_m B'_klm (the last
     // occupation folded in through sum_m n_m = N when the block is built)
@@ -68,7 +76,7 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
         if (n[k] >= 1) {
 #pragma unroll
             for (int l = 0; l < M; l++) {
-                if (l == k) continue;
+                if (l == k || (HALF && l < k)) continue;
                 Z F = cf[cf_off_A(M) + k * M + l];
 #pragma unroll
                 for (int m = 0; m < M - 1; m++) {
@@ -97,7 +105,7 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
             Z g = mk<R>(0, 0);
 #pragma unroll
             for (int q = 0; q < M; q++) {
-                if (q == k) continue;
+                if (q == k || (HALF && q < k)) continue;
 #pragma unroll
                 for (int l = q; l < M; l++) {
                     if (l == k) continue;
@@ -119,7 +127,7 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
                 Z g = mk<R>(0, 0);
 #pragma unroll
                 for (int q = 0; q < M; q++) {
-                    if (q == k || q == s) continue;
+                    if (q == k || q == s || (HALF && q < k)) continue;
 #pragma unroll
                     for (int l = q; l < M; l++) {
                         if (l == k || l == s) continue;
@@ -132,6 +140,11 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
                 acc.y += a * g.y;
             }
         }
+    }
+    if constexpr (HALF) {
+        // E_i |c_i|^2 + 2 Re(conj(c_i) acc)
+        const R n2 = ci.x * ci.x + ci.y * ci.y;
+        return mk<R>(e.x * n2 + 2 * (ci.x * acc.x + ci.y * acc.y), 0);
     }
     return acc;
 }
@@ -192,9 +205,24 @@ template <typename Zw> struct EpiLanczos {
     }
 };
 
+// Last Lanczos iteration with the half-row form: only <w_it, H w_it> (real)
+// into slot 2 it; alpha is formed from it in the scalar update.
+struct EpiRayleigh {
+    LanczosHcArgs a;
+    __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 row, double* lds) const {
+        double v[1] = {valid ? row.x : 0.0};
+        block_sum<1>(v, lds);
+        if (threadIdx.x == 0) {
+            const int it = a.it;
+            a.part[(size_t) (2 * it) * gridDim.x + blockIdx.x] = v[0];
+            a.part[(size_t) (2 * it + 1) * gridDim.x + blockIdx.x] = 0;
+        }
+    }
+};
+
 // ----------------------------------------------------------------- kernels
 
-template <int M, typename R, int IDX, typename Zin, typename Epi>
+template <int M, typename R, int IDX, typename Zin, typename Epi, bool HALF = false>
 __global__ __launch_bounds__(kHcBlock) void k_hc_fused(RowCtx ctx, const Zin* __restrict__ c,
                                                        const cplx<R>* __restrict__ cf,
                                                        const uint32_t* __restrict__ Wg, int ldw,
@@ -214,7 +242,7 @@ __global__ __launch_bounds__(kHcBlock) void k_hc_fused(RowCtx ctx, const Zin* __
     const uint32_t i = b * kHcBlock + threadIdx.x;
     const bool valid = i < ctx.dim;
     double2 v = make_double2(0, 0);
-    if (valid) v = to_d2(hc_row_fused<M, R, IDX, Zin>(i, ctx, c, cf, Wl, ldw, sql));
+    if (valid) v = to_d2(hc_row_fused<M, R, IDX, Zin, HALF>(i, ctx, c, cf, Wl, ldw, sql));
     epi(i, valid, v, red);
 }
 
@@ -229,7 +257,7 @@ static inline size_t fused_smem(const Space* S, size_t rsize) {
     return 64 + ((wsz + 15) & ~(size_t) 15) + rsize * (S->npar + 3);
 }
 
-template <int M, typename R, typename Zin, typename Epi>
+template <int M, typename R, typename Zin, typename Epi, bool HALF = false>
 static inline int launch_fused(const Space* S, const Zin* c, const cplx<R>* cf, int index, int stop_check,
                         const SilState* st, Epi epi, hipStream_t stream) {
     const bool fly = index == kIndexOnTheFly;
@@ -245,11 +273,11 @@ static inline int launch_fused(const Space* S, const Zin* c, const cplx<R>* cf, 
     const unsigned nb = hc_grid_blocks(S->dim);
     RowCtx ctx = row_ctx(S);
     if (fly) {
-        k_hc_fused<M, R, kIdxFly, Zin, Epi><<<nb, kHcBlock, smem, stream>>>(
+        k_hc_fused<M, R, kIdxFly, Zin, Epi, HALF><<<nb, kHcBlock, smem, stream>>>(
             ctx, c, cf, S->d_W, S->ldw, stop_check, st, epi);
     } else {
         if constexpr (std::is_same<R, double>::value)
-            k_hc_fused<M, R, kIdxTables, Zin, Epi><<<nb, kHcBlock, smem, stream>>>(
+            k_hc_fused<M, R, kIdxTables, Zin, Epi, HALF><<<nb, kHcBlock, smem, stream>>>(
                 ctx, c, cf, S->d_W, S->ldw, stop_check, st, epi);
     }
     MCTDHB_HIP_CHECK(hipGetLastError());

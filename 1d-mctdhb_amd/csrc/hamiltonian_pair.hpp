@@ -406,6 +406,14 @@ int HcFused<M>::plain_tiled(const Space* S, const void* c, const void* cf, void*
 template <int M>
 int HcFused<M>::lanczos(const Space* S, const void* cf, const LanczosHcArgs& a, int prec, int index,
                         hipStream_t st) {
+    if (a.last && hc_lanczos_half(S, prec)) {
+        if (prec == kPrecFP64)
+            return launch_fused<M, double, double2, EpiRayleigh, true>(S, (const double2*) a.w[a.it],
+                                                                       (const double2*) cf, index, 1, a.st,
+                                                                       EpiRayleigh{a}, st);
+        return launch_fused<M, float, float2, EpiRayleigh, true>(S, (const float2*) a.w[a.it], (const float2*) cf,
+                                                                 kIndexOnTheFly, 1, a.st, EpiRayleigh{a}, st);
+    }
     if (prec == kPrecFP64)
         return launch_fused<M, double, double2>(S, (const double2*) a.w[a.it], (const double2*) cf, index, 1, a.st,
                                                 EpiLanczos<double2>{a}, st);

@@ -64,10 +64,20 @@ __global__ __launch_bounds__(256) void k_reduce_slots(const double* __restrict__
 
 // After the H.C pass of iteration it: alpha_it and the reorthogonalisation
 // coefficients (on w_k) for the next streaming pass.
-// On the last iteration only alpha is formed.
+// On the last iteration only alpha is formed.  last == 2: the H.C pass
+// summed <w_it, H w_it> (the half-row form), and alpha = <v_it, H v_it -
+// beta v_{it-1}> = s_it^2 <w_it, H w_it> - beta conj(<v_{it-1}, v_it>).
 __device__ void sil_after_hc(SilState* st, const volatile double* tot, int it, int last) {
     if (it == 0) st->gram[0] = make_double2(tot[2 * (it + 1)], 0);  // <v_0, v_0> = |C|^2
-    const double2 alpha = make_double2(tot[2 * it], tot[2 * it + 1]);
+    double2 alpha = make_double2(tot[2 * it], tot[2 * it + 1]);
+    if (last == 2) {
+        const double s = st->s[it];
+        alpha = make_double2(s * s * tot[2 * it], 0);
+        if (it > 0) {
+            const double2 gp = st->gram[it - 1];
+            alpha = alpha - make_double2(st->beta[it - 1] * gp.x, -st->beta[it - 1] * gp.y);
+        }
+    }
     st->alpha[it] = alpha;
     if (last) return;
     for (int k = 0; k <= it; k++) {
@@ -608,7 +618,8 @@ static int sil_run_lanczos(Sil* L, const double2* C, const double2* hob, const d
         rc = hc_lanczos(S, L->cf64, L->cf32, a, L->prec, L->index, st);
         if (rc) return rc;
         if (a.last)
-            k_reduce_scalars<0><<<2, kReduceBlock, 0, st>>>(L->part, nbh, L->tot, L->d_st, it, 2 * it, 1);
+            k_reduce_scalars<0><<<2, kReduceBlock, 0, st>>>(L->part, nbh, L->tot, L->d_st, it, 2 * it,
+                                                             hc_lanczos_half(S, L->prec) && !L->generic ? 2 : 1);
         else
             k_reduce_scalars<0><<<lanczos_hc_slots(it), kReduceBlock, 0, st>>>(L->part, nbh, L->tot, L->d_st, it, 0, 0);
         if (it == L->lm - 1) break;
