@@ -24,6 +24,23 @@ __device__ __forceinline__ cplx<R> gather_b(const Zin* __restrict__ c, uint32_t 
     return mk<R>((R) v.x, (R) v.y);
 }
 
+// Raw buffer loads over a complex64 vector c[0 .. dim): a 16-byte load that
+// the compiler keeps whole, with the hardware range check behind it.
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t coef_rsrc(const float2* c, uint32_t dim) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*) c, (short) 0, (int) (dim * 8u), 0x00020000);
+}
+__device__ __forceinline__ float4 ld16(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
+}
+__device__ __forceinline__ float2 ld8(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    const u32x2_t v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
+}
+
 // acc += (G * t) for complex G (uniform) and t.
 template <typename R> __device__ __forceinline__ void cmac(cplx<R>& acc, const cplx<R>& G, const cplx<R>& t) {
     acc.x += G.x * t.x;
@@ -69,11 +86,25 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
     const Z ci = gather_b<R, Zin>(c, x.is);
     Z acc = HALF ? mk<R>(0, 0) : cmul(e, ci);
 
+    // Targets that differ only in whether one added particle lands in orbital
+    // 0 or 1 have consecutive ranks (rank(n + e_1) = rank(n + e_0) + 1 when
+    // the rest agrees): in the complex64 form one 16-byte load fetches both.
+    // Such pairs exist when the removed orbitals are >= 2.
+    constexpr bool ADJ = !HALF && std::is_same<Zin, float2>::value;
+    const __amdgpu_buffer_rsrc_t rs = coef_rsrc((const float2*) c, ctx.dim);  // used only when ADJ
+    auto pair_at = [&](uint32_t off, Z& lo, Z& hi) {
+        const float4 v = ld16(rs, off);
+        lo = mk<R>(v.x, v.y);
+        hi = mk<R>(v.z, v.w);
+    };
+
     // single jumps k -> l: F_kl = A'_kl + sum_{m < M-1} n_m B'_klm (the last
     // occupation folded in through sum_m n_m = N when the block is built)
 #pragma unroll
     for (int k = 0; k < M; k++) {
         if (n[k] >= 1) {
+            Z c_to0{}, c_to1{};
+            if (ADJ && k >= 2) pair_at(x.single(k, 0), c_to0, c_to1);
 #pragma unroll
             for (int l = 0; l < M; l++) {
                 if (l == k || (HALF && l < k)) continue;
@@ -84,7 +115,10 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
                     F.x += rn[m] * B.x;
                     F.y += rn[m] * B.y;
                 }
-                const Z t = cscale(gather_b<R, Zin>(c, x.single(k, l)), x.s0[k] * x.s1[l]);
+                Z cj;
+                if (ADJ && k >= 2 && l <= 1) cj = l == 0 ? c_to0 : c_to1;
+                else cj = gather_b<R, Zin>(c, x.single(k, l));
+                const Z t = cscale(cj, x.s0[k] * x.s1[l]);
                 cmac<R>(acc, F, t);
             }
         }
@@ -103,6 +137,8 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
     for (int k = 0; k < M; k++) {
         if (n[k] >= 2) {
             Z g = mk<R>(0, 0);
+            // with k >= 2: (0,0),(0,1) share a load, and (0,l),(1,l) for l >= 2
+            Z c01{}, c1l[M];
 #pragma unroll
             for (int q = 0; q < M; q++) {
                 if (q == k || (HALF && q < k)) continue;
@@ -110,7 +146,13 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
                 for (int l = q; l < M; l++) {
                     if (l == k) continue;
                     const Z G = cf[cf_off_ot(M) + (k * M + q) * M + l];
-                    cmac<R>(g, G, cscale(gather_b<R, Zin>(c, x.same(k, q, l)), P[q][l]));
+                    Z cj;
+                    if (ADJ && k >= 2 && q == 0 && l == 0) pair_at(x.same(k, 0, 0), cj, c01);
+                    else if (ADJ && k >= 2 && q == 0 && l == 1) cj = c01;
+                    else if (ADJ && k >= 2 && q == 0) pair_at(x.same(k, 0, l), cj, c1l[l]);
+                    else if (ADJ && k >= 2 && q == 1 && l >= 2) cj = c1l[l];
+                    else cj = gather_b<R, Zin>(c, x.same(k, q, l));
+                    cmac<R>(g, G, cscale(cj, P[q][l]));
                 }
             }
             const R a = x.sm[k] * x.s0[k];
@@ -125,6 +167,8 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
         for (int s = k + 1; s < M; s++) {
             if (n[k] >= 1 && n[s] >= 1) {
                 Z g = mk<R>(0, 0);
+                // with 2 <= k < s: the same sharing as above
+                Z c01{}, c1l[M];
 #pragma unroll
                 for (int q = 0; q < M; q++) {
                     if (q == k || q == s || (HALF && q < k)) continue;
@@ -132,7 +176,13 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
                     for (int l = q; l < M; l++) {
                         if (l == k || l == s) continue;
                         const Z G = cf[cf_off_tt(M) + ((k * M + s) * M + q) * M + l];
-                        cmac<R>(g, G, cscale(gather_b<R, Zin>(c, x.diff(k, s, q, l)), P[q][l]));
+                        Z cj;
+                        if (ADJ && k >= 2 && q == 0 && l == 0) pair_at(x.diff(k, s, 0, 0), cj, c01);
+                        else if (ADJ && k >= 2 && q == 0 && l == 1) cj = c01;
+                        else if (ADJ && k >= 2 && q == 0) pair_at(x.diff(k, s, 0, l), cj, c1l[l]);
+                        else if (ADJ && k >= 2 && q == 1 && l >= 2) cj = c1l[l];
+                        else cj = gather_b<R, Zin>(c, x.diff(k, s, q, l));
+                        cmac<R>(g, G, cscale(cj, P[q][l]));
                     }
                 }
                 const R a = x.s0[k] * x.s0[s];

@@ -28,24 +28,9 @@ struct PairCtx {
 // coefficient vector carries one zero element before and after it
 // (c[-1], c[dim]), so the window (j0, j0 + 1) of a pair, j0 in [-1, dim - 1],
 // is always readable and no gather needs a bounds test.
-typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
-typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
-
 // Resource over c[-1 .. dim]; byte offset 8 (j + 1) addresses c[j].
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t padded_rsrc(const float2* c, uint32_t dim) {
     return __builtin_amdgcn_make_buffer_rsrc((void*) (c - 1), (short) 0, (int) ((dim + 2) * 8u), 0x00020000);
-}
-// Resource over c[0 .. dim).
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t coef_rsrc(const float2* c, uint32_t dim) {
-    return __builtin_amdgcn_make_buffer_rsrc((void*) c, (short) 0, (int) (dim * 8u), 0x00020000);
-}
-__device__ __forceinline__ float4 ld16(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
-    return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
-}
-__device__ __forceinline__ float2 ld8(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    const u32x2_t v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
-    return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
 }
 
 __device__ __forceinline__ float2 lo2(const float4& v) { return make_float2(v.x, v.y); }
