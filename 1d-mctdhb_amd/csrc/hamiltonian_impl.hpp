@@ -53,9 +53,10 @@ template <typename R> __device__ __forceinline__ void cmac(cplx<R>& acc, const c
 // <c, H c> instead: H is Hermitian and the jump types come in inverse pairs
 // (remove R, add A) / (remove A, add R), so sum_i conj(c_i) (H c)_i =
 // sum_i E_i |c_i|^2 + 2 Re sum_i conj(c_i) sum_{T in half} H(i, T(i)) c_T(i),
-// keeping of each pair the type whose sorted removal list comes first
-// (k < l for k -> l, k < q for the double jumps).  The real result is
-// returned in .x.
+// keeping of each pair the type whose sorted addition list comes first
+// (l < k for k -> l, q < k for the double jumps: this keeps the jumps into
+// orbitals 0 and 1, whose targets pair up for 16-byte loads).  The real
+// result is returned in .x.
 template <int M, typename R, int IDX, typename Zin, bool HALF = false>
 __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, const Zin* __restrict__ c,
                                                 const cplx<R>* __restrict__ cf, const uint32_t* __restrict__ Wl,
@@ -90,7 +91,7 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
     // 0 or 1 have consecutive ranks (rank(n + e_1) = rank(n + e_0) + 1 when
     // the rest agrees): in the complex64 form one 16-byte load fetches both.
     // Such pairs exist when the removed orbitals are >= 2.
-    constexpr bool ADJ = !HALF && std::is_same<Zin, float2>::value;
+    constexpr bool ADJ = std::is_same<Zin, float2>::value;
     const __amdgpu_buffer_rsrc_t rs = coef_rsrc((const float2*) c, ctx.dim);  // used only when ADJ
     auto pair_at = [&](uint32_t off, Z& lo, Z& hi) {
         const float4 v = ld16(rs, off);
@@ -107,7 +108,7 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
             if (ADJ && k >= 2) pair_at(x.single(k, 0), c_to0, c_to1);
 #pragma unroll
             for (int l = 0; l < M; l++) {
-                if (l == k || (HALF && l < k)) continue;
+                if (l == k || (HALF && l > k)) continue;
                 Z F = cf[cf_off_A(M) + k * M + l];
 #pragma unroll
                 for (int m = 0; m < M - 1; m++) {
@@ -141,7 +142,7 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
             Z c01{}, c1l[M];
 #pragma unroll
             for (int q = 0; q < M; q++) {
-                if (q == k || (HALF && q < k)) continue;
+                if (q == k || (HALF && q > k)) continue;
 #pragma unroll
                 for (int l = q; l < M; l++) {
                     if (l == k) continue;
@@ -171,7 +172,7 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
                 Z c01{}, c1l[M];
 #pragma unroll
                 for (int q = 0; q < M; q++) {
-                    if (q == k || q == s || (HALF && q < k)) continue;
+                    if (q == k || q == s || (HALF && q > k)) continue;
 #pragma unroll
                     for (int l = q; l < M; l++) {
                         if (l == k || l == s) continue;
