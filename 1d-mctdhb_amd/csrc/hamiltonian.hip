@@ -279,7 +279,7 @@ __device__ double2 hc_row_generic(uint32_t i, int M, const RowCtx& ctx, const do
 
 template <typename Zout> struct EpiPlainG {
     Zout* out;
-    __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 v, double* lds) const {
+    __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 v, double2 ci, double* lds) const {
         if (valid) out[i] = v;
     }
 };
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(kHcBlockG) void k_hc_generic(int M, RowCtx ctx, con
     const bool valid = i < ctx.dim;
     double2 v = make_double2(0, 0);
     if (valid) v = hc_row_generic(i, M, ctx, c, hob, hint);
-    epi(i, valid, v, red);
+    epi(i, valid, v, valid ? c[i] : make_double2(0, 0), red);
 }
 
 static RowCtx row_ctx_g(const Space* S) {

@@ -60,7 +60,7 @@ template <typename R> __device__ __forceinline__ void cmac(cplx<R>& acc, const c
 template <int M, typename R, int IDX, typename Zin, bool HALF = false>
 __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, const Zin* __restrict__ c,
                                                 const cplx<R>* __restrict__ cf, const uint32_t* __restrict__ Wl,
-                                                int ldw, const R* __restrict__ sql) {
+                                                int ldw, const R* __restrict__ sql, cplx<R>& ci_out) {
     using Z = cplx<R>;
     Cfg<M, R, IDX, coef_shift<Zin>()> x;
     x.template load_impl<true>(i, ctx, Wl, ldw, sql);
@@ -85,6 +85,7 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
         }
     }
     const Z ci = gather_b<R, Zin>(c, x.is);
+    ci_out = ci;
     Z acc = HALF ? mk<R>(0, 0) : cmul(e, ci);
 
     // Targets that differ only in whether one added particle lands in orbital
@@ -202,26 +203,27 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
 
 // --------------------------------------------------------------- epilogues
 
+// Epilogues receive the row value and the row's own coefficient c_i (as
+// gathered by the row), and are called by every thread of the block.
 template <typename Zout> struct EpiPlain {
     Zout* out;
-    __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 v, double* lds) const {
+    __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 v, double2 ci, double* lds) const {
         if (valid) out[i] = from_d2<Zout>(v);
     }
 };
 
-// The Lanczos epilogue.  On the last iteration (a.last) only alpha_it is
-// needed: out is not stored and only the slot pair of k = it is written.
 template <typename Zw> struct EpiLanczos {
     LanczosHcArgs a;
-    __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 row, double* lds) const {
+    __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 row, double2 ci, double* lds) const {
         const SilState* st = a.st;
         const int it = a.it;
-        double2 o = make_double2(0, 0);
+        // w_it[i] is the row's own coefficient; w_{it-1}[i] is read once
+        double2 o = make_double2(0, 0), wprev = make_double2(0, 0);
         if (valid) {
             o = cscale(row, st->s[it]);
             if (it > 0) {
-                const double bp = st->beta[it - 1] * st->s[it - 1];
-                o = o - cscale(to_d2(((const Zw*) a.w[it - 1])[i]), bp);
+                wprev = to_d2(((const Zw*) a.w[it - 1])[i]);
+                o = o - cscale(wprev, st->beta[it - 1] * st->s[it - 1]);
             }
             if (!a.last) a.out[i] = o;
         }
@@ -231,7 +233,7 @@ template <typename Zw> struct EpiLanczos {
         for (int k = a.last ? it : 0; k <= it; k++) {
             double v[2] = {0, 0};
             if (valid) {
-                const double2 wk = to_d2(((const Zw*) a.w[k])[i]);
+                const double2 wk = k == it ? ci : (k == it - 1 ? wprev : to_d2(((const Zw*) a.w[k])[i]));
                 const double2 p = cconjmul(wk, o);
                 v[0] = p.x * st->s[k];
                 v[1] = p.y * st->s[k];
@@ -244,10 +246,7 @@ template <typename Zw> struct EpiLanczos {
         }
         if (a.last) return;
         double v1[1] = {0};
-        if (valid && it == 0) {
-            const double2 w0 = to_d2(((const Zw*) a.w[0])[i]);
-            v1[0] = w0.x * w0.x + w0.y * w0.y;
-        }
+        if (valid && it == 0) v1[0] = ci.x * ci.x + ci.y * ci.y;  // |w_0|^2
         block_sum<1>(v1, lds);
         if (threadIdx.x == 0) {
             part[(2 * (it + 1)) * ps] = v1[0];
@@ -260,7 +259,7 @@ template <typename Zw> struct EpiLanczos {
 // into slot 2 it; alpha is formed from it in the scalar update.
 struct EpiRayleigh {
     LanczosHcArgs a;
-    __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 row, double* lds) const {
+    __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 row, double2 ci, double* lds) const {
         double v[1] = {valid ? row.x : 0.0};
         block_sum<1>(v, lds);
         if (threadIdx.x == 0) {
@@ -292,9 +291,13 @@ __global__ __launch_bounds__(kHcBlock) void k_hc_fused(RowCtx ctx, const Zin* __
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t i = b * kHcBlock + threadIdx.x;
     const bool valid = i < ctx.dim;
-    double2 v = make_double2(0, 0);
-    if (valid) v = to_d2(hc_row_fused<M, R, IDX, Zin, HALF>(i, ctx, c, cf, Wl, ldw, sql));
-    epi(i, valid, v, red);
+    double2 v = make_double2(0, 0), ci = make_double2(0, 0);
+    if (valid) {
+        cplx<R> own;
+        v = to_d2(hc_row_fused<M, R, IDX, Zin, HALF>(i, ctx, c, cf, Wl, ldw, sql, own));
+        ci = to_d2(own);
+    }
+    epi(i, valid, v, ci, red);
 }
 
 static inline RowCtx row_ctx(const Space* S) {

@@ -334,7 +334,8 @@ __global__ __launch_bounds__(kHcBlock) void k_hc_pair(RowCtx ctx, PairCtx pc, co
     if (has1) {
         hc_pair_fused<M>(i0, ctx, padded_rsrc(c, ctx.dim), cf, Wl, ldw, sql, r0, r1);
     } else if (has0) {
-        r0 = hc_row_fused<M, float, kIdxFly, float2>(i0, ctx, c, cf, Wl, ldw, sql);
+        float2 own;
+        r0 = hc_row_fused<M, float, kIdxFly, float2>(i0, ctx, c, cf, Wl, ldw, sql, own);
     }
     epi(i0, has0, has1, to_d2(r0), to_d2(r1), red);
 }
