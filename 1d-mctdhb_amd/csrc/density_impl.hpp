@@ -285,7 +285,7 @@ int DmFused<M>::run(const Space* S, const void* c, int prec, int index, double2*
         return kInvalidArgument;
     }
     RowCtx ctx = make_row_ctx(S->dim, S->npar, S->d_ht, S->d_map, S->d_mapot, S->d_maptt, S->d_strideot,
-                              S->d_stridett);
+                              S->d_stridett, S->d_occ8);
     if (prec == kDmMixed)
         k_density<M, float, kIdxFly><<<nb, kDmBlock, smem, st>>>(ctx, (const float2*) c, S->d_W, S->ldw, use_sq,
                                                                  part);

@@ -301,7 +301,8 @@ __global__ __launch_bounds__(kHcBlock) void k_hc_fused(RowCtx ctx, const Zin* __
 }
 
 static inline RowCtx row_ctx(const Space* S) {
-    return make_row_ctx(S->dim, S->npar, S->d_ht, S->d_map, S->d_mapot, S->d_maptt, S->d_strideot, S->d_stridett);
+    return make_row_ctx(S->dim, S->npar, S->d_ht, S->d_map, S->d_mapot, S->d_maptt, S->d_strideot, S->d_stridett,
+                        S->d_occ8);
 }
 
 // Layout matches k_hc_fused: 64 B reduction scratch, the W table region

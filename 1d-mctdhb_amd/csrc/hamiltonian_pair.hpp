@@ -44,8 +44,14 @@ __device__ __forceinline__ void hc_pair_fused(uint32_t i0, const RowCtx& ctx, __
     using Z = float2;
     // occupations of i0; those of i0 + 1 follow
     int na[M], nb[M];
+    if (M <= 8 && ctx.occ8) {
+        const uint2 w = *(const uint2*) (ctx.occ8 + (size_t) i0 * 8);
 #pragma unroll
-    for (int k = 0; k < M; k++) na[k] = ctx.ht[(size_t) i0 * M + k];
+        for (int k = 0; k < M; k++) na[k] = ((k < 4 ? w.x : w.y) >> (8 * (k & 3))) & 255u;
+    } else {
+#pragma unroll
+        for (int k = 0; k < M; k++) na[k] = ctx.ht[(size_t) i0 * M + k];
+    }
 #pragma unroll
     for (int k = 0; k < M; k++) nb[k] = na[k];
     nb[0] -= 1;

@@ -11,6 +11,7 @@ struct RowCtx {
     uint32_t dim;
     int npar;
     const uint16_t* ht;
+    const uint8_t* occ8;  // occupations as bytes, row stride 8 (M <= 8, N <= 255), or null
     const uint32_t* map;
     const uint32_t* mapot;
     const uint32_t* maptt;
@@ -46,8 +47,15 @@ template <int M, typename R, int IDX, int SHIFT = 0> struct Cfg {
         i = idx;
         is = idx << SHIFT;
         ctx = &c;
+        if (M <= 8 && c.occ8) {
+            // one 8-byte load instead of M 2-byte loads
+            const uint2 w = *(const uint2*) (c.occ8 + (size_t) i * 8);
 #pragma unroll
-        for (int k = 0; k < M; k++) n[k] = c.ht[(size_t) i * M + k];
+            for (int k = 0; k < M; k++) n[k] = ((k < 4 ? w.x : w.y) >> (8 * (k & 3))) & 255u;
+        } else {
+#pragma unroll
+            for (int k = 0; k < M; k++) n[k] = c.ht[(size_t) i * M + k];
+        }
 #pragma unroll
         for (int k = 0; k < M; k++) {
             if constexpr (LUT) {
@@ -133,11 +141,13 @@ template <int M, typename R, int IDX, int SHIFT = 0> struct Cfg {
 };
 
 inline RowCtx make_row_ctx(uint32_t dim, int npar, const uint16_t* ht, const uint32_t* map, const uint32_t* mapot,
-                           const uint32_t* maptt, const uint32_t* sot, const uint32_t* stt) {
+                           const uint32_t* maptt, const uint32_t* sot, const uint32_t* stt,
+                           const uint8_t* occ8 = nullptr) {
     RowCtx c;
     c.dim = dim;
     c.npar = npar;
     c.ht = ht;
+    c.occ8 = occ8;
     c.map = map;
     c.mapot = mapot;
     c.maptt = maptt;

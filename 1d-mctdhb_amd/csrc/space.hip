@@ -175,6 +175,14 @@ __global__ void k_scan_apply(const uint32_t* __restrict__ cnt, uint32_t n, const
     }
 }
 
+__global__ void k_occ8(int M, uint32_t dim, const uint16_t* __restrict__ ht, uint2* __restrict__ occ8) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= dim) return;
+    uint32_t w[2] = {0, 0};
+    for (int k = 0; k < M; k++) w[k >> 2] |= (uint32_t) ht[(size_t) i * M + k] << (8 * (k & 3));
+    occ8[i] = make_uint2(w[0], w[1]);
+}
+
 // Pair slots.  Within a run (fixed P_1 .. P_{M-2}) the rank grows by one as
 // P_0 = n_0 falls by one, i.e. as n_1 grows: run position = n_1.  A
 // configuration at an even position starts a pair when the run goes on
@@ -360,6 +368,11 @@ int space_create(int npar, int norb, int flags, Space** out) {
         MCTDHB_HIP_CHECK(hipGetLastError());
         S->tables = true;
     }
+    if (norb <= 8 && npar <= 255) {
+        ALLOC(S->d_occ8, (size_t) 8 * dim);
+        k_occ8<<<div_up(dim, tb), tb, 0, st>>>(norb, dim, S->d_ht, (uint2*) S->d_occ8);
+        MCTDHB_HIP_CHECK(hipGetLastError());
+    }
     if (norb >= 2) {
         rc = build_pair_slots(S, st);
         if (rc) {
@@ -385,6 +398,7 @@ void space_destroy(Space* S) {
     hipFree(S->d_stridett);
     hipFree(S->d_slots);
     hipFree(S->d_pad);
+    hipFree(S->d_occ8);
     delete S;
 }
 

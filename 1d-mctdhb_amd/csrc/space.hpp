@@ -33,6 +33,9 @@ struct Space {
     // n_2 .. n_{M-1}, ranks consecutive as one particle moves from orbital 0
     // to orbital 1), slots[n_pairs, n_pairs + n_singles) the configurations
     // left alone at the end of odd-length runs; all in rank order.
+    // occupations as bytes with row stride 8 (built when M <= 8, N <= 255):
+    // one 8-byte load per configuration in the fused kernels
+    uint8_t* d_occ8 = nullptr;
     uint32_t* d_slots = nullptr;
     uint32_t n_pairs = 0, n_singles = 0;
     // complex64 staging with one zero element on each side (dim + 2), for
