@@ -212,7 +212,34 @@ template <typename Zout> struct EpiPlain {
     }
 };
 
+// LDS bytes in front of the W table of the fused kernels: per-wave partial
+// sums of up to 2 kMaxKrylov + 4 slots.
+constexpr int kRedBytes = 8 * (kHcBlock / 64) * (2 * kMaxKrylov + 4);
+
+// Per-wave partial sums into LDS, one barrier for any number of slots:
+// put() reduces a value over the wave, finish() adds the waves in a fixed
+// order and stores slot t of this block at part[t * ps].
+struct SlotSums {
+    double* red;
+    int nw, wid, lane;
+    __device__ __forceinline__ SlotSums(double* lds)
+        : red(lds), nw(blockDim.x >> 6), wid(threadIdx.x >> 6), lane(threadIdx.x & 63) {}
+    __device__ __forceinline__ void put(int slot, double v) const {
+        v = wave_sum(v);
+        if (lane == 0) red[slot * nw + wid] = v;
+    }
+    __device__ __forceinline__ void finish(int slot0, int nslot, double* part, size_t ps) const {
+        __syncthreads();
+        for (int t = threadIdx.x; t < nslot; t += blockDim.x) {
+            double s = 0;
+            for (int w = 0; w < nw; w++) s += red[(slot0 + t) * nw + w];
+            part[(size_t) (slot0 + t) * ps] = s;
+        }
+    }
+};
+
 template <typename Zw> struct EpiLanczos {
+    static constexpr bool kReuse = std::is_same<Zw, float2>::value;
     LanczosHcArgs a;
     __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 row, double2 ci, double* lds) const {
         const SilState* st = a.st;
@@ -228,30 +255,30 @@ template <typename Zw> struct EpiLanczos {
             if (!a.last) a.out[i] = o;
         }
         // slot-major partials: slot t of block b at part[t * gridDim.x + b]
-        double* part = a.part + blockIdx.x;
-        const size_t ps = gridDim.x;
-        for (int k = a.last ? it : 0; k <= it; k++) {
-            double v[2] = {0, 0};
+        const SlotSums ss(lds);
+        const int k0 = a.last ? it : 0;
+        for (int k = k0; k <= it; k++) {
+            double2 p = make_double2(0, 0);
             if (valid) {
-                const double2 wk = k == it ? ci : (k == it - 1 ? wprev : to_d2(((const Zw*) a.w[k])[i]));
-                const double2 p = cconjmul(wk, o);
-                v[0] = p.x * st->s[k];
-                v[1] = p.y * st->s[k];
+                // (complex64 only: reusing c_i in the complex128 kernel
+                // lengthens its live range and costs more than the load)
+                const double2 wk = (kReuse && k == it) ? ci
+                                                       : (k == it - 1 ? wprev : to_d2(((const Zw*) a.w[k])[i]));
+                p = cscale(cconjmul(wk, o), st->s[k]);
             }
-            block_sum<2>(v, lds);
-            if (threadIdx.x == 0) {
-                part[(2 * k) * ps] = v[0];
-                part[(2 * k + 1) * ps] = v[1];
+            ss.put(2 * k, p.x);
+            ss.put(2 * k + 1, p.y);
+        }
+        if (!a.last) {
+            double n2 = 0;  // |w_0|^2
+            if (valid && it == 0) {
+                const double2 w0 = kReuse ? ci : to_d2(((const Zw*) a.w[0])[i]);
+                n2 = w0.x * w0.x + w0.y * w0.y;
             }
+            ss.put(2 * (it + 1), n2);
+            ss.put(2 * (it + 1) + 1, 0.0);
         }
-        if (a.last) return;
-        double v1[1] = {0};
-        if (valid && it == 0) v1[0] = ci.x * ci.x + ci.y * ci.y;  // |w_0|^2
-        block_sum<1>(v1, lds);
-        if (threadIdx.x == 0) {
-            part[(2 * (it + 1)) * ps] = v1[0];
-            part[(2 * (it + 1) + 1) * ps] = 0;
-        }
+        ss.finish(2 * k0, a.last ? 2 : 2 * (it + 1) + 2, a.part + blockIdx.x, gridDim.x);
     }
 };
 
@@ -279,10 +306,10 @@ __global__ __launch_bounds__(kHcBlock) void k_hc_fused(RowCtx ctx, const Zin* __
                                                        int stop_check, const SilState* st, Epi epi) {
     extern __shared__ __align__(16) unsigned char smem[];
     if (stop_check && st->stop) return;
-    double* red = (double*) smem;                         // block_sum scratch (kHcBlock/64 * 2 doubles)
-    uint32_t* Wl = (uint32_t*) (smem + 64);               // W table
+    double* red = (double*) smem;                         // partial sums (kRedBytes)
+    uint32_t* Wl = (uint32_t*) (smem + kRedBytes);        // W table
     const int wsz = (M - 1) * ldw;
-    R* sql = (R*) (smem + 64 + ((wsz * 4 + 15) & ~15));   // sqrt LUT
+    R* sql = (R*) (smem + kRedBytes + ((wsz * 4 + 15) & ~15));  // sqrt LUT
     if constexpr (IDX == kIdxFly) {
         for (int t = threadIdx.x; t < wsz; t += blockDim.x) Wl[t] = Wg[t];
     }
@@ -295,7 +322,7 @@ __global__ __launch_bounds__(kHcBlock) void k_hc_fused(RowCtx ctx, const Zin* __
     if (valid) {
         cplx<R> own;
         v = to_d2(hc_row_fused<M, R, IDX, Zin, HALF>(i, ctx, c, cf, Wl, ldw, sql, own));
-        ci = to_d2(own);
+        if constexpr (std::is_same<Zin, float2>::value) ci = to_d2(own);
     }
     epi(i, valid, v, ci, red);
 }
@@ -305,11 +332,11 @@ static inline RowCtx row_ctx(const Space* S) {
                         S->d_occ8);
 }
 
-// Layout matches k_hc_fused: 64 B reduction scratch, the W table region
+// Layout matches k_hc_fused: partial-sum scratch, the W table region
 // (reserved in both index modes so the sqrt LUT offset is the same), sqrt LUT.
 static inline size_t fused_smem(const Space* S, size_t rsize) {
     size_t wsz = (size_t) (S->norb - 1) * S->ldw * 4;
-    return 64 + ((wsz + 15) & ~(size_t) 15) + rsize * (S->npar + 3);
+    return kRedBytes + ((wsz + 15) & ~(size_t) 15) + rsize * (S->npar + 3);
 }
 
 template <int M, typename R, typename Zin, typename Epi, bool HALF = false>

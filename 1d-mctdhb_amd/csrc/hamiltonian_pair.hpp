@@ -284,35 +284,29 @@ struct EpiLanczosPair {
                 if (has1) a.out[i0 + 1] = o1;
             }
         }
-        double* part = a.part + blockIdx.x;
-        const size_t ps = gridDim.x;
-        for (int k = a.last ? it : 0; k <= it; k++) {
-            double v[2] = {0, 0};
+        const SlotSums ss(lds);
+        const int k0 = a.last ? it : 0;
+        for (int k = k0; k <= it; k++) {
+            double2 p = make_double2(0, 0);
             if (has0) {
                 double2 x0, x1;
                 pair_of(a.w[k], x0, x1);
-                const double2 p = cconjmul(x0, o0) + cconjmul(x1, o1);
-                v[0] = p.x * st->s[k];
-                v[1] = p.y * st->s[k];
+                p = cscale(cconjmul(x0, o0) + cconjmul(x1, o1), st->s[k]);
             }
-            block_sum<2>(v, lds);
-            if (threadIdx.x == 0) {
-                part[(2 * k) * ps] = v[0];
-                part[(2 * k + 1) * ps] = v[1];
+            ss.put(2 * k, p.x);
+            ss.put(2 * k + 1, p.y);
+        }
+        if (!a.last) {
+            double n2 = 0;
+            if (has0 && it == 0) {
+                double2 x0, x1;
+                pair_of(a.w[0], x0, x1);
+                n2 = x0.x * x0.x + x0.y * x0.y + x1.x * x1.x + x1.y * x1.y;
             }
+            ss.put(2 * (it + 1), n2);
+            ss.put(2 * (it + 1) + 1, 0.0);
         }
-        if (a.last) return;
-        double v1s[1] = {0};
-        if (has0 && it == 0) {
-            double2 x0, x1;
-            pair_of(a.w[0], x0, x1);
-            v1s[0] = x0.x * x0.x + x0.y * x0.y + x1.x * x1.x + x1.y * x1.y;
-        }
-        block_sum<1>(v1s, lds);
-        if (threadIdx.x == 0) {
-            part[(2 * (it + 1)) * ps] = v1s[0];
-            part[(2 * (it + 1) + 1) * ps] = 0;
-        }
+        ss.finish(2 * k0, a.last ? 2 : 2 * (it + 1) + 2, a.part + blockIdx.x, gridDim.x);
     }
 };
 
@@ -325,9 +319,9 @@ __global__ __launch_bounds__(kHcBlock) void k_hc_pair(RowCtx ctx, PairCtx pc, co
     extern __shared__ __align__(16) unsigned char smem[];
     if (stop_check && st->stop) return;
     double* red = (double*) smem;
-    uint32_t* Wl = (uint32_t*) (smem + 64);
+    uint32_t* Wl = (uint32_t*) (smem + kRedBytes);
     const int wsz = (M - 1) * ldw;
-    float* sql = (float*) (smem + 64 + ((wsz * 4 + 15) & ~15));
+    float* sql = (float*) (smem + kRedBytes + ((wsz * 4 + 15) & ~15));
     for (int t = threadIdx.x; t < wsz; t += blockDim.x) Wl[t] = Wg[t];
     for (int t = threadIdx.x; t < ctx.npar + 3; t += blockDim.x) sql[t] = sqrtf((float) t);
     __syncthreads();
