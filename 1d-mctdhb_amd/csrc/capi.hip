@@ -467,6 +467,81 @@ extern "C" int mctdhb_propagate_step_dev(mctdhb_space* sp, mctdhb_sil* s, mctdhb
     return mctdhb_density_sync_dev(sp, coef, rho1, rho2, rinv, nullptr, 1e-9, MCTDHB_FP64, stream);
 }
 
+struct mctdhb_step_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" int mctdhb_step_graph_create(mctdhb_space* sp, mctdhb_sil* s, mctdhb_orbital* o, int use_fft, void* orb,
+                                        void* coef, void* hob, void* hint, void* rho1, void* rho2, void* rinv,
+                                        mctdhb_step_graph** out) {
+    if (!out || !sp || !s || !o || !orb || !coef || !hob || !hint || !rho1 || !rho2 || !rinv) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    *out = nullptr;
+    if (sp->S->norb > 7) {
+        set_error("step graphs cover 1 <= norb <= 7 (the fused density kernels)");
+        return kInvalidArgument;
+    }
+    // Everything the step allocates or tunes on first use is done here, so
+    // the capture below records kernels and device copies only.
+    if (!sp->dm_work) MCTDHB_HIP_CHECK(hipMalloc(&sp->dm_work, dm_workspace_bytes(sp->S)));
+    if (s->L->prec == kPrecMixed) {
+        const int rc = hc_tune_tiling(sp->S, nullptr);
+        if (rc) return rc;
+    }
+    MCTDHB_HIP_CHECK(hipDeviceSynchronize());
+    hipStream_t cs = nullptr;
+    MCTDHB_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t graph = nullptr;
+    int rc = kOk;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        set_error("hipStreamBeginCapture failed");
+        rc = kDeviceError;
+    } else {
+        rc = mctdhb_propagate_step_dev(sp, s, o, use_fft, orb, coef, hob, hint, rho1, rho2, rinv, cs);
+        const hipError_t e = hipStreamEndCapture(cs, &graph);
+        if (!rc && e != hipSuccess) {
+            set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            rc = kDeviceError;
+        }
+    }
+    hipStreamDestroy(cs);
+    hipGraphExec_t exec = nullptr;
+    if (!rc && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        set_error("hipGraphInstantiate failed");
+        rc = kDeviceError;
+    }
+    if (rc) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+    }
+    mctdhb_step_graph* g = new mctdhb_step_graph();
+    g->graph = graph;
+    g->exec = exec;
+    *out = g;
+    return kOk;
+}
+
+extern "C" int mctdhb_step_graph_launch(mctdhb_step_graph* g, void* stream) {
+    if (!g) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    MCTDHB_HIP_CHECK(hipGraphLaunch(g->exec, (hipStream_t) stream));
+    return kOk;
+}
+
+extern "C" int mctdhb_step_graph_destroy(mctdhb_step_graph* g) {
+    if (!g) return kOk;
+    hipDeviceSynchronize();
+    hipGraphExecDestroy(g->exec);
+    hipGraphDestroy(g->graph);
+    delete g;
+    return kOk;
+}
+
 // ------------------------------------------------------------------ layer 1
 
 [[noreturn]] static void fail_stop(const char* what) {

@@ -273,12 +273,9 @@ int linear_cn(OrbWork* w, double2* orb, hipStream_t st) {
     const double2 a1dt = cscale(cmul(q.d1coef, dt), 1.0 / dx / 4);
     const double2 up = cscale(a2dt, 0.5) + a1dt, lo = cscale(a2dt, 0.5) - a1dt;
     const double2 mid = make_double2(-a2dt.x, 1.0 - a2dt.y);
-    double2* scratch = nullptr;
-    MCTDHB_HIP_CHECK(hipMallocAsync((void**) &scratch, sizeof(double2) * 8 * q.grid_size * q.norb, st));
     k_cn_sequential<<<div_up(q.norb, 64), 64, 0, st>>>(orb, q.norb, q.grid_size, q.bounds, w->d_cn, up, lo, mid,
-                                                       scratch);
+                                                       w->d_cnwork);
     MCTDHB_HIP_CHECK(hipGetLastError());
-    MCTDHB_HIP_CHECK(hipFreeAsync(scratch, st));
     return kOk;
 }
 
@@ -673,7 +670,8 @@ __device__ int herm_eig_dev(int n, double2* a, double* lam, double2* v) {
 // (lapack_interface.c:126-163, synchronize.c:50-54), then the inverse from
 // the lower triangle (lapack_interface.c:10-59).  One thread, M <= 30.
 __global__ void k_density_sync_small(int M, double2* __restrict__ rho1, double2* __restrict__ rinv,
-                                     double* __restrict__ eig_out, double reg_factor, double2* __restrict__ work) {
+                                     double* __restrict__ eig_out, double reg_factor) {
+    extern __shared__ double2 work[];  // 3 * M * M
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double2* a = work;              // M*M
     double2* v = work + M * M;      // M*M
@@ -748,11 +746,9 @@ int density_sync_small(int norb, double2* rho1, double2* rinv, double* eigvals, 
         set_error("norb exceeds 30");
         return kInvalidArgument;
     }
-    double2* work = nullptr;
-    MCTDHB_HIP_CHECK(hipMallocAsync((void**) &work, sizeof(double2) * 3 * norb * norb, st));
-    k_density_sync_small<<<1, 64, 0, st>>>(norb, rho1, rinv, eigvals, reg_factor, work);
+    // at most 3 * 30 * 30 * 16 = 43200 bytes of LDS
+    k_density_sync_small<<<1, 64, sizeof(double2) * 3 * norb * norb, st>>>(norb, rho1, rinv, eigvals, reg_factor);
     MCTDHB_HIP_CHECK(hipGetLastError());
-    MCTDHB_HIP_CHECK(hipFreeAsync(work, st));
     return kOk;
 }
 
@@ -820,6 +816,7 @@ int orbwork_create(const OrbEq& eq, int imagtime, const double* pot_host, OrbWor
     ok = ok && hipMalloc((void**) &w->d_cn, sizeof(double2) * 3 * grid) == hipSuccess;
     ok = ok && hipMalloc((void**) &w->d_stage, sizeof(double2) * 7 * (size_t) M * grid) == hipSuccess;
     ok = ok && hipMalloc((void**) &w->d_red, sizeof(double) * 1024) == hipSuccess;
+    ok = ok && hipMalloc((void**) &w->d_cnwork, sizeof(double2) * 8 * (size_t) M * grid) == hipSuccess;
     if (!ok) {
         orbwork_destroy(w);
         set_error("device allocation failed for the orbital workspace");
@@ -850,6 +847,7 @@ void orbwork_destroy(OrbWork* w) {
     hipFree(w->d_cn);
     hipFree(w->d_stage);
     hipFree(w->d_red);
+    hipFree(w->d_cnwork);
     delete w;
 }
 

@@ -28,6 +28,7 @@ struct OrbWork {
     double2* d_cn = nullptr;     // 3 * grid_size: upper | lower | mid
     double2* d_stage = nullptr;  // 7 * norb * grid_size: up to six stages and y
     double* d_red = nullptr;     // reduction scratch
+    double2* d_cnwork = nullptr; // 8 * norb * grid_size: Crank-Nicolson sweep scratch
 };
 
 int orbwork_create(const OrbEq& eq, int imagtime, const double* pot_host, OrbWork** out);
@@ -51,7 +52,8 @@ int orthonormalize(OrbWork* w, double2* orb, hipStream_t st);
 // Gram-Schmidt in imaginary time); hob/hint are rebuilt from the orbitals.
 int splitstep(OrbWork* w, int use_fft, double2* orb, double2* hob, double2* hint, const double2* rho2,
               const double2* rinv, hipStream_t st);
-// rho1 -> eigenvalues, conditional regularisation, inverse (M x M, one wave).
+// rho1 -> eigenvalues, conditional regularisation, inverse (M x M, one wave,
+// LDS work space: no allocation, so the call can be captured in a graph).
 int density_sync_small(int norb, double2* rho1, double2* rinv, double* eigvals, double reg_factor,
                        hipStream_t st);
 

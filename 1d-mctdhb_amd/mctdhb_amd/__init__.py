@@ -118,6 +118,9 @@ def lib():
     L.mctdhb_splitstep_dev.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]
     L.mctdhb_density_sync_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, _i, _vp]
     L.mctdhb_propagate_step_dev.argtypes = [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.mctdhb_step_graph_create.argtypes = [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _cpp]
+    L.mctdhb_step_graph_launch.argtypes = [_vp, _vp]
+    L.mctdhb_step_graph_destroy.argtypes = [_vp]
     # reference-named drop-in layer
     L.space_dimension.argtypes = [C.c_uint16, C.c_uint16]
     L.space_dimension.restype = C.c_uint32
@@ -331,6 +334,30 @@ def propagate_step(space: DeviceSpace, sil: DeviceSil, orbital: DeviceOrbital, o
     check(lib().mctdhb_propagate_step_dev(space.handle, sil._h, orbital.handle, int(bool(use_fft)), _ptr(orb),
                                           _ptr(coef), _ptr(hob), _ptr(hint), _ptr(rho1), _ptr(rho2), _ptr(rinv),
                                           _stream(stream)))
+
+
+class StepGraph:
+    """propagate_step recorded once as a HIP graph; launch() replays it on
+    the same buffers."""
+
+    def __init__(self, space: DeviceSpace, sil: DeviceSil, orbital: DeviceOrbital, orb, coef, hob, hint, rho1,
+                 rho2, rinv, use_fft=True):
+        h = _vp()
+        check(lib().mctdhb_step_graph_create(space.handle, sil._h, orbital.handle, int(bool(use_fft)), _ptr(orb),
+                                             _ptr(coef), _ptr(hob), _ptr(hint), _ptr(rho1), _ptr(rho2),
+                                             _ptr(rinv), C.byref(h)))
+        self._h = h
+        self._keep = (space, sil, orbital)
+
+    def launch(self, stream=None):
+        check(lib().mctdhb_step_graph_launch(self._h, _stream(stream)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mctdhb_step_graph_destroy(self._h)
+            self._h = None
+
+    __del__ = close
 
 
 class Multiconf:

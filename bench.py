@@ -104,6 +104,88 @@ def probe_hc(args, space, C0, H, G, stream):
                       "copy_bytes_read": COPY_PROBE_BYTES, "copy_bytes_written": COPY_PROBE_BYTES}), flush=True)
 
 
+def full_step_bench(args, stream):
+    """The whole MCTDHB time step (orbital split step, h/g rebuild, SIL,
+    density sync; reference driver.c:159-202) on the reference's own
+    ground-state case: N=10, M=4 (NC=286), Ng=256 on [-8, 8], harmonic trap,
+    g=1, imaginary time dt=0.005, Lanczos dimension 5, FP64.  Eager steps and
+    replays of the captured step graph run from the same initial state; the
+    oracle's full step is timed on a bounded sample of the same run."""
+    import torch
+    import mctdhb_amd as amd
+    from mctdhb_amd import inputs
+
+    N, M, grid, dt, lm, g = 10, 4, 256, 0.005, 5, 1.0
+    x = np.linspace(-8.0, 8.0, grid)
+    pot = 0.5 * x ** 2
+    orb0 = inputs.hermite_gauss(M, x, seed=args.seed, noise=1e-3)
+    orb0[:, -1] = orb0[:, 0]
+    space = amd.DeviceSpace(N, M, tables=False)
+    dim = math.comb(N + M - 1, M - 1)
+
+    def state():
+        w = amd.DeviceOrbital(M, grid, x[0], x[-1], dt, g, pot, imagtime=True)
+        sil = amd.DeviceSil(space, lm, amd.FP64, amd.INDEX_FLY)
+        d = {"orb": torch.from_numpy(np.ascontiguousarray(orb0.ravel())).cuda(),
+             "coef": torch.from_numpy(inputs.coefficients(dim, args.seed)).cuda()}
+        for k, n in (("hob", M * M), ("hint", M ** 4), ("r1", M * M), ("r2", M ** 4), ("rinv", M * M)):
+            d[k] = torch.zeros(n, dtype=torch.complex128, device="cuda")
+        w.orthonormalize(d["orb"], stream=stream)
+        w.orbital_matrices(d["orb"], d["hob"], d["hint"], stream=stream)
+        amd.density_sync(space, d["coef"], d["r1"], d["r2"], d["rinv"], stream=stream)
+        torch.cuda.synchronize()
+        return w, sil, d
+
+    warm, timed = 20, 500
+
+    def timeit(fn):
+        for _ in range(warm):
+            fn()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(timed):
+            fn()
+        b.record(stream)
+        b.synchronize()
+        return a.elapsed_time(b) / timed
+
+    w_e, sil_e, de = state()
+    host0 = {k: v.cpu().numpy() for k, v in de.items()}
+    eager_ms = timeit(lambda: amd.propagate_step(space, sil_e, w_e, de["orb"], de["coef"], de["hob"], de["hint"],
+                                                 de["r1"], de["r2"], de["rinv"], stream=stream))
+    w_g, sil_g, dg = state()
+    graph = amd.StepGraph(space, sil_g, w_g, dg["orb"], dg["coef"], dg["hob"], dg["hint"], dg["r1"], dg["r2"],
+                          dg["rinv"])
+    graph_ms = timeit(lambda: graph.launch(stream))
+    same = all(torch.equal(de[k], dg[k]) for k in de)
+    r1 = de["r1"].cpu().numpy().reshape(M, M)
+    graph.close()
+    out = {"workload": f"imaginary-time MCTDHB steps N={N} M={M} NC={dim} Ng={grid} dt={dt} m={lm} FP64",
+           "eager_ms_per_step": eager_ms, "graph_ms_per_step": graph_ms,
+           "graph_steps_per_s": 1e3 / graph_ms, "graph_vs_eager": eager_ms / graph_ms,
+           "time_to_2000_steps_s": 2000 * graph_ms / 1e3, "graph_equals_eager_bitwise": bool(same),
+           "after_steps": warm + timed,
+           "natural_occupations": [float(v) for v in np.sort(np.linalg.eigvalsh(r1))[::-1] / N]}
+    log(f"full step: eager {eager_ms:.3f} ms, graph {graph_ms:.3f} ms, bitwise equal {same}")
+    if not args.no_cpu:
+        import oracle
+
+        S = oracle.Space(N, M)
+        eq = oracle.make_orbeq(M, grid, x[0], x[-1], dt, g, pot, imagtime=True)
+        h = {k: np.ascontiguousarray(v) for k, v in host0.items()}
+        lvec = np.zeros(lm * dim, complex)
+        steps = 200
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            oracle.lib().orc_propagate_step(S._p, eq, 1, 1, lm, h["orb"], h["coef"], h["hob"], h["hint"], h["r1"],
+                                            h["r2"], h["rinv"], lvec)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": steps / el, "unit": "steps/s",
+                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                               "kind": "port", "sample": f"{steps} full steps of the same run"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +199,8 @@ def main():
     ap.add_argument("--hc-reps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-full-step", action="store_true",
+                    help="skip the whole-step (orbitals + SIL + densities) eager/graph timing")
     ap.add_argument("--probe-hc", action="store_true",
                     help="only the mixed H.C kernel (5 launches) and a plain copy kernel of known size, "
                          "for hardware-counter runs")
@@ -266,6 +350,8 @@ def main():
                 "this kernel computes jump targets from rank deltas and reads no tables",
     }
 
+    full = full_step_bench(args, stream) if not args.no_full_step else None
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -309,6 +395,7 @@ def main():
         "table_build_s": table_build_s,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "full_step": full,
     }
     print(json.dumps(line), flush=True)
 

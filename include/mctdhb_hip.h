@@ -268,6 +268,21 @@ int mctdhb_propagate_step_dev(mctdhb_space* space, mctdhb_sil* sil, mctdhb_orbit
                               void* coef, void* hob, void* hint, void* rho1, void* rho2, void* rinv,
                               void* stream);
 
+/* The same step recorded once as a HIP graph and replayed per step: the
+ * body of the reference's time loop (driver.c:159-202 inside
+ * driver.c:283-320) as one launch.  The graph holds the buffer addresses
+ * given here and the scalars of the moment of capture (dt, g, Runge-Kutta
+ * order, Lanczos size); the trap is read from the orbital workspace at each
+ * launch, so mctdhb_orbital_set_potential between launches takes effect.
+ * Covers 1 <= norb <= 7 (the fused density kernels).  Replays run in stream
+ * order on the stream given to launch. */
+typedef struct mctdhb_step_graph mctdhb_step_graph;
+int mctdhb_step_graph_create(mctdhb_space* space, mctdhb_sil* sil, mctdhb_orbital* w, int use_fft, void* orb,
+                             void* coef, void* hob, void* hint, void* rho1, void* rho2, void* rinv,
+                             mctdhb_step_graph** out);
+int mctdhb_step_graph_launch(mctdhb_step_graph* g, void* stream);
+int mctdhb_step_graph_destroy(mctdhb_step_graph* g);
+
 #ifdef __cplusplus
 }
 #endif

@@ -199,3 +199,38 @@ def test_ground_state_known_answer_on_device(orc, amd):
     r1 = gu.host(d["r1"])
     e = orc.total_energy(norb, r1, gu.host(d["r2"]), gu.host(d["hob"]), gu.host(d["hint"])).real / npar
     assert abs(e - 0.9685859) < 3e-4, e
+
+
+@pytest.mark.parametrize("imagtime,use_fft,prec", [(True, True, 0), (False, True, 0), (False, False, 0),
+                                                   (False, True, 1), (True, False, 1)])
+def test_step_graph_replays_match_eager_steps(orc, amd, imagtime, use_fft, prec):
+    """The captured step, replayed, gives after every replay the same arrays
+    as the eager step on a second set of buffers: the same kernels run in the
+    same order on the same data, so the bar is bit-exact.  Eager steps, copies
+    to the host and a trap change run between replays."""
+    import torch
+
+    print(f"step graph imag={imagtime} fft={use_fft} prec={prec}", flush=True)
+    npar, norb, grid, lm, dt, g = 6, 4, 129, 5, 0.01, 1.0
+    S, eq, D, sil, w, h, d = _full_setup(orc, amd, npar, norb, grid, imagtime, g, dt, lm=lm, prec=prec)
+    if prec:
+        D.hc_tiling(0)  # both step forms then run the same H.C tiling
+    x = np.linspace(-8.0, 8.0, grid)
+    sil_g = amd.DeviceSil(D, lm, prec=prec)
+    w_g = amd.DeviceOrbital(norb, grid, x[0], x[-1], dt, g, 0.5 * x ** 2, imagtime=imagtime)
+    d_g = {k: gu.dev(v) for k, v in h.items()}
+    graph = amd.StepGraph(D, sil_g, w_g, d_g["orb"], d_g["coef"], d_g["hob"], d_g["hint"], d_g["r1"], d_g["r2"],
+                          d_g["rinv"], use_fft=use_fft)
+    for r in range(6):
+        if r == 3:  # the trap is read at each launch
+            pot = 0.5 * x ** 2 + 0.3 * np.cos(x)
+            w.set_potential(pot)
+            w_g.set_potential(pot)
+        amd.propagate_step(D, sil, w, d["orb"], d["coef"], d["hob"], d["hint"], d["r1"], d["r2"], d["rinv"],
+                           use_fft=use_fft)
+        graph.launch()
+        torch.cuda.synchronize()
+        for k in ("orb", "coef", "hob", "hint", "r1", "r2", "rinv"):
+            a, b = gu.host(d_g[k]), gu.host(d[k])
+            assert np.array_equal(a, b), (r, k, gu.rel_err(a, b))
+    graph.close()
