@@ -155,20 +155,31 @@ __global__ __launch_bounds__(256) void k_dm_reduce(const double2* __restrict__ p
 
 __device__ __forceinline__ double2 dconj(double2 a) { return make_double2(a.x, -a.y); }
 
-// Symmetric fill of reference density_matrices.c:23-50 and :76-442.
-__global__ void k_dm_assemble(int M, const double2* __restrict__ tot, double2* __restrict__ rho1,
-                              double2* __restrict__ rho2) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Symmetric fill of reference density_matrices.c:23-50 and :76-442.  Every
+// element of rho1 and rho2 is written exactly once, so the fill is spread
+// over the threads of one block with no ordering: work item t is a diagonal
+// index k (t < M), an orbital pair k < l (Rules 1-5, 7 and 3), a (k, q < l)
+// triple of Rule 6 or a (k < s, q < l) quadruple of Rule 8.
+__global__ __launch_bounds__(256) void k_dm_assemble(int M, const double2* __restrict__ tot,
+                                                     double2* __restrict__ rho1, double2* __restrict__ rho2) {
     const DmLayout L{M};
-    const int M2 = M * M, M3 = M2 * M;
+    const int M2 = M * M, M3 = M2 * M, np = c2(M);
     auto R2 = [&](int a, int b, int cc, int d) -> double2& { return rho2[a + b * M + cc * M2 + d * M3]; };
-    for (int k = 0; k < M; k++) {
-        rho1[k * M + k] = make_double2(tot[L.d1() + k].x, 0);
-        if (rho2) R2(k, k, k, k) = make_double2(tot[L.d2() + k].x, 0);
-    }
-    for (int k = 0; k < M; k++)
-        for (int l = k + 1; l < M; l++) {
-            const int p = L.pidx(k, l);
+    auto pair_of = [&](int p, int& a, int& b) {
+        a = 0;
+        while (p >= M - 1 - a) p -= M - 1 - a++;
+        b = a + 1 + p;
+    };
+    const int n_diag = M, n_pair = np, n_r6 = rho2 ? M * np : 0, n_r8 = rho2 ? np * np : 0;
+    for (int t = threadIdx.x; t < n_diag + n_pair + n_r6 + n_r8; t += blockDim.x) {
+        if (t < n_diag) {
+            const int k = t;
+            rho1[k * M + k] = make_double2(tot[L.d1() + k].x, 0);
+            if (rho2) R2(k, k, k, k) = make_double2(tot[L.d2() + k].x, 0);
+        } else if (t < n_diag + n_pair) {
+            const int p = t - n_diag;
+            int k, l;
+            pair_of(p, k, l);
             const double2* T = tot + L.pair() + p * (M + 1);
             rho1[k * M + l] = T[0];
             rho1[l * M + k] = dconj(T[0]);
@@ -203,29 +214,29 @@ __global__ void k_dm_assemble(int M, const double2* __restrict__ tot, double2* _
             const double2 r3 = tot[L.o3() + p];  // Rule 3 (k < q = l)
             R2(k, k, l, l) = r3;
             R2(l, l, k, k) = dconj(r3);
+        } else if (t < n_diag + n_pair + n_r6) {  // Rule 6
+            const int u = t - n_diag - n_pair, k = u / np;
+            int q, l;
+            pair_of(u % np, q, l);
+            if (q == k || l == k) continue;
+            const double2 r = tot[L.o6() + k * c2(M - 1) + L.pidx_excl1(k, q, l)];
+            R2(k, k, q, l) = r;
+            R2(k, k, l, q) = r;
+            R2(l, q, k, k) = dconj(r);
+            R2(q, l, k, k) = dconj(r);
+        } else {  // Rule 8
+            const int u = t - n_diag - n_pair - n_r6;
+            int k, s, q, l;
+            pair_of(u / np, k, s);
+            pair_of(u % np, q, l);
+            if (q == k || q == s || l == k || l == s) continue;
+            const double2 r = tot[L.t8() + L.pidx(k, s) * c2(M - 2) + L.pidx_excl2(k, s, q, l)];
+            R2(k, s, q, l) = r;
+            R2(s, k, q, l) = r;
+            R2(k, s, l, q) = r;
+            R2(s, k, l, q) = r;
         }
-    if (!rho2) return;
-    for (int k = 0; k < M; k++)  // Rule 6
-        for (int q = 0; q < M; q++)
-            for (int l = q + 1; l < M; l++) {
-                if (q == k || l == k) continue;
-                const double2 r = tot[L.o6() + k * c2(M - 1) + L.pidx_excl1(k, q, l)];
-                R2(k, k, q, l) = r;
-                R2(k, k, l, q) = r;
-                R2(l, q, k, k) = dconj(r);
-                R2(q, l, k, k) = dconj(r);
-            }
-    for (int k = 0; k < M; k++)  // Rule 8
-        for (int s = k + 1; s < M; s++)
-            for (int q = 0; q < M; q++)
-                for (int l = q + 1; l < M; l++) {
-                    if (q == k || q == s || l == k || l == s) continue;
-                    const double2 r = tot[L.t8() + L.pidx(k, s) * c2(M - 2) + L.pidx_excl2(k, s, q, l)];
-                    R2(k, s, q, l) = r;
-                    R2(s, k, q, l) = r;
-                    R2(k, s, l, q) = r;
-                    R2(s, k, l, q) = r;
-                }
+    }
 }
 
 static int dispatch_dm(const Space* S, const void* c, int prec, int index, double2* part, unsigned nb,
@@ -279,7 +290,7 @@ int density_matrices(const Space* S, const void* coef, double2* rho1, double2* r
     }
     if (rc) return rc;
     k_dm_reduce<<<E, 256, 0, st>>>(part, (int) nb, E, tot);
-    k_dm_assemble<<<1, 64, 0, st>>>(M, tot, rho1, rho2);
+    k_dm_assemble<<<1, 256, 0, st>>>(M, tot, rho1, rho2);
     MCTDHB_HIP_CHECK(hipGetLastError());
     return kOk;
 }

@@ -241,10 +241,10 @@ __device__ __forceinline__ void run_pass(const RowCtx& ctx, const cplx<R>* __res
 }
 
 template <int M, typename R, int IDX, int... P>
-__device__ __forceinline__ void run_all(std::integer_sequence<int, P...>, const RowCtx& ctx,
+__device__ __forceinline__ void run_one(std::integer_sequence<int, P...>, int pass, const RowCtx& ctx,
                                         const cplx<R>* __restrict__ c, uint32_t base, const uint32_t* Wl, int ldw,
                                         const R* sql, double* red, double2* part) {
-    (run_pass<M, R, IDX, P>(ctx, c, base, Wl, ldw, sql, red, part), ...);
+    ((P == pass ? run_pass<M, R, IDX, P>(ctx, c, base, Wl, ldw, sql, red, part) : (void) 0), ...);
 }
 
 template <int M, typename R, int IDX>
@@ -264,8 +264,12 @@ __global__ __launch_bounds__(kDmBlock) void k_density(RowCtx ctx, const cplx<R>*
         for (int t = threadIdx.x; t < ctx.npar + 3; t += blockDim.x) sql[t] = sqrt((R) t);
     __syncthreads();
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-    run_all<M, R, IDX>(std::make_integer_sequence<int, Passes<M>::count()>{}, ctx, c, b * kDmTile, Wl, ldw,
-                       use_sq ? sql : nullptr, red, part);
+    // gridDim.y == 1: every pass in turn over the tile; otherwise pass blockIdx.y only
+    const int p0 = gridDim.y > 1 ? (int) blockIdx.y : 0;
+    const int p1 = gridDim.y > 1 ? p0 + 1 : Passes<M>::count();
+    for (int p = p0; p < p1; p++)
+        run_one<M, R, IDX>(std::make_integer_sequence<int, Passes<M>::count()>{}, p, ctx, c, b * kDmTile, Wl, ldw,
+                           use_sq ? sql : nullptr, red, part);
 }
 
 template <int M> static constexpr size_t dm_red_bytes() {
@@ -286,15 +290,17 @@ int DmFused<M>::run(const Space* S, const void* c, int prec, int index, double2*
     }
     RowCtx ctx = make_row_ctx(S->dim, S->npar, S->d_ht, S->d_map, S->d_mapot, S->d_maptt, S->d_strideot,
                               S->d_stridett, S->d_occ8);
+    // Spaces too small to fill the chip with tiles run one pass per block.
+    const dim3 grid(nb, nb < 256 ? Passes<M>::count() : 1);
     if (prec == kDmMixed)
-        k_density<M, float, kIdxFly><<<nb, kDmBlock, smem, st>>>(ctx, (const float2*) c, S->d_W, S->ldw, use_sq,
-                                                                 part);
+        k_density<M, float, kIdxFly><<<grid, kDmBlock, smem, st>>>(ctx, (const float2*) c, S->d_W, S->ldw, use_sq,
+                                                                   part);
     else if (fly)
-        k_density<M, double, kIdxFly><<<nb, kDmBlock, smem, st>>>(ctx, (const double2*) c, S->d_W, S->ldw,
-                                                                  use_sq, part);
+        k_density<M, double, kIdxFly><<<grid, kDmBlock, smem, st>>>(ctx, (const double2*) c, S->d_W, S->ldw,
+                                                                    use_sq, part);
     else
-        k_density<M, double, kIdxTables><<<nb, kDmBlock, smem, st>>>(ctx, (const double2*) c, S->d_W, S->ldw,
-                                                                     use_sq, part);
+        k_density<M, double, kIdxTables><<<grid, kDmBlock, smem, st>>>(ctx, (const double2*) c, S->d_W, S->ldw,
+                                                                       use_sq, part);
     MCTDHB_HIP_CHECK(hipGetLastError());
     return kOk;
 }

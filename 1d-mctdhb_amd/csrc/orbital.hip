@@ -397,28 +397,51 @@ __global__ __launch_bounds__(256) void k_nonlinear(const double2* __restrict__ o
     der[idx] = cmul(mi_tf, total);
 }
 
-int nonlinear_rhs(OrbWork* w, const double2* orb, const double2* hob, const double2* hint, const double2* rho2,
-                  const double2* rinv, double2* der, hipStream_t st) {
-    const OrbEq& q = w->eq;
-    const int total = q.norb * q.grid_size;
-    k_nonlinear<<<div_up(total, 256), 256, 0, st>>>(orb, q.norb, q.grid_size, q.g, q.time_fac, w->d_pot, hob, hint,
-                                                    rho2, rinv, der);
-    MCTDHB_HIP_CHECK(hipGetLastError());
-    return kOk;
-}
-
-__global__ void k_axpy_stage(const double2* __restrict__ y0, const double2* __restrict__ k, double a,
-                             double2* __restrict__ out, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = y0[i] + cscale(k[i], a);
-}
-
-__global__ void k_rk4_combine(double2* __restrict__ y, const double2* __restrict__ k1, const double2* __restrict__ k2,
-                              const double2* __restrict__ k3, const double2* __restrict__ k4, double dt, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        const double2 s = k1[i] + cscale(k2[i], 2.0) + cscale(k3[i], 2.0) + k4[i];
-        y[i] = y[i] + cscale(s, dt / 6);
+// The same term with the grid-independent factors summed once per step.
+// Grouping the terms of the loops above by the orbital product they
+// multiply gives
+//   nonlinear(k, x) = g sum_{s, q <= l} A[k][ql][s] conj(f_s) f_q f_l
+//                     - sum_j L[k][j] f_j
+// with c_ql = 1 (q == l) or 2 (q < l) and, from the a == s, a < s and swap
+// branches,
+//   A[k][ql][s] = c_ql (sum_{a <= s} rinv[k][a] rho[a + M s + M^2 q + M^3 l]
+//                       + sum_{t > s} rinv[k][t] rho[s + M t + M^2 q + M^3 l]),
+//   L[k][j] = hob[j][k] + sum over the same (a, s, q, l, coefficient) of
+//             coefficient * hint[...] as in the loops above.
+// ql runs over q = 0..M-1, l = q..M-1.  Layout: A (M * P * M entries,
+// P = M (M + 1) / 2), then L (M * M).  One block.
+__global__ __launch_bounds__(256) void k_nl_coef(int M, const double2* __restrict__ hob,
+                                                 const double2* __restrict__ hint, const double2* __restrict__ rho,
+                                                 const double2* __restrict__ rinv, double2* __restrict__ out) {
+    const int mm = M * M, mmm = mm * M, P = M * (M + 1) / 2, nA = M * P * M;
+    for (int e = threadIdx.x; e < nA + mm; e += blockDim.x) {
+        if (e < nA) {
+            const int s = e % M, ql = (e / M) % P, k = e / (M * P);
+            int q = 0, rem = ql;
+            while (rem >= M - q) rem -= M - q++;
+            const int l = q + rem;
+            double2 acc = make_double2(0, 0);
+            for (int a = 0; a <= s; a++) acc = acc + cmul(rinv[k * M + a], rho[a + M * s + mm * q + mmm * l]);
+            for (int t = s + 1; t < M; t++) acc = acc + cmul(rinv[k * M + t], rho[s + M * t + mm * q + mmm * l]);
+            out[e] = cscale(acc, q == l ? 1.0 : 2.0);
+        } else {
+            const int k = (e - nA) / M, j = (e - nA) % M;
+            double2 acc = hob[j * M + k];
+            for (int a = 0; a < M; a++)
+                for (int q = 0; q < M; q++)
+                    for (int l = q; l < M; l++) {
+                        const double c = q == l ? 1.0 : 2.0;
+                        const double2 rka = cscale(rinv[k * M + a], c);
+                        acc = acc + cmul(cmul(rka, rho[a + M * a + mm * q + mmm * l]), hint[j + a * M + l * mm + q * mmm]);
+                        for (int t = a + 1; t < M; t++) {
+                            const double2 r = rho[a + M * t + mm * q + mmm * l];
+                            const double2 rm = cmul(rka, r), sw = cmul(cscale(rinv[k * M + t], c), r);
+                            acc = acc + cmul(rm, hint[j + t * M + l * mm + q * mmm]);
+                            acc = acc + cmul(sw, hint[j + a * M + l * mm + q * mmm]);
+                        }
+                    }
+            out[e] = acc;
+        }
     }
 }
 
@@ -430,7 +453,124 @@ struct RkSum {
     int ns;
 };
 
-// (y0 and out may be the same array: each element is read before it is written.)
+// der[k][x] = -i tf (nonlinear(k, x) + V(x) f_k(x)) with f = orb + dt * stage
+// (the stage sum formed in registers), from the k_nl_coef tables.
+// Grid: (grid points / 64, M); each block serves one k, so the table reads
+// are uniform across the block.
+template <int M>
+__global__ __launch_bounds__(64) void k_nl_apply(int n, const double2* __restrict__ orb, RkSum stage, double dt,
+                                                 double g, double2 tf, const double* __restrict__ V,
+                                                 const double2* __restrict__ coef, double2* __restrict__ der) {
+    constexpr int P = M * (M + 1) / 2;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (x >= n) return;
+    double2 f[M];
+#pragma unroll
+    for (int a = 0; a < M; a++) {
+        double2 v = orb[(size_t) a * n + x];
+        if (stage.ns) {
+            double2 acc = cscale(stage.k[0][(size_t) a * n + x], stage.a[0]);
+            for (int j = 1; j < stage.ns; j++) acc = acc + cscale(stage.k[j][(size_t) a * n + x], stage.a[j]);
+            v = v + cscale(acc, dt);
+        }
+        f[a] = v;
+    }
+    const double2* A = coef + (size_t) k * P * M;
+    double2 acc[M];
+#pragma unroll
+    for (int s = 0; s < M; s++) acc[s] = make_double2(0, 0);
+    int ql = 0;
+#pragma unroll
+    for (int q = 0; q < M; q++)
+#pragma unroll
+        for (int l = q; l < M; l++, ql++) {
+            const double2 pql = cmul(f[q], f[l]);
+#pragma unroll
+            for (int s = 0; s < M; s++) acc[s] = acc[s] + cmul(A[ql * M + s], pql);
+        }
+    double2 cubic = make_double2(0, 0);
+#pragma unroll
+    for (int s = 0; s < M; s++) cubic = cubic + cconjmul(f[s], acc[s]);
+    double2 total = cscale(cubic, g) + cscale(f[k], V[x]);
+    const double2* L = coef + (size_t) M * P * M + k * M;
+#pragma unroll
+    for (int j = 0; j < M; j++) total = total - cmul(L[j], f[j]);
+    const double2 mi_tf = make_double2(tf.y, -tf.x);
+    der[(size_t) k * n + x] = cmul(mi_tf, total);
+}
+
+__global__ void k_rk_sum(const double2* y0, RkSum s, double dt, double2* out, int n);
+
+// Grid-independent factors of the nonlinear term (orbitals up to
+// kNlFusedMaxOrb; larger counts use k_nonlinear directly).
+constexpr int kNlFusedMaxOrb = 8;
+
+static int nonlinear_prepare(OrbWork* w, const double2* hob, const double2* hint, const double2* rho2,
+                             const double2* rinv, hipStream_t st) {
+    if (w->eq.norb > kNlFusedMaxOrb) return kOk;
+    k_nl_coef<<<1, 256, 0, st>>>(w->eq.norb, hob, hint, rho2, rinv, w->d_nl);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
+// der = rhs(orb + dt * stage) after nonlinear_prepare.
+static int nonlinear_stage(OrbWork* w, const double2* orb, const RkSum& stage, const double2* hob,
+                           const double2* hint, const double2* rho2, const double2* rinv, double2* der,
+                           hipStream_t st) {
+    const OrbEq& q = w->eq;
+    const int n = q.grid_size, M = q.norb;
+    if (M > kNlFusedMaxOrb) {
+        const int total = M * n;
+        const double2* in = orb;
+        if (stage.ns) {
+            double2* y = w->d_stage + 6 * (size_t) total;
+            k_rk_sum<<<div_up(total, 256), 256, 0, st>>>(orb, stage, q.tstep, y, total);
+            in = y;
+        }
+        k_nonlinear<<<div_up(total, 256), 256, 0, st>>>(in, M, n, q.g, q.time_fac, w->d_pot, hob, hint, rho2,
+                                                        rinv, der);
+        MCTDHB_HIP_CHECK(hipGetLastError());
+        return kOk;
+    }
+    const dim3 grid(div_up(n, 64), M);
+#define NL_CASE(m)                                                                                             \
+    case m:                                                                                                   \
+        k_nl_apply<m><<<grid, 64, 0, st>>>(n, orb, stage, q.tstep, q.g, q.time_fac, w->d_pot, w->d_nl, der); \
+        break;
+    switch (M) {
+        NL_CASE(1)
+        NL_CASE(2)
+        NL_CASE(3)
+        NL_CASE(4)
+        NL_CASE(5)
+        NL_CASE(6)
+        NL_CASE(7)
+        NL_CASE(8)
+    }
+#undef NL_CASE
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
+int nonlinear_rhs(OrbWork* w, const double2* orb, const double2* hob, const double2* hint, const double2* rho2,
+                  const double2* rinv, double2* der, hipStream_t st) {
+    int rc = nonlinear_prepare(w, hob, hint, rho2, rinv, st);
+    if (rc) return rc;
+    return nonlinear_stage(w, orb, RkSum{}, hob, hint, rho2, rinv, der, st);
+}
+
+__global__ void k_rk4_combine(double2* __restrict__ y, const double2* __restrict__ k1, const double2* __restrict__ k2,
+                              const double2* __restrict__ k3, const double2* __restrict__ k4, double dt, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double2 s = k1[i] + cscale(k2[i], 2.0) + cscale(k3[i], 2.0) + k4[i];
+        y[i] = y[i] + cscale(s, dt / 6);
+    }
+}
+
+// out = y0 + dt (a_0 k_0 + a_1 k_1 + ...); y0 and out may be the same array:
+// each element is read before it is written.
 __global__ void k_rk_sum(const double2* y0, RkSum s, double dt, double2* out, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -459,21 +599,16 @@ static int rk_tableau(OrbWork* w, int ns, const double* A, const double* b, doub
     const int total = q.norb * q.grid_size;
     const unsigned nb = div_up(total, 256);
     double2* k = w->d_stage;
-    double2* y = w->d_stage + 6 * (size_t) total;
-    int rc;
+    int rc = nonlinear_prepare(w, hob, hint, rho2, rinv, st);
+    if (rc) return rc;
     for (int s = 0; s < ns; s++) {
-        const double2* in = orb;
-        if (s > 0) {
-            RkSum r{};
-            for (int j = 0; j < s; j++) {
-                r.k[j] = k + (size_t) j * total;
-                r.a[j] = A[s * ns + j];
-            }
-            r.ns = s;
-            k_rk_sum<<<nb, 256, 0, st>>>(orb, r, q.tstep, y, total);
-            in = y;
+        RkSum r{};
+        for (int j = 0; j < s; j++) {
+            r.k[j] = k + (size_t) j * total;
+            r.a[j] = A[s * ns + j];
         }
-        if ((rc = nonlinear_rhs(w, in, hob, hint, rho2, rinv, k + (size_t) s * total, st))) return rc;
+        r.ns = s;
+        if ((rc = nonlinear_stage(w, orb, r, hob, hint, rho2, rinv, k + (size_t) s * total, st))) return rc;
     }
     RkSum r{};
     for (int j = 0; j < ns; j++) {
@@ -493,10 +628,11 @@ static int rk_tableau(OrbWork* w, int ns, const double* A, const double* b, doub
 // S_p = <F_j, F_i> evaluated after points < p were updated,
 // S_{p+1} = S_p (1 - w_p |F_j[p]|^2), so S_p = S_0 prod_{q<p} (1 - w_q |F_j[q]|^2):
 // one reduction and one multiplicative scan per (i, j).
-__global__ __launch_bounds__(1024) void k_gram_schmidt(double2* __restrict__ F, int M, int n, double dx) {
+__global__ __launch_bounds__(256) void k_gram_schmidt(double2* __restrict__ F, int M, int n, double dx) {
     __shared__ double red[64];
-    __shared__ double scan[1024];
+    __shared__ double wave_prod[4];
     const int T = blockDim.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int per = (n + T - 1) / T;  // contiguous points per thread
     const int p0 = threadIdx.x * per;
     auto renorm = [&](double2* f) {
@@ -525,15 +661,17 @@ __global__ __launch_bounds__(1024) void k_gram_schmidt(double2* __restrict__ F, 
             double prod = 1.0;
             for (int p = p0; p < p0 + per && p < n; p++)
                 prod *= 1.0 - simpson_w(p, n, dx) * (Fj[p].x * Fj[p].x + Fj[p].y * Fj[p].y);
-            scan[threadIdx.x] = prod;
-            __syncthreads();
-            for (int off = 1; off < T; off <<= 1) {
-                const double o = threadIdx.x >= (unsigned) off ? scan[threadIdx.x - off] : 1.0;
-                __syncthreads();
-                scan[threadIdx.x] *= o;
-                __syncthreads();
+            // inclusive product scan within the wave, then over the waves
+            double incl = prod;
+            for (int off = 1; off < 64; off <<= 1) {
+                const double o = __shfl_up(incl, off, 64);
+                if (lane >= off) incl *= o;
             }
-            double run = threadIdx.x > 0 ? scan[threadIdx.x - 1] : 1.0;
+            if (lane == 63) wave_prod[wave] = incl;
+            double run = __shfl_up(incl, 1, 64);
+            if (lane == 0) run = 1.0;
+            __syncthreads();
+            for (int u = 0; u < wave; u++) run *= wave_prod[u];
             for (int p = p0; p < p0 + per && p < n; p++) {
                 const double2 coef = cscale(S0, run);
                 Fi[p] = Fi[p] - cmul(coef, Fj[p]);
@@ -546,7 +684,7 @@ __global__ __launch_bounds__(1024) void k_gram_schmidt(double2* __restrict__ F, 
 }
 
 int orthonormalize(OrbWork* w, double2* orb, hipStream_t st) {
-    k_gram_schmidt<<<1, 1024, 0, st>>>(orb, w->eq.norb, w->eq.grid_size, w->eq.dx);
+    k_gram_schmidt<<<1, 256, 0, st>>>(orb, w->eq.norb, w->eq.grid_size, w->eq.dx);
     MCTDHB_HIP_CHECK(hipGetLastError());
     return kOk;
 }
@@ -577,16 +715,22 @@ int splitstep(OrbWork* w, int use_fft, double2* orb, double2* hob, double2* hint
     } else if (w->rk_order == 5) {
         rc = rk_tableau(w, 6, kRk5A, kRk5B, orb, hob, hint, rho2, rinv, st);
     } else {
-        double2 *k1 = w->d_stage, *k2 = k1 + total, *k3 = k2 + total, *k4 = k3 + total, *y = k4 + total;
+        // classical RK4: stages at orb, orb + dt/2 k1, orb + dt/2 k2, orb + dt k3
+        double2 *k1 = w->d_stage, *k2 = k1 + total, *k3 = k2 + total, *k4 = k3 + total;
         const double dt = q.tstep;
         const unsigned nb = div_up(total, 256);
-        if ((rc = nonlinear_rhs(w, orb, hob, hint, rho2, rinv, k1, st))) return rc;
-        k_axpy_stage<<<nb, 256, 0, st>>>(orb, k1, 0.5 * dt, y, total);
-        if ((rc = nonlinear_rhs(w, y, hob, hint, rho2, rinv, k2, st))) return rc;
-        k_axpy_stage<<<nb, 256, 0, st>>>(orb, k2, 0.5 * dt, y, total);
-        if ((rc = nonlinear_rhs(w, y, hob, hint, rho2, rinv, k3, st))) return rc;
-        k_axpy_stage<<<nb, 256, 0, st>>>(orb, k3, dt, y, total);
-        if ((rc = nonlinear_rhs(w, y, hob, hint, rho2, rinv, k4, st))) return rc;
+        RkSum r{};
+        r.a[0] = 0.5;
+        if ((rc = nonlinear_prepare(w, hob, hint, rho2, rinv, st))) return rc;
+        if ((rc = nonlinear_stage(w, orb, r, hob, hint, rho2, rinv, k1, st))) return rc;
+        r.ns = 1;
+        r.k[0] = k1;
+        if ((rc = nonlinear_stage(w, orb, r, hob, hint, rho2, rinv, k2, st))) return rc;
+        r.k[0] = k2;
+        if ((rc = nonlinear_stage(w, orb, r, hob, hint, rho2, rinv, k3, st))) return rc;
+        r.k[0] = k3;
+        r.a[0] = 1.0;
+        if ((rc = nonlinear_stage(w, orb, r, hob, hint, rho2, rinv, k4, st))) return rc;
         k_rk4_combine<<<nb, 256, 0, st>>>(orb, k1, k2, k3, k4, dt, total);
     }
     if (rc) return rc;
@@ -598,25 +742,35 @@ int splitstep(OrbWork* w, int use_fft, double2* orb, double2* hob, double2* hint
 
 // ---------------------------------------------------- M x M algebra
 
-// Hermitian eigen-decomposition by cyclic complex Jacobi rotations (one
-// thread; a is overwritten), eigenvalues ascending with eigenvectors in the
-// columns of v.  Returns 0 on convergence.
-__device__ int herm_eig_dev(int n, double2* a, double* lam, double2* v) {
-    for (int r = 0; r < n; r++)
+// Hermitian eigen-decomposition by cyclic complex Jacobi rotations on one
+// wave (n <= 64; a is overwritten, all in LDS): rotations run in the cyclic
+// order, lane r applies each one to row/column r.  Eigenvalues ascending in
+// lam (LDS), eigenvectors in the columns of v.  Every lane of the block
+// must call it.
+__device__ void herm_eig_wave(int n, double2* a, double* lam, double2* v, double* rowsum) {
+    const int r = threadIdx.x;
+    if (r < n)
         for (int c = 0; c < n; c++) v[r * n + c] = make_double2(r == c ? 1.0 : 0.0, 0.0);
-    int ok = 0;
+    __syncthreads();
     for (int sweep = 0; sweep < 100; sweep++) {
-        double off = 0, tot = 0;
-        for (int r = 0; r < n; r++)
+        if (r < n) {
+            double off = 0, tot = 0;
             for (int c = 0; c < n; c++) {
                 const double m2 = a[r * n + c].x * a[r * n + c].x + a[r * n + c].y * a[r * n + c].y;
                 tot += m2;
                 if (r != c) off += m2;
             }
-        if (off <= 1e-30 * tot || off == 0) {
-            ok = 1;
-            break;
+            rowsum[2 * r] = off;
+            rowsum[2 * r + 1] = tot;
         }
+        __syncthreads();
+        double off = 0, tot = 0;
+        for (int c = 0; c < n; c++) {
+            off += rowsum[2 * c];
+            tot += rowsum[2 * c + 1];
+        }
+        __syncthreads();
+        if (off <= 1e-30 * tot || off == 0) break;
         for (int p = 0; p < n - 1; p++)
             for (int q = p + 1; q < n; q++) {
                 const double2 apq = a[p * n + q];
@@ -627,118 +781,141 @@ __device__ int herm_eig_dev(int n, double2* a, double* lam, double2* v) {
                 const double tau = (aqq - app) / (2 * mag);
                 const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1 + tau * tau));
                 const double c = 1 / sqrt(1 + t * t), s = t * c;
-                for (int r = 0; r < n; r++) {
+                __syncthreads();
+                if (r < n) {
                     const double2 arp = a[r * n + p], arq = a[r * n + q];
                     a[r * n + p] = cscale(arp, c) - cmul(cscale(dconj2(ph), s), arq);
                     a[r * n + q] = cmul(cscale(ph, s), arp) + cscale(arq, c);
-                }
-                for (int r = 0; r < n; r++) {
-                    const double2 apr = a[p * n + r], aqr = a[q * n + r];
-                    a[p * n + r] = cscale(apr, c) - cmul(cscale(ph, s), aqr);
-                    a[q * n + r] = cmul(cscale(dconj2(ph), s), apr) + cscale(aqr, c);
-                }
-                a[p * n + q] = make_double2(0, 0);
-                a[q * n + p] = make_double2(0, 0);
-                for (int r = 0; r < n; r++) {
                     const double2 vrp = v[r * n + p], vrq = v[r * n + q];
                     v[r * n + p] = cscale(vrp, c) - cmul(cscale(dconj2(ph), s), vrq);
                     v[r * n + q] = cmul(cscale(ph, s), vrp) + cscale(vrq, c);
                 }
+                __syncthreads();
+                if (r < n) {
+                    const double2 apr = a[p * n + r], aqr = a[q * n + r];
+                    a[p * n + r] = cscale(apr, c) - cmul(cscale(ph, s), aqr);
+                    a[q * n + r] = cmul(cscale(dconj2(ph), s), apr) + cscale(aqr, c);
+                }
+                __syncthreads();
+                if (r == 0) {
+                    a[p * n + q] = make_double2(0, 0);
+                    a[q * n + p] = make_double2(0, 0);
+                }
+                __syncthreads();
             }
     }
-    for (int r = 0; r < n; r++) lam[r] = a[r * n + r].x;
+    if (r < n) lam[r] = a[r * n + r].x;
+    __syncthreads();
     for (int x = 0; x < n - 1; x++) {
         int best = x;
         for (int y = x + 1; y < n; y++)
             if (lam[y] < lam[best]) best = y;
+        __syncthreads();
         if (best != x) {
-            const double t = lam[x];
-            lam[x] = lam[best];
-            lam[best] = t;
-            for (int r = 0; r < n; r++) {
+            if (r == 0) {
+                const double t = lam[x];
+                lam[x] = lam[best];
+                lam[best] = t;
+            }
+            if (r < n) {
                 const double2 tv = v[r * n + x];
                 v[r * n + x] = v[r * n + best];
                 v[r * n + best] = tv;
             }
         }
+        __syncthreads();
     }
-    return ok ? 0 : 1;
 }
 
 // rho1 eigenvalues (synchronize.c:49), regularisation when the smallest
 // natural occupation is below reg_factor times the largest
 // (lapack_interface.c:126-163, synchronize.c:50-54), then the inverse from
-// the lower triangle (lapack_interface.c:10-59).  One thread, M <= 30.
-__global__ void k_density_sync_small(int M, double2* __restrict__ rho1, double2* __restrict__ rinv,
-                                     double* __restrict__ eig_out, double reg_factor) {
-    extern __shared__ double2 work[];  // 3 * M * M
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// the lower triangle (lapack_interface.c:10-59).  One wave, M <= 30; lane
+// r owns row r in the eigen-solve and column r in the regularisation and
+// the Gauss-Jordan elimination.
+__global__ __launch_bounds__(64) void k_density_sync_small(int M, double2* __restrict__ rho1,
+                                                           double2* __restrict__ rinv, double* __restrict__ eig_out,
+                                                           double reg_factor) {
+    extern __shared__ double2 work[];  // 4 * M * M
+    __shared__ double lam[64], rowsum[128];
+    __shared__ double2 fac[64];
+    const int r = threadIdx.x;
     double2* a = work;              // M*M
     double2* v = work + M * M;      // M*M
     double2* inv = work + 2 * M * M;
-    double lam[30];
+    double2* rl = work + 3 * M * M;  // rho1 as regularised
     // zheev reads the upper triangle with a real diagonal
-    for (int r = 0; r < M; r++)
-        for (int c = 0; c < M; c++)
-            a[r * M + c] = r == c ? make_double2(rho1[r * M + r].x, 0)
-                                  : (c > r ? rho1[r * M + c] : dconj2(rho1[c * M + r]));
-    herm_eig_dev(M, a, lam, v);
-    if (eig_out)
-        for (int r = 0; r < M; r++) eig_out[r] = lam[r];
-    if (lam[0] / lam[M - 1] < reg_factor) {
-        const double x = lam[M - 1] * reg_factor;
-        for (int i = 0; i < M; i++)
-            for (int j = 0; j < M; j++) {
-                double2 z = make_double2(0, 0);
-                for (int k = 0; k < M; k++)
-                    z = z + cmul(cscale(v[i * M + k], x * exp(-lam[k] / x)), dconj2(v[j * M + k]));
-                rho1[i * M + j] = rho1[i * M + j] + z;
-            }
-    }
-    // Gauss-Jordan inverse with partial pivoting of the Hermitian matrix
-    // given by the lower triangle and real diagonal
-    for (int r = 0; r < M; r++)
+    if (r < M)
         for (int c = 0; c < M; c++) {
             a[r * M + c] = r == c ? make_double2(rho1[r * M + r].x, 0)
-                                  : (c < r ? rho1[r * M + c] : dconj2(rho1[c * M + r]));
-            inv[r * M + c] = make_double2(r == c ? 1.0 : 0.0, 0);
+                                  : (c > r ? rho1[r * M + c] : dconj2(rho1[c * M + r]));
+            rl[r * M + c] = rho1[r * M + c];
         }
+    __syncthreads();
+    herm_eig_wave(M, a, lam, v, rowsum);
+    if (eig_out && r < M) eig_out[r] = lam[r];
+    if (lam[0] / lam[M - 1] < reg_factor) {
+        const double x = lam[M - 1] * reg_factor;
+        if (r < M)
+            for (int i = 0; i < M; i++) {
+                double2 z = make_double2(0, 0);
+                for (int k = 0; k < M; k++)
+                    z = z + cmul(cscale(v[i * M + k], x * exp(-lam[k] / x)), dconj2(v[r * M + k]));
+                rl[i * M + r] = rl[i * M + r] + z;
+                rho1[i * M + r] = rl[i * M + r];
+            }
+    }
+    __syncthreads();
+    // Gauss-Jordan inverse with partial pivoting of the Hermitian matrix
+    // given by the lower triangle and real diagonal
+    if (r < M)
+        for (int c = 0; c < M; c++) {
+            a[c * M + r] = c == r ? make_double2(rl[r * M + r].x, 0)
+                                  : (r < c ? rl[c * M + r] : dconj2(rl[r * M + c]));
+            inv[c * M + r] = make_double2(c == r ? 1.0 : 0.0, 0);
+        }
+    __syncthreads();
     for (int col = 0; col < M; col++) {
         int piv = col;
         double best = 0;
-        for (int r = col; r < M; r++) {
-            const double m2 = a[r * M + col].x * a[r * M + col].x + a[r * M + col].y * a[r * M + col].y;
+        for (int i = col; i < M; i++) {
+            const double m2 = a[i * M + col].x * a[i * M + col].x + a[i * M + col].y * a[i * M + col].y;
             if (m2 > best) {
                 best = m2;
-                piv = r;
+                piv = i;
             }
         }
         if (best == 0) break;
-        if (piv != col)
-            for (int c = 0; c < M; c++) {
-                double2 t = a[col * M + c];
-                a[col * M + c] = a[piv * M + c];
-                a[piv * M + c] = t;
-                t = inv[col * M + c];
-                inv[col * M + c] = inv[piv * M + c];
-                inv[piv * M + c] = t;
-            }
+        __syncthreads();
+        if (piv != col && r < M) {
+            double2 t = a[col * M + r];
+            a[col * M + r] = a[piv * M + r];
+            a[piv * M + r] = t;
+            t = inv[col * M + r];
+            inv[col * M + r] = inv[piv * M + r];
+            inv[piv * M + r] = t;
+        }
+        __syncthreads();
         const double2 d = cdiv(make_double2(1, 0), a[col * M + col]);
-        for (int c = 0; c < M; c++) {
-            a[col * M + c] = cmul(a[col * M + c], d);
-            inv[col * M + c] = cmul(inv[col * M + c], d);
+        __syncthreads();
+        if (r < M) {
+            a[col * M + r] = cmul(a[col * M + r], d);
+            inv[col * M + r] = cmul(inv[col * M + r], d);
+            fac[r] = a[r * M + col];
         }
-        for (int r = 0; r < M; r++) {
-            if (r == col) continue;
-            const double2 f = a[r * M + col];
-            if (f.x == 0 && f.y == 0) continue;
-            for (int c = 0; c < M; c++) {
-                a[r * M + c] = a[r * M + c] - cmul(f, a[col * M + c]);
-                inv[r * M + c] = inv[r * M + c] - cmul(f, inv[col * M + c]);
+        __syncthreads();
+        if (r < M)
+            for (int i = 0; i < M; i++) {
+                if (i == col) continue;
+                const double2 f = fac[i];
+                if (f.x == 0 && f.y == 0) continue;
+                a[i * M + r] = a[i * M + r] - cmul(f, a[col * M + r]);
+                inv[i * M + r] = inv[i * M + r] - cmul(f, inv[col * M + r]);
             }
-        }
+        __syncthreads();
     }
-    for (int i = 0; i < M * M; i++) rinv[i] = inv[i];
+    if (r < M)
+        for (int c = 0; c < M; c++) rinv[c * M + r] = inv[c * M + r];
 }
 
 int density_sync_small(int norb, double2* rho1, double2* rinv, double* eigvals, double reg_factor, hipStream_t st) {
@@ -746,8 +923,8 @@ int density_sync_small(int norb, double2* rho1, double2* rinv, double* eigvals, 
         set_error("norb exceeds 30");
         return kInvalidArgument;
     }
-    // at most 3 * 30 * 30 * 16 = 43200 bytes of LDS
-    k_density_sync_small<<<1, 64, sizeof(double2) * 3 * norb * norb, st>>>(norb, rho1, rinv, eigvals, reg_factor);
+    // at most 4 * 30 * 30 * 16 = 57600 bytes of LDS
+    k_density_sync_small<<<1, 64, sizeof(double2) * 4 * norb * norb, st>>>(norb, rho1, rinv, eigvals, reg_factor);
     MCTDHB_HIP_CHECK(hipGetLastError());
     return kOk;
 }
@@ -817,6 +994,8 @@ int orbwork_create(const OrbEq& eq, int imagtime, const double* pot_host, OrbWor
     ok = ok && hipMalloc((void**) &w->d_stage, sizeof(double2) * 7 * (size_t) M * grid) == hipSuccess;
     ok = ok && hipMalloc((void**) &w->d_red, sizeof(double) * 1024) == hipSuccess;
     ok = ok && hipMalloc((void**) &w->d_cnwork, sizeof(double2) * 8 * (size_t) M * grid) == hipSuccess;
+    ok = ok && hipMalloc((void**) &w->d_nl, sizeof(double2) * ((size_t) M * M * M * (M + 1) / 2 + M * M)) ==
+                   hipSuccess;
     if (!ok) {
         orbwork_destroy(w);
         set_error("device allocation failed for the orbital workspace");
@@ -848,6 +1027,7 @@ void orbwork_destroy(OrbWork* w) {
     hipFree(w->d_stage);
     hipFree(w->d_red);
     hipFree(w->d_cnwork);
+    hipFree(w->d_nl);
     delete w;
 }
 

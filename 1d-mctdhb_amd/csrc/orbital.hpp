@@ -29,6 +29,7 @@ struct OrbWork {
     double2* d_stage = nullptr;  // 7 * norb * grid_size: up to six stages and y
     double* d_red = nullptr;     // reduction scratch
     double2* d_cnwork = nullptr; // 8 * norb * grid_size: Crank-Nicolson sweep scratch
+    double2* d_nl = nullptr;     // grid-independent factors of the nonlinear term
 };
 
 int orbwork_create(const OrbEq& eq, int imagtime, const double* pot_host, OrbWork** out);

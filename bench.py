@@ -107,15 +107,17 @@ def probe_hc(args, space, C0, H, G, stream):
 def full_step_bench(args, stream):
     """The whole MCTDHB time step (orbital split step, h/g rebuild, SIL,
     density sync; reference driver.c:159-202) on the reference's own
-    ground-state case: N=10, M=4 (NC=286), Ng=256 on [-8, 8], harmonic trap,
-    g=1, imaginary time dt=0.005, Lanczos dimension 5, FP64.  Eager steps and
+    ground-state case: N=10, M=4 (NC=286), 256 grid intervals on [-8, 8]
+    (grid_size 257, the last point the periodic copy of the first, FFT length
+    256), harmonic trap, g=1, imaginary time dt=0.005, Lanczos dimension 5,
+    FP64.  Eager steps and
     replays of the captured step graph run from the same initial state; the
     oracle's full step is timed on a bounded sample of the same run."""
     import torch
     import mctdhb_amd as amd
     from mctdhb_amd import inputs
 
-    N, M, grid, dt, lm, g = 10, 4, 256, 0.005, 5, 1.0
+    N, M, grid, dt, lm, g = 10, 4, 257, 0.005, 5, 1.0
     x = np.linspace(-8.0, 8.0, grid)
     pot = 0.5 * x ** 2
     orb0 = inputs.hermite_gauss(M, x, seed=args.seed, noise=1e-3)
