@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <chrono>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -49,6 +50,10 @@ struct mctdhb_sil {
     Sil* L = nullptr;
     const mctdhb_space* sp = nullptr;
 };
+
+// Process-wide regularisation factor of the density sync of a whole step
+// (reference synchronize.c:9, DEFAULT_REGULARIZATION_FACTOR of synchronize.h:23).
+static double g_reg_factor = 1e-9;
 
 // ------------------------------------------------------------------ layer 2
 
@@ -464,13 +469,48 @@ extern "C" int mctdhb_propagate_step_dev(mctdhb_space* sp, mctdhb_sil* s, mctdhb
     const double2 dt = w->eq.prop_dt;
     rc = sil_step(s->L, (double2*) coef, (const double2*) hob, (const double2*) hint, dt.x, dt.y, w->imagtime, st);
     if (rc) return rc;
-    return mctdhb_density_sync_dev(sp, coef, rho1, rho2, rinv, nullptr, 1e-9, MCTDHB_FP64, stream);
+    return mctdhb_density_sync_dev(sp, coef, rho1, rho2, rinv, nullptr, g_reg_factor, MCTDHB_FP64, stream);
 }
 
 struct mctdhb_step_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
+
+// Records what body enqueues on a private stream into g.  Everything body
+// allocates or tunes on first use must already be done: the capture takes
+// kernels and device copies only.
+template <typename Body> static int capture_graph(Body body, mctdhb_step_graph* g) {
+    MCTDHB_HIP_CHECK(hipDeviceSynchronize());
+    hipStream_t cs = nullptr;
+    MCTDHB_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipGraph_t graph = nullptr;
+    int rc = kOk;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        set_error("hipStreamBeginCapture failed");
+        rc = kDeviceError;
+    } else {
+        rc = body(cs);
+        const hipError_t e = hipStreamEndCapture(cs, &graph);
+        if (!rc && e != hipSuccess) {
+            set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            rc = kDeviceError;
+        }
+    }
+    hipStreamDestroy(cs);
+    hipGraphExec_t exec = nullptr;
+    if (!rc && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        set_error("hipGraphInstantiate failed");
+        rc = kDeviceError;
+    }
+    if (rc) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+    }
+    g->graph = graph;
+    g->exec = exec;
+    return kOk;
+}
 
 extern "C" int mctdhb_step_graph_create(mctdhb_space* sp, mctdhb_sil* s, mctdhb_orbital* o, int use_fft, void* orb,
                                         void* coef, void* hob, void* hint, void* rho1, void* rho2, void* rinv,
@@ -491,35 +531,16 @@ extern "C" int mctdhb_step_graph_create(mctdhb_space* sp, mctdhb_sil* s, mctdhb_
         const int rc = hc_tune_tiling(sp->S, nullptr);
         if (rc) return rc;
     }
-    MCTDHB_HIP_CHECK(hipDeviceSynchronize());
-    hipStream_t cs = nullptr;
-    MCTDHB_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-    hipGraph_t graph = nullptr;
-    int rc = kOk;
-    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-        set_error("hipStreamBeginCapture failed");
-        rc = kDeviceError;
-    } else {
-        rc = mctdhb_propagate_step_dev(sp, s, o, use_fft, orb, coef, hob, hint, rho1, rho2, rinv, cs);
-        const hipError_t e = hipStreamEndCapture(cs, &graph);
-        if (!rc && e != hipSuccess) {
-            set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-            rc = kDeviceError;
-        }
-    }
-    hipStreamDestroy(cs);
-    hipGraphExec_t exec = nullptr;
-    if (!rc && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
-        set_error("hipGraphInstantiate failed");
-        rc = kDeviceError;
-    }
+    mctdhb_step_graph* g = new mctdhb_step_graph();
+    const int rc = capture_graph(
+        [&](hipStream_t cs) {
+            return mctdhb_propagate_step_dev(sp, s, o, use_fft, orb, coef, hob, hint, rho1, rho2, rinv, cs);
+        },
+        g);
     if (rc) {
-        if (graph) hipGraphDestroy(graph);
+        delete g;
         return rc;
     }
-    mctdhb_step_graph* g = new mctdhb_step_graph();
-    g->graph = graph;
-    g->exec = exec;
     *out = g;
     return kOk;
 }
@@ -1053,7 +1074,6 @@ struct DevOrbital {
 
 std::mutex g_orb_mu;
 std::unordered_map<const void*, DevOrbital*> g_orb;
-double g_reg_factor = 1e-9;  // DEFAULT_REGULARIZATION_FACTOR (synchronize.h:23)
 
 void free_dev_orbital(DevOrbital* d) {
     if (!d) return;
@@ -1406,4 +1426,361 @@ extern "C" void mctdhb_propagate_step(MCTDHBDataStruct m, Carray orb_works, Carr
     if (M == 1) return;
     propagate_coef_sil(m, coef_works);
     sync_density_matrices(m->multiconfig_space, psi);
+}
+
+// ----------------------------------------------- layer 1: job loop
+//
+// integration_driver (reference driver.c:203-333) with the state kept on the
+// GPU for the whole run: imaginary time replays one captured step graph per
+// step; real time runs the step's pieces eagerly so the trap and g of the
+// new time can be evaluated on the host between the orbital step and the
+// h/g rebuild, as the reference orders them (driver.c:175-186).  The host
+// structs receive the state at monitor, record and end points.
+
+namespace {
+
+Bool g_final_diag = TRUE;          // driver.c:15
+Bool g_auto_convergence = FALSE;   // driver.c:16
+uint8_t g_energy_digits = 11;      // driver.c:17
+double g_eig_residue_tol = 1e-6;   // driver.c:18
+double g_impr_ortho_thr = 1e-8;    // driver.c:19
+
+// composite Simpson rule of calculus.c:8-39 with its summation order
+double2 simps_host(int n, double h, const double2* f) {
+    double sr = 0, si = 0;
+    const int last = n % 2 == 0 ? n - 4 : n - 2;
+    for (int i = 0; i < last; i += 2) {
+        sr += f[i].x + 4 * f[i + 1].x + f[i + 2].x;
+        si += f[i].y + 4 * f[i + 1].y + f[i + 2].y;
+    }
+    sr *= h / 3;
+    si *= h / 3;
+    if (n % 2 == 0) {
+        sr += (f[n - 4].x + 3 * (f[n - 3].x + f[n - 2].x) + f[n - 1].x) * 3 * h / 8;
+        si += (f[n - 4].y + 3 * (f[n - 3].y + f[n - 2].y) + f[n - 1].y) * 3 * h / 8;
+    }
+    return make_double2(sr, si);
+}
+
+// <f, g> = Simpson of conj(f) g (calculus.c:130-142)
+double2 scalar_product_host(int n, double h, const dcomplex* f, const dcomplex* g) {
+    std::vector<double2> t(n);
+    const double* a = (const double*) f;
+    const double* b = (const double*) g;
+    for (int i = 0; i < n; i++)
+        t[i] = make_double2(a[2 * i] * b[2 * i] + a[2 * i + 1] * b[2 * i + 1],
+                            a[2 * i] * b[2 * i + 1] - a[2 * i + 1] * b[2 * i]);
+    return simps_host(n, h, t.data());
+}
+
+}  // namespace
+
+extern "C" double overlap_residual(uint16_t norb, uint16_t grid_size, double dx, Cmatrix orb) {
+    double sum = 0;
+    for (int k = 0; k < norb; k++)
+        for (int l = k + 1; l < norb; l++) {
+            const double2 o = scalar_product_host(grid_size, dx, orb[k], orb[l]);
+            sum += hypot(o.x, o.y);
+        }
+    return sum / norb;
+}
+
+extern "C" double avg_orbitals_norm(uint16_t norb, uint16_t grid_size, double dx, Cmatrix orb) {
+    double sum = 0;
+    std::vector<double2> m(grid_size);
+    for (int k = 0; k < norb; k++) {
+        const double* f = (const double*) orb[k];
+        for (int i = 0; i < grid_size; i++) m[i] = make_double2(f[2 * i] * f[2 * i] + f[2 * i + 1] * f[2 * i + 1], 0);
+        sum += sqrt(simps_host(grid_size, dx, m.data()).x);  // cplx_function_norm (calculus.c:144-155)
+    }
+    return sum / norb;
+}
+
+extern "C" void set_final_diagonalization(Bool shall_diag) { g_final_diag = shall_diag; }
+
+extern "C" void set_autoconvergence_check(Bool must_check) { g_auto_convergence = must_check; }
+
+extern "C" void set_energy_convergence_digits(uint8_t edig) {
+    if (edig < 4) printf("\n\nWARNING: %u digits is too small for energy convergence criteria.\n\n", edig);
+    if (edig > 13) {
+        printf("\n\nERROR: %u digits for convergence is too much due to roundoff.\n\n", edig);
+        exit(EXIT_FAILURE);
+    }
+    g_energy_digits = edig;
+}
+
+extern "C" void set_energy_convergence_eig_residual(double eig_res) {
+    if (eig_res > 0.01)
+        printf("\n\nWARNING: Too large eigenvalue residual to define convergence %.2lf\n\n", eig_res);
+    g_eig_residue_tol = eig_res;
+}
+
+extern "C" void set_overlap_residue_threshold(double over_res) {
+    if (over_res > REALTIME_OVERLAP_RESIDUE_TOL) {
+        printf("\n\nERROR: Overlap residue cannot exceed %.3lf but %.3lf was given\n\n", REALTIME_OVERLAP_RESIDUE_TOL,
+               over_res);
+        exit(EXIT_FAILURE);
+    }
+    g_impr_ortho_thr = over_res;
+}
+
+namespace {
+
+// Krylov size of the closing diagonalisation (driver.c:38-44)
+uint16_t closing_lanczos_iterations(uint32_t dim) {
+    if (dim / 2 > 50) return 50;
+    if (dim < 4) return 0;
+    return (uint16_t) (dim / 2);
+}
+
+// Device state of one run of the job loop.
+struct RunState {
+    MCTDHBDataStruct m = nullptr;
+    mctdhb_space* sp = nullptr;
+    DevOrbital* d = nullptr;
+    mctdhb_sil* sil = nullptr;  // the run's own SIL workspace
+    double2* d_rho1 = nullptr;
+    double2* d_tmp = nullptr;   // 2 M^2 scratch of the monitor
+    double* d_occ = nullptr;    // natural occupations for the monitor
+    int use_fft = 1;
+
+    // the reference's step (driver.c:158-202); with params set, the new
+    // trap and g are evaluated after the orbital step
+    int step(uint32_t curr_step, bool params, hipStream_t st) {
+        OrbitalEquation eq = m->orb_eq;
+        const int M = m->state->norb;
+        int rc = mctdhb_splitstep_dev(d->o, use_fft, d->d_orb, d->d_hob, d->d_hint, d->d_rho2, d->d_rinv, st);
+        if (rc) return rc;
+        if (params) {
+            eq->t = (curr_step + 1) * eq->tstep;
+            sync_equation_params(eq);
+            if ((rc = mctdhb_orbital_set_potential(d->o, eq->pot_grid)) ||
+                (rc = mctdhb_orbital_set_interaction(d->o, eq->g)))
+                return rc;
+        }
+        if (eq->bounds == PERIODIC_BOUNDS && (rc = periodic_bounds(d->o->w, d->d_orb, st))) return rc;
+        if ((rc = mctdhb_orbital_matrices_dev(d->o, d->d_orb, d->d_hob, d->d_hint, st))) return rc;
+        if (M == 1) return kOk;
+        const double2 dt = d->o->w->eq.prop_dt;
+        if ((rc = mctdhb_sil_step_dev(sil, sp->d_coef, d->d_hob, d->d_hint, dt.x, dt.y, d->o->w->imagtime, st)))
+            return rc;
+        return mctdhb_density_sync_dev(sp, sp->d_coef, d_rho1, d->d_rho2, d->d_rinv, nullptr, g_reg_factor,
+                                       MCTDHB_FP64, st);
+    }
+
+    // device -> host structs; the coefficients only when asked
+    void to_host(bool coef) {
+        ManyBodyState psi = m->state;
+        const int M = psi->norb;
+        if (hipDeviceSynchronize() != hipSuccess) fail_stop("integration_driver");
+        orbitals_d2h(psi->orbitals, d->d_orb, M, psi->grid_size);
+        cmatrix_d2h(psi->hob, d->d_hob, M);
+        d2h(psi->hint, d->d_hint, sizeof(dcomplex) * M * M * M * M);
+        if (M > 1) {
+            cmatrix_d2h(psi->ob_denmat, d_rho1, M);
+            d2h(psi->tb_denmat, d->d_rho2, sizeof(dcomplex) * M * M * M * M);
+            cmatrix_d2h(psi->inv_ob_denmat, d->d_rinv, M);
+        }
+        if (coef) d2h(psi->coef, sp->d_coef, sizeof(dcomplex) * psi->space_dim);
+    }
+
+    // one monitor line: [t / tend] E/N, largest and smallest natural
+    // occupation over N, then the eigenvalue residue (imaginary time) or the
+    // overlap residue and the norms (real time); returns E/N and the residue
+    double monitor(double* eig_res_out) {
+        ManyBodyState psi = m->state;
+        OrbitalEquation eq = m->orb_eq;
+        const int M = psi->norb, N = psi->npar;
+        to_host(m->integ_type == REALTIME);
+        double occ[MAX_ORBITALS] = {(double) N};
+        if (M > 1) {  // eigenvalues of rho1 from a copy (no regularisation)
+            if (hipMemcpy(d_tmp, d_rho1, sizeof(double2) * M * M, hipMemcpyDeviceToDevice) != hipSuccess ||
+                density_sync_small(M, d_tmp, d_tmp + M * M, d_occ, 0.0, nullptr))
+                fail_stop("natural occupations");
+            d2h(occ, d_occ, sizeof(double) * M);
+        }
+        const dcomplex e = total_energy(psi);
+        double ev[2];
+        memcpy(ev, &e, sizeof ev);
+        char fmt[32];
+        snprintf(fmt, sizeof fmt, " %%%d.%dE", monitor_energy_digits + 7, monitor_energy_digits);
+        printf("\n[%7.3lf / %.1lf]", eq->t, eq->tend);
+        printf(fmt, ev[0] / N);
+        printf("%6.2lf%10.6lf", occ[M - 1] / N, occ[0] / N);
+        double res = 0;
+        if (m->integ_type == IMAGTIME) {
+            if (M > 1 && mctdhb_eig_residual_dev(sp, sp->d_coef, d->d_hob, d->d_hint, ev[0], &res, nullptr))
+                fail_stop("eig_residual");
+            printf("%11.7lf", res);
+        } else {
+            double c2 = 0;
+            const double* c = (const double*) psi->coef;
+            for (uint32_t i = 0; i < 2 * psi->space_dim; i++) c2 += c[i] * c[i];
+            printf("%10.2E%11.7lf%11.7lf", overlap_residual(M, psi->grid_size, eq->dx, psi->orbitals),
+                   avg_orbitals_norm(M, psi->grid_size, eq->dx, psi->orbitals), sqrt(c2));
+        }
+        fflush(stdout);
+        if (eig_res_out) *eig_res_out = res;
+        return ev[0] / N;
+    }
+};
+
+}  // namespace
+
+extern "C" void integration_driver(MCTDHBDataStruct m, uint16_t rec_nsteps, char prefix[], double tend,
+                                   uint8_t monitor_rate) {
+    ManyBodyState psi = m->state;
+    OrbitalEquation eq = m->orb_eq;
+    const int M = psi->norb, grid = psi->grid_size;
+    const bool imag = m->integ_type == IMAGTIME;
+    if (m->orb_integ_method != SPLITSTEP) {
+        printf("\n\nERROR: only the SPLITSTEP orbital integrator is provided\n\n");
+        exit(EXIT_FAILURE);
+    }
+    if (m->coef_integ_method != LANCZOS && M > 1) {
+        printf("\n\nERROR: only the LANCZOS coefficient integrator is provided\n\n");
+        exit(EXIT_FAILURE);
+    }
+    eq->tend = tend;
+    if (rec_nsteps == 0) rec_nsteps = 1;
+    if (monitor_rate == 0) monitor_rate = 1;
+
+    RunState R;
+    R.m = m;
+    R.sp = ensure_dev(m->multiconfig_space);
+    R.d = dev_orbital(eq);
+    R.use_fft = m->orb_der_method != FINITEDIFF;  // orbital_integration.c:375-381
+    if (mctdhb_orbital_set_rk_order(R.d->o, m->rk_order)) fail_stop("Runge-Kutta order");
+    const size_t n = psi->space_dim;
+    stage_alloc(&R.sp->d_coef, n);
+    stage_alloc(&R.d_rho1, (size_t) M * M);
+    stage_alloc(&R.d_tmp, 2 * (size_t) M * M);
+    if (hipMalloc((void**) &R.d_occ, sizeof(double) * M) != hipSuccess) fail_stop("integration_driver");
+    orbitals_h2d(R.d->d_orb, psi->orbitals, M, grid);
+    cmatrix_h2d(R.d->d_hob, psi->hob, M);
+    h2d(R.d->d_hint, psi->hint, sizeof(dcomplex) * M * M * M * M);
+    h2d(R.sp->d_coef, psi->coef, sizeof(dcomplex) * n);
+    if (M > 1) {
+        cmatrix_h2d(R.d_rho1, psi->ob_denmat, M);
+        h2d(R.d->d_rho2, psi->tb_denmat, sizeof(dcomplex) * M * M * M * M);
+        cmatrix_h2d(R.d->d_rinv, psi->inv_ob_denmat, M);
+        const int lm = m->coef_workspace->lan_work->iter;
+        if (mctdhb_sil_create(R.sp, lm, MCTDHB_FP64, MCTDHB_INDEX_FLY, &R.sil)) fail_stop("Lanczos workspace");
+        if (!R.sp->dm_work && hipMalloc(&R.sp->dm_work, dm_workspace_bytes(R.sp->S)) != hipSuccess)
+            fail_stop("density workspace");
+    }
+
+    const dcomplex e0 = total_energy(psi);
+    double prev_e;
+    memcpy(&prev_e, &e0, sizeof prev_e);  // real part
+    prev_e /= psi->npar;
+    printf("\nColumns: [time] E/N, largest and smallest natural occupation / N, %s\n",
+           imag ? "eigenvalue residue" : "overlap residue, orbital norm, coefficient norm");
+    R.monitor(nullptr);
+    // initial conditions (driver.c:236-251)
+    if (imag) {
+        std::string init = std::string(prefix) + "_init";
+        record_raw_state((char*) init.c_str(), psi);
+    } else {
+        toggle_new_append_files();
+        append_processed_state(prefix, psi);
+        append_timestep_potential(prefix, eq);
+        toggle_new_append_files();
+    }
+
+    mctdhb_step_graph graph;
+    bool have_graph = false;
+    if (imag && M <= 7) {  // larger orbital counts use a density path that synchronises
+        const int rc = capture_graph([&](hipStream_t cs) { return R.step(0, false, cs); }, &graph);
+        if (rc) fail_stop("step graph");
+        have_graph = true;
+    }
+
+    const auto t0 = std::chrono::steady_clock::now();
+    uint32_t steps = 0;
+    double t = 0;
+    while (t < tend) {
+        if (have_graph) {
+            if (hipGraphLaunch(graph.exec, nullptr) != hipSuccess) fail_stop("step graph launch");
+            eq->t = (steps + 1) * eq->tstep;
+        } else {
+            if (R.step(steps, !imag, nullptr)) fail_stop("mctdhb_propagate_step");
+            if (imag) eq->t = (steps + 1) * eq->tstep;
+        }
+        steps++;
+        t = steps * eq->tstep;
+        if (!imag && steps % rec_nsteps == 0) {
+            R.to_host(false);
+            append_processed_state(prefix, psi);
+            append_timestep_potential(prefix, eq);
+        }
+        if (steps % monitor_rate == 0) {
+            double res = 0;
+            const double e = R.monitor(&res);
+            if (imag && g_auto_convergence) {  // driver.c:46-67
+                const double etol = pow(0.1, (double) g_energy_digits) * prev_e;
+                if (fabs(e - prev_e) < etol && res < g_eig_residue_tol) break;
+                prev_e = e;
+            }
+            if (!imag) {  // driver.c:69-94
+                const double o = overlap_residual(M, grid, eq->dx, psi->orbitals);
+                if (o > g_impr_ortho_thr && !m->orb_workspace->impr_ortho) {
+                    printf("\n\n== Start improving orthogonality ==\n\n");
+                    m->orb_workspace->impr_ortho = TRUE;
+                }
+                if (o > REALTIME_OVERLAP_RESIDUE_TOL) {
+                    printf("\n\nOverlap residue fell below critical value %.3lf. Aborting\n\n",
+                           REALTIME_OVERLAP_RESIDUE_TOL);
+                    exit(EXIT_FAILURE);
+                }
+            }
+        }
+    }
+    if (hipDeviceSynchronize() != hipSuccess) fail_stop("integration_driver");
+    const double used = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (have_graph) {
+        hipGraphExecDestroy(graph.exec);
+        hipGraphDestroy(graph.graph);
+    }
+    const int secs = (int) used;
+    printf("\n\n%u Total steps propagated in %.0lf(s): %d hour(s) %d minute(s)", steps, used, secs / 3600,
+           (secs % 3600) / 60);
+
+    if (imag) {  // driver.c:294-323
+        if (g_auto_convergence && t < tend)
+            printf("\nThe result converged before expected with t = %.1lf of %.1lf total propagation time", t,
+                   tend);
+        const uint16_t lan_it = closing_lanczos_iterations((uint32_t) n);
+        if (g_final_diag && t >= tend && lan_it > 0 && M > 1) {
+            mctdhb_sil* big = nullptr;
+            if (mctdhb_sil_create(R.sp, lan_it, MCTDHB_FP64, MCTDHB_INDEX_FLY, &big)) fail_stop("Lanczos workspace");
+            double e = 0;
+            if (mctdhb_lowest_state_dev(big, R.sp->d_coef, R.d->d_hob, R.d->d_hint, &e, nullptr)) {
+                printf("\n\nERROR IN DIAGONALIZATION\n\n");
+                exit(EXIT_FAILURE);
+            }
+            mctdhb_sil_destroy(big);
+            printf("\n\n== Additional coefficients diagonalization ==\n");
+            R.monitor(nullptr);
+        }
+        R.to_host(true);
+        char fname[STR_BUFF_SIZE];
+        set_output_fname(prefix, ONE_BODY_POTENTIAL_REC, fname);
+        FILE* f = fopen(fname, "w");
+        if (!f) {
+            printf("\n\nIOERROR: impossible to open file %s with mode w\n\n", fname);
+            exit(EXIT_FAILURE);
+        }
+        for (int i = 0; i < grid; i++) fprintf(f, "%.14E\n", eq->pot_grid[i]);
+        fclose(f);
+        record_raw_state(prefix, psi);
+    } else {
+        R.to_host(true);
+        record_time_interaction(prefix, eq, rec_nsteps);
+        record_time_array(prefix, tend, eq->tstep, rec_nsteps);
+    }
+    if (R.sil) mctdhb_sil_destroy(R.sil);
+    hipFree(R.d_rho1);
+    hipFree(R.d_tmp);
+    hipFree(R.d_occ);
 }

@@ -129,6 +129,79 @@ void propagate_coef_sil(MCTDHBDataStruct mctdhb, Carray cnext);                 
 double   eig_residual(MultiConfiguration multiconf, Carray c, Cmatrix hob, Carray hint,
                       double e);                                                 /* integrator_monitor.c:9-28 */
 dcomplex total_energy(ManyBodyState psi);                                        /* integrator_monitor.c:55-88 */
+double   overlap_residual(uint16_t norb, uint16_t grid_size, double dx, Cmatrix orb); /* integrator_monitor.c:30-42 */
+double   avg_orbitals_norm(uint16_t norb, uint16_t grid_size, double dx, Cmatrix orb); /* integrator_monitor.c:44-53 */
+
+/* include/integrator/driver.h: the job loop and its settings.  The loop
+ * keeps the state on the GPU (imaginary time replays one captured step
+ * graph per step) and copies it to the host structs at monitor, record and
+ * end points.  Printing is a compact monitor line, not the reference's
+ * column layout. */
+#define REALTIME_OVERLAP_RESIDUE_TOL 0.001                                       /* driver.h:7 */
+void set_final_diagonalization(Bool shall_diag);                                 /* driver.c:97-101 */
+void set_autoconvergence_check(Bool must_check);                                 /* driver.c:103-107 */
+void set_energy_convergence_digits(uint8_t edig);                                /* driver.c:109-128 */
+void set_energy_convergence_eig_residual(double eig_res);                        /* driver.c:130-141 */
+void set_overlap_residue_threshold(double over_res);                             /* driver.c:143-157 */
+void integration_driver(MCTDHBDataStruct mctdhb, uint16_t rec_nsteps, char prefix[], double tend,
+                        uint8_t monitor_rate);                                   /* driver.c:203-333 */
+
+/* include/function_tools/builtin_potential.h, builtin_time_parameter.h */
+single_particle_pot get_builtin_pot(char pot_name[]);                            /* builtin_potential.c:215-260 */
+void potfunc_harmonic(double t, uint16_t npts, Rarray x, void* params, Rarray V); /* builtin_potential.c:36-50 */
+void potfunc_doublewell(double t, uint16_t npts, Rarray x, void* params, Rarray V); /* :51-69 */
+void potfunc_harmonicgauss(double t, uint16_t npts, Rarray x, void* params, Rarray V); /* :70-94 */
+void potfunc_barrier(double t, uint16_t npts, Rarray x, void* params, Rarray V);  /* :95-121 */
+void potfunc_time_trapezoid_barrier(double t, uint16_t npts, Rarray x, void* params, Rarray V); /* :123-143 */
+void potfunc_opticallattice(double t, uint16_t npts, Rarray x, void* params, Rarray V); /* :145-161 */
+void potfunc_time_trapezoid_opticallattice(double t, uint16_t npts, Rarray x, void* params,
+                                           Rarray V);                            /* :163-180 */
+void potfunc_step(double t, uint16_t npts, Rarray x, void* params, Rarray V);    /* :182-196 */
+void potfunc_square_well(double t, uint16_t npts, Rarray x, void* params, Rarray V); /* :198-213 */
+double timeparam_linear_ramp(double t, void* params);                           /* builtin_time_parameter.c:6-15 */
+time_dependent_parameter get_builtin_param_func(char funcname[]);               /* builtin_time_parameter.c:17-30 */
+
+/* include/assistant/dataio.h: text formats.  Complex numbers are written
+ * as " (%.16E%+.16Ej)" tokens (numpy.loadtxt reads them as complex128) and
+ * read with the scanf formats below; '#' starts a comment line. */
+#define MIN_PARAMS_INLINE 21                                                     /* dataio.h:32 */
+#define PARAMS_FNAME_SUFFIX "_mctdhb_parameters.dat"                             /* dataio.h:34 */
+typedef enum { COMMON_INP, MULTIPLE_INP, LAST_JOB_OUT } JobsInputHandle;         /* dataio.h:70-75 */
+typedef enum {
+    ORBITALS_REC,
+    COEFFICIENTS_REC,
+    ONE_BODY_MATRIX_REC,
+    TWO_BODY_MATRIX_REC,
+    PARAMETERS_REC,
+    ONE_BODY_POTENTIAL_REC
+} RecordDataType;                                                                /* dataio.h:78-86 */
+extern char    orb_cplx_read_fmt[STR_BUFF_SIZE];                                 /* dataio.c:14 */
+extern char    coef_cplx_read_fmt[STR_BUFF_SIZE];                                /* dataio.c:15 */
+extern char    inp_dirname[STR_BUFF_SIZE];                                       /* dataio.c:16 */
+extern char    out_dirname[STR_BUFF_SIZE];                                       /* dataio.c:17 */
+extern char    integrator_desc_fname[STR_BUFF_SIZE];                             /* dataio.c:18 */
+extern uint8_t monitor_energy_digits;                                            /* dataio.c:20 */
+extern Bool    new_empty_append_files;                                           /* dataio.c:29 */
+void toggle_new_append_files(void);                                              /* dataio.c:41-50 */
+void set_orbitals_from_file(char fname[], ManyBodyState psi);                    /* dataio.c:52-66 */
+void set_coef_from_file(char fname[], ManyBodyState psi);                        /* dataio.c:68-72 */
+MCTDHBDataStruct get_mctdhb_from_files(char par_fname[], char integ_fname[], uint32_t line,
+                                       single_particle_pot custom_pot_fun,
+                                       time_dependent_parameter custom_inter_fun, void* custom_pot_params,
+                                       void* custom_inter_params);               /* dataio.c:74-293 */
+MCTDHBDataStruct full_setup_mctdhb_current_dir(char fprefix[], uint32_t job_num, JobsInputHandle which_inp,
+                                               single_particle_pot custom_pot_fun,
+                                               time_dependent_parameter custom_inter_fun,
+                                               void* custom_pot_params,
+                                               void* custom_inter_params);       /* dataio.c:295-378 */
+uint32_t auto_number_of_jobs(char prefix[]);                                     /* dataio.c:519-527 */
+void set_output_fname(char prefix[], RecordDataType id, char* fname);            /* dataio.c:529-558 */
+void append_processed_state(char prefix[], ManyBodyState psi);                   /* dataio.c:705-746 */
+void record_raw_state(char prefix[], ManyBodyState psi);                         /* dataio.c:747-764 */
+void append_timestep_potential(char prefix[], OrbitalEquation eq_desc);          /* dataio.c:765-781 */
+void record_time_interaction(char prefix[], OrbitalEquation eq_desc, uint16_t recn); /* dataio.c:782-813 */
+void record_time_array(char prefix[], double tend, double tstep, uint16_t recn); /* dataio.c:814-835 */
+void append_mctdhb_parameters(char prefix[], MCTDHBDataStruct mctdhb);           /* dataio.c:836-866 */
 
 /* ================================================= 2. device-resident */
 
@@ -263,7 +336,8 @@ int mctdhb_density_sync_dev(mctdhb_space* space, const void* coef, void* rho1, v
                             void* eigvals, double reg_factor, int prec, void* stream);
 /* One MCTDHB time step for a time-independent trap and interaction
  * (driver.c:159-202): orbital split step, h/g rebuild, SIL coefficient
- * step, density sync.  All arrays device-resident. */
+ * step, density sync with the regularisation factor of
+ * set_regulatization_factor (default 1e-9).  All arrays device-resident. */
 int mctdhb_propagate_step_dev(mctdhb_space* space, mctdhb_sil* sil, mctdhb_orbital* w, int use_fft, void* orb,
                               void* coef, void* hob, void* hint, void* rho1, void* rho2, void* rinv,
                               void* stream);
