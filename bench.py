@@ -203,6 +203,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-full-step", action="store_true",
                     help="skip the whole-step (orbitals + SIL + densities) eager/graph timing")
+    ap.add_argument("--full-step-only", action="store_true",
+                    help="only the whole-step timing (prints its JSON object)")
     ap.add_argument("--probe-hc", action="store_true",
                     help="only the mixed H.C kernel (5 launches) and a plain copy kernel of known size, "
                          "for hardware-counter runs")
@@ -215,6 +217,10 @@ def main():
     import torch
     import mctdhb_amd as amd
     from mctdhb_amd import inputs
+
+    if args.full_step_only:
+        print(json.dumps(full_step_bench(args, torch.cuda.current_stream())), flush=True)
+        return
 
     N, M, lm, dt = args.npar, args.norb, args.lanczos, args.dt
     dev = "cuda"
