@@ -437,6 +437,48 @@ extern "C" int mctdhb_splitstep_dev(mctdhb_orbital* o, int use_fft, void* orb, v
                      (const double2*) rinv, (hipStream_t) stream);
 }
 
+extern "C" int mctdhb_orbital_set_integrator(mctdhb_orbital* o, int orb_method, int robust) {
+    if (!o) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    if (orb_method != FULLSTEP_RUNGEKUTTA && orb_method != SPLITSTEP) {
+        set_error("orbital integrator must be FULLSTEP_RUNGEKUTTA (30) or SPLITSTEP (31)");
+        return kInvalidArgument;
+    }
+    o->w->fullstep = orb_method == FULLSTEP_RUNGEKUTTA;
+    o->w->robust = robust != 0;
+    return kOk;
+}
+
+extern "C" int mctdhb_fullstep_dev(mctdhb_orbital* o, int use_fd, int robust, void* orb, const void* rho2,
+                                   const void* rinv, void* stream) {
+    ORB_CHECK(!orb || !rho2 || !rinv);
+    return fullstep(o->w, use_fd, robust, (double2*) orb, (const double2*) rho2, (const double2*) rinv,
+                    (hipStream_t) stream);
+}
+
+extern "C" int mctdhb_fullstep_rhs_dev(mctdhb_orbital* o, int use_fd, int robust, const void* orb, const void* rho2,
+                                       const void* rinv, void* der, void* stream) {
+    ORB_CHECK(!orb || !rho2 || !rinv || !der);
+    return fullstep_rhs(o->w, use_fd, robust, (const double2*) orb, (const double2*) rho2, (const double2*) rinv,
+                        (double2*) der, (hipStream_t) stream);
+}
+
+extern "C" int mctdhb_projector_dev(mctdhb_orbital* o, int robust, const void* orb, const void* hact, void* proj,
+                                    void* stream) {
+    ORB_CHECK(!orb || !hact || !proj);
+    return projector(o->w, robust, (const double2*) orb, (const double2*) hact, (double2*) proj,
+                     (hipStream_t) stream);
+}
+
+// The orbital part of a whole step: the integrator set on the workspace.
+static int orbital_step(OrbWork* w, int use_fft, double2* orb, double2* hob, double2* hint, const double2* rho2,
+                        const double2* rinv, hipStream_t st) {
+    if (w->fullstep) return fullstep(w, !use_fft, w->robust, orb, rho2, rinv, st);
+    return splitstep(w, use_fft, orb, hob, hint, rho2, rinv, st);
+}
+
 extern "C" int mctdhb_density_sync_dev(mctdhb_space* sp, const void* coef, void* rho1, void* rho2, void* rinv,
                                        void* eigvals, double reg_factor, int prec, void* stream) {
     if (!sp || !coef || !rho1 || !rho2 || !rinv) {
@@ -458,8 +500,8 @@ extern "C" int mctdhb_propagate_step_dev(mctdhb_space* sp, mctdhb_sil* s, mctdhb
     }
     hipStream_t st = (hipStream_t) stream;
     OrbWork* w = o->w;
-    int rc = splitstep(w, use_fft, (double2*) orb, (double2*) hob, (double2*) hint, (const double2*) rho2,
-                       (const double2*) rinv, st);
+    int rc = orbital_step(w, use_fft, (double2*) orb, (double2*) hob, (double2*) hint, (const double2*) rho2,
+                          (const double2*) rinv, st);
     if (rc) return rc;
     // driver.c:179-186: periodic bounds, then h and g from the new orbitals
     if (w->eq.bounds == kPeriodicBounds && (rc = periodic_bounds(w, (double2*) orb, st))) return rc;
@@ -1368,24 +1410,46 @@ extern "C" void sync_equation_params(OrbitalEquation eq) {
     eq->pot_func(eq->t, eq->grid_size, eq->grid_pts, eq->pot_extra_args, eq->pot_grid);
 }
 
-// The orbital split step on the device from the host state; leaves the
-// orbitals, h and g of the new orbitals in the device staging of d.
-static void splitstep_dev_from_host(MCTDHBDataStruct m, DevOrbital* d) {
-    ManyBodyState psi = m->state;
-    const int M = psi->norb;
-    if (m->orb_integ_method != SPLITSTEP) {
-        printf("\n\nERROR: only the SPLITSTEP orbital integrator is provided\n\n");
+// The integrator settings of m on the device workspace of d.  Finite
+// differences select Crank-Nicolson (split step) or the finite-difference
+// operator (full step), the other derivative methods the FFT
+// (orbital_integration.c:375-381, :142-175); DVR is not provided.
+static int configure_orbital(MCTDHBDataStruct m, DevOrbital* d) {
+    if (m->orb_integ_method != SPLITSTEP && m->orb_integ_method != FULLSTEP_RUNGEKUTTA) {
+        printf("\n\nERROR: unknown orbital integrator %d\n\n", (int) m->orb_integ_method);
+        exit(EXIT_FAILURE);
+    }
+    if (m->orb_integ_method == FULLSTEP_RUNGEKUTTA && m->orb_der_method == DVR) {
+        printf("\n\nERROR: DVR derivatives are not provided; use FINITEDIFF (302) or SPECTRAL (301)\n\n");
         exit(EXIT_FAILURE);
     }
     if (mctdhb_orbital_set_rk_order(d->o, m->rk_order)) fail_stop("Runge-Kutta order");
+    if (mctdhb_orbital_set_integrator(d->o, m->orb_integ_method, m->orb_workspace->impr_ortho))
+        fail_stop("orbital integrator");
+    return m->orb_der_method != FINITEDIFF;
+}
+
+// The orbital step of m on the device from the host state; leaves the
+// orbitals (and, for the split step, h and g of the intermediate
+// orbitals) in the device staging of d.
+static void orbital_step_from_host(MCTDHBDataStruct m, DevOrbital* d) {
+    ManyBodyState psi = m->state;
+    const int M = psi->norb;
+    const int use_fft = configure_orbital(m, d);
     orbitals_h2d(d->d_orb, psi->orbitals, M, psi->grid_size);
     h2d(d->d_rho2, psi->tb_denmat, sizeof(dcomplex) * M * M * M * M);
     cmatrix_h2d(d->d_rinv, psi->inv_ob_denmat, M);
-    // orbital_integration.c:375-381: finite differences use Crank-Nicolson,
-    // the other derivative methods the FFT
-    const int use_fft = m->orb_der_method != FINITEDIFF;
-    if (mctdhb_splitstep_dev(d->o, use_fft, d->d_orb, d->d_hob, d->d_hint, d->d_rho2, d->d_rinv, nullptr))
-        fail_stop("propagate_splitstep_orb");
+    if (orbital_step(d->o->w, use_fft, d->d_orb, d->d_hob, d->d_hint, d->d_rho2, d->d_rinv, nullptr))
+        fail_stop("orbital step");
+}
+
+// The split step alone (orbital_integration.c:351-434).
+static void splitstep_dev_from_host(MCTDHBDataStruct m, DevOrbital* d) {
+    if (m->orb_integ_method != SPLITSTEP) {
+        printf("\n\nERROR: propagate_splitstep_orb needs the SPLITSTEP orbital integrator\n\n");
+        exit(EXIT_FAILURE);
+    }
+    orbital_step_from_host(m, d);
 }
 
 extern "C" void propagate_splitstep_orb(MCTDHBDataStruct m, Carray orb_next) {
@@ -1400,6 +1464,102 @@ extern "C" void propagate_splitstep_orb(MCTDHBDataStruct m, Carray orb_next) {
     d2h(orb_next, d->d_orb, sizeof(dcomplex) * M * psi->grid_size);
 }
 
+namespace {
+
+// composite Simpson rule of calculus.c:8-39 with its summation order
+double2 simps_host(int n, double h, const double2* f) {
+    double sr = 0, si = 0;
+    const int last = n % 2 == 0 ? n - 4 : n - 2;
+    for (int i = 0; i < last; i += 2) {
+        sr += f[i].x + 4 * f[i + 1].x + f[i + 2].x;
+        si += f[i].y + 4 * f[i + 1].y + f[i + 2].y;
+    }
+    sr *= h / 3;
+    si *= h / 3;
+    if (n % 2 == 0) {
+        sr += (f[n - 4].x + 3 * (f[n - 3].x + f[n - 2].x) + f[n - 1].x) * 3 * h / 8;
+        si += (f[n - 4].y + 3 * (f[n - 3].y + f[n - 2].y) + f[n - 1].y) * 3 * h / 8;
+    }
+    return make_double2(sr, si);
+}
+
+// <f, g> = Simpson of conj(f) g (calculus.c:130-142)
+double2 scalar_product_host(int n, double h, const dcomplex* f, const dcomplex* g) {
+    std::vector<double2> t(n);
+    const double* a = (const double*) f;
+    const double* b = (const double*) g;
+    for (int i = 0; i < n; i++)
+        t[i] = make_double2(a[2 * i] * b[2 * i] + a[2 * i + 1] * b[2 * i + 1],
+                            a[2 * i] * b[2 * i + 1] - a[2 * i + 1] * b[2 * i]);
+    return simps_host(n, h, t.data());
+}
+
+}  // namespace
+
+extern "C" void propagate_fullstep_orb_rk(MCTDHBDataStruct m, Carray orb_next) {
+    if (m->orb_integ_method != FULLSTEP_RUNGEKUTTA) {
+        printf("\n\nERROR: propagate_fullstep_orb_rk needs the FULLSTEP_RUNGEKUTTA orbital integrator\n\n");
+        exit(EXIT_FAILURE);
+    }
+    DevOrbital* d = dev_orbital(m->orb_eq);
+    orbital_step_from_host(m, d);
+    ManyBodyState psi = m->state;
+    orbitals_d2h(psi->orbitals, d->d_orb, psi->norb, psi->grid_size);
+    d2h(orb_next, d->d_orb, sizeof(dcomplex) * psi->norb * psi->grid_size);
+}
+
+static void projector_from_host(OrbitalEquation eq, uint16_t norb, Cmatrix orb, Cmatrix hact, Cmatrix project,
+                                int robust) {
+    DevOrbital* d = dev_orbital(eq);
+    const int grid = eq->grid_size;
+    double2* d_h = nullptr;
+    stage_alloc(&d_h, (size_t) norb * grid);
+    orbitals_h2d(d->d_orb, orb, norb, grid);
+    orbitals_h2d(d_h, hact, norb, grid);
+    if (mctdhb_projector_dev(d->o, robust, d->d_orb, d_h, d_h, nullptr)) fail_stop("projector");
+    orbitals_d2h(project, d_h, norb, grid);
+    hipFree(d_h);
+}
+
+extern "C" void robust_multiorb_projector(OrbitalEquation eq, uint16_t norb, Cmatrix orb, Cmatrix hact,
+                                          Cmatrix project) {
+    projector_from_host(eq, norb, orb, hact, project, 1);
+}
+
+extern "C" void simple_multiorb_projector(OrbitalEquation eq, uint16_t norb, Cmatrix orb, Cmatrix hact,
+                                          Cmatrix project) {
+    projector_from_host(eq, norb, orb, hact, project, 0);
+}
+
+// Writes exactly the entries the reference writes (orbital_matrices.c:7-20):
+// the j loop starts at 2, so overlap[0][1] and overlap[1][0] keep what the
+// caller's memory held.
+extern "C" void set_overlap_matrix(uint16_t norb, uint16_t grid_size, double dx, Cmatrix orb, Cmatrix overlap) {
+    auto put = [](dcomplex* dst, double2 v) { memcpy(dst, &v, sizeof v); };
+    for (int i = 0; i < norb; i++) {
+        put(&overlap[i][i], scalar_product_host(grid_size, dx, orb[i], orb[i]));
+        for (int j = 2; j < norb; j++) {
+            const double2 v = scalar_product_host(grid_size, dx, orb[i], orb[j]);
+            put(&overlap[i][j], v);
+            put(&overlap[j][i], make_double2(v.x, -v.y));
+        }
+    }
+}
+
+extern "C" void set_orbital_hob(OrbitalEquation eq, uint16_t norb, Cmatrix orb, Cmatrix hob) {
+    DevOrbital* d = dev_orbital(eq);
+    orbitals_h2d(d->d_orb, orb, norb, eq->grid_size);
+    if (mctdhb_orbital_matrices_dev(d->o, d->d_orb, d->d_hob, d->d_hint, nullptr)) fail_stop("set_orbital_hob");
+    cmatrix_d2h(hob, d->d_hob, norb);
+}
+
+extern "C" void set_orbital_hint(OrbitalEquation eq, uint16_t norb, Cmatrix orb, Carray hint) {
+    DevOrbital* d = dev_orbital(eq);
+    orbitals_h2d(d->d_orb, orb, norb, eq->grid_size);
+    if (mctdhb_orbital_matrices_dev(d->o, d->d_orb, d->d_hob, d->d_hint, nullptr)) fail_stop("set_orbital_hint");
+    d2h(hint, d->d_hint, sizeof(dcomplex) * norb * norb * norb * norb);
+}
+
 extern "C" void mctdhb_propagate_step(MCTDHBDataStruct m, Carray orb_works, Carray coef_works, uint32_t curr_step) {
     ManyBodyState psi = m->state;
     OrbitalEquation eq = m->orb_eq;
@@ -1409,7 +1569,7 @@ extern "C" void mctdhb_propagate_step(MCTDHBDataStruct m, Carray orb_works, Carr
         exit(EXIT_FAILURE);
     }
     DevOrbital* d = dev_orbital(eq);
-    splitstep_dev_from_host(m, d);
+    orbital_step_from_host(m, d);
     // driver.c:176-186: new time, equation parameters, periodic bounds, h and g
     eq->t = (curr_step + 1) * eq->tstep;
     if (m->integ_type == REALTIME) {
@@ -1444,34 +1604,6 @@ Bool g_auto_convergence = FALSE;   // driver.c:16
 uint8_t g_energy_digits = 11;      // driver.c:17
 double g_eig_residue_tol = 1e-6;   // driver.c:18
 double g_impr_ortho_thr = 1e-8;    // driver.c:19
-
-// composite Simpson rule of calculus.c:8-39 with its summation order
-double2 simps_host(int n, double h, const double2* f) {
-    double sr = 0, si = 0;
-    const int last = n % 2 == 0 ? n - 4 : n - 2;
-    for (int i = 0; i < last; i += 2) {
-        sr += f[i].x + 4 * f[i + 1].x + f[i + 2].x;
-        si += f[i].y + 4 * f[i + 1].y + f[i + 2].y;
-    }
-    sr *= h / 3;
-    si *= h / 3;
-    if (n % 2 == 0) {
-        sr += (f[n - 4].x + 3 * (f[n - 3].x + f[n - 2].x) + f[n - 1].x) * 3 * h / 8;
-        si += (f[n - 4].y + 3 * (f[n - 3].y + f[n - 2].y) + f[n - 1].y) * 3 * h / 8;
-    }
-    return make_double2(sr, si);
-}
-
-// <f, g> = Simpson of conj(f) g (calculus.c:130-142)
-double2 scalar_product_host(int n, double h, const dcomplex* f, const dcomplex* g) {
-    std::vector<double2> t(n);
-    const double* a = (const double*) f;
-    const double* b = (const double*) g;
-    for (int i = 0; i < n; i++)
-        t[i] = make_double2(a[2 * i] * b[2 * i] + a[2 * i + 1] * b[2 * i + 1],
-                            a[2 * i] * b[2 * i + 1] - a[2 * i + 1] * b[2 * i]);
-    return simps_host(n, h, t.data());
-}
 
 }  // namespace
 
@@ -1549,7 +1681,7 @@ struct RunState {
     int step(uint32_t curr_step, bool params, hipStream_t st) {
         OrbitalEquation eq = m->orb_eq;
         const int M = m->state->norb;
-        int rc = mctdhb_splitstep_dev(d->o, use_fft, d->d_orb, d->d_hob, d->d_hint, d->d_rho2, d->d_rinv, st);
+        int rc = orbital_step(d->o->w, use_fft, d->d_orb, d->d_hob, d->d_hint, d->d_rho2, d->d_rinv, st);
         if (rc) return rc;
         if (params) {
             eq->t = (curr_step + 1) * eq->tstep;
@@ -1633,10 +1765,6 @@ extern "C" void integration_driver(MCTDHBDataStruct m, uint16_t rec_nsteps, char
     OrbitalEquation eq = m->orb_eq;
     const int M = psi->norb, grid = psi->grid_size;
     const bool imag = m->integ_type == IMAGTIME;
-    if (m->orb_integ_method != SPLITSTEP) {
-        printf("\n\nERROR: only the SPLITSTEP orbital integrator is provided\n\n");
-        exit(EXIT_FAILURE);
-    }
     if (m->coef_integ_method != LANCZOS && M > 1) {
         printf("\n\nERROR: only the LANCZOS coefficient integrator is provided\n\n");
         exit(EXIT_FAILURE);
@@ -1649,8 +1777,7 @@ extern "C" void integration_driver(MCTDHBDataStruct m, uint16_t rec_nsteps, char
     R.m = m;
     R.sp = ensure_dev(m->multiconfig_space);
     R.d = dev_orbital(eq);
-    R.use_fft = m->orb_der_method != FINITEDIFF;  // orbital_integration.c:375-381
-    if (mctdhb_orbital_set_rk_order(R.d->o, m->rk_order)) fail_stop("Runge-Kutta order");
+    R.use_fft = configure_orbital(m, R.d);
     const size_t n = psi->space_dim;
     stage_alloc(&R.sp->d_coef, n);
     stage_alloc(&R.d_rho1, (size_t) M * M);
@@ -1726,7 +1853,8 @@ extern "C" void integration_driver(MCTDHBDataStruct m, uint16_t rec_nsteps, char
                 const double o = overlap_residual(M, grid, eq->dx, psi->orbitals);
                 if (o > g_impr_ortho_thr && !m->orb_workspace->impr_ortho) {
                     printf("\n\n== Start improving orthogonality ==\n\n");
-                    m->orb_workspace->impr_ortho = TRUE;
+                    m->orb_workspace->impr_ortho = TRUE;  // the full-equation step's robust projector
+                    if (mctdhb_orbital_set_integrator(R.d->o, m->orb_integ_method, 1)) fail_stop("projector");
                 }
                 if (o > REALTIME_OVERLAP_RESIDUE_TOL) {
                     printf("\n\nOverlap residue fell below critical value %.3lf. Aborting\n\n",

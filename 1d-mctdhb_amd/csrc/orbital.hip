@@ -414,7 +414,8 @@ __global__ __launch_bounds__(256) void k_nl_coef(int M, const double2* __restric
                                                  const double2* __restrict__ hint, const double2* __restrict__ rho,
                                                  const double2* __restrict__ rinv, double2* __restrict__ out) {
     const int mm = M * M, mmm = mm * M, P = M * (M + 1) / 2, nA = M * P * M;
-    for (int e = threadIdx.x; e < nA + mm; e += blockDim.x) {
+    // without hob (the full-equation step) only the A table is needed
+    for (int e = threadIdx.x; e < nA + (hob ? mm : 0); e += blockDim.x) {
         if (e < nA) {
             const int s = e % M, ql = (e / M) % P, k = e / (M * P);
             int q = 0, rem = ql;
@@ -700,6 +701,335 @@ int periodic_bounds(OrbWork* w, double2* orb, hipStream_t st) {
     return kOk;
 }
 
+// --------------------------------------- full-equation Runge-Kutta step
+//
+// Reference orbital_integration.c:158-348.  Per stage, on the stage input y:
+//   H_k = L y_k + g sum A[k][ql][s] conj(y_s) y_q y_l    (linear + interacting)
+//   P[s][k] = <y_s, H_k>, G[s][t] = <y_s, y_t>           (Simpson)
+//   C = S^-1 P (robust; S = G with the reference's overlap pattern) or C = P
+//   der_k = -i tf (H_k - sum_s y_s C[s][k])
+// L is the reference's finite-difference operator (split_linear_orbitals.c:
+// 209-255) or its spectral one (:257-286) plus V.
+
+__device__ void gj_inverse_wave(int M, double2* a, double2* inv, double2* fac);
+
+// L y_k at point x by finite differences (n = grid size).
+__device__ __forceinline__ double2 horb_fd_at(const double2* f, int n, int x, int bounds, double2 up, double2 lo,
+                                              double c0, const double* V) {
+    if (bounds == kPeriodicBounds) {
+        if (x == n - 1) x = 0;  // out[n-1] = out[0]
+        const int xm = x == 0 ? n - 2 : x - 1;
+        const int xp = x == n - 2 ? 0 : x + 1;
+        return cscale(f[x], c0 + V[x]) + cmul(up, f[xp]) + cmul(lo, f[xm]);
+    }
+    double2 out = cscale(f[x], c0 + V[x]);
+    if (x + 1 < n) out = out + cmul(up, f[x + 1]);
+    if (x > 0) out = out + cmul(lo, f[x - 1]);
+    return out;
+}
+
+struct FsLin {
+    int fd, bounds;
+    double2 up, lo;
+    double c0;
+};
+
+// H[k][x] = (finite-difference L y_k, or the spectral L y_k already in H)
+// + g * interacting part from the A table.  Grid (n / 64, M), one k per block.
+template <int M>
+__global__ __launch_bounds__(64) void k_fs_haction(int n, const double2* __restrict__ y, FsLin lin,
+                                                   const double* __restrict__ V, const double2* __restrict__ coef,
+                                                   double g, double2* __restrict__ H) {
+    constexpr int P = M * (M + 1) / 2;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (x >= n) return;
+    double2 f[M];
+#pragma unroll
+    for (int a = 0; a < M; a++) f[a] = y[(size_t) a * n + x];
+    const double2* A = coef + (size_t) k * P * M;
+    double2 acc[M];
+#pragma unroll
+    for (int s = 0; s < M; s++) acc[s] = make_double2(0, 0);
+    int ql = 0;
+#pragma unroll
+    for (int q = 0; q < M; q++)
+#pragma unroll
+        for (int l = q; l < M; l++, ql++) {
+            const double2 pql = cmul(f[q], f[l]);
+#pragma unroll
+            for (int s = 0; s < M; s++) acc[s] = acc[s] + cmul(A[ql * M + s], pql);
+        }
+    double2 cubic = make_double2(0, 0);
+#pragma unroll
+    for (int s = 0; s < M; s++) cubic = cubic + cconjmul(f[s], acc[s]);
+    const size_t o = (size_t) k * n + x;
+    const double2 l = lin.fd ? horb_fd_at(y + (size_t) k * n, n, x, lin.bounds, lin.up, lin.lo, lin.c0, V) : H[o];
+    H[o] = l + cscale(cubic, g);
+}
+
+// The same for any orbital count, orbitals read from memory.
+__global__ __launch_bounds__(64) void k_fs_haction_any(int M, int n, const double2* __restrict__ y, FsLin lin,
+                                                       const double* __restrict__ V,
+                                                       const double2* __restrict__ coef, double g,
+                                                       double2* __restrict__ H) {
+    const int P = M * (M + 1) / 2;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (x >= n) return;
+    const double2* A = coef + (size_t) k * P * M;
+    double2 cubic = make_double2(0, 0);
+    for (int s = 0; s < M; s++) {
+        double2 acc = make_double2(0, 0);
+        int ql = 0;
+        for (int q = 0; q < M; q++)
+            for (int l = q; l < M; l++, ql++)
+                acc = acc + cmul(A[ql * M + s], cmul(y[(size_t) q * n + x], y[(size_t) l * n + x]));
+        cubic = cubic + cconjmul(y[(size_t) s * n + x], acc);
+    }
+    const size_t o = (size_t) k * n + x;
+    const double2 l = lin.fd ? horb_fd_at(y + (size_t) k * n, n, x, lin.bounds, lin.up, lin.lo, lin.c0, V) : H[o];
+    H[o] = l + cscale(cubic, g);
+}
+
+// dst_k = IDFT(sym * DFT(src_k)) + V src_k, dst[n] = dst[0]: the spectral
+// linear part (split_linear_orbitals.c:257-286); one block per orbital.
+__global__ __launch_bounds__(256) void k_fft_apply(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                   int grid, int n, int log2n, const double2* __restrict__ tw,
+                                                   const double2* __restrict__ sym, const double* __restrict__ V) {
+    extern __shared__ double2 X[];
+    const double2* f = src + (size_t) blockIdx.x * grid;
+    double2* o = dst + (size_t) blockIdx.x * grid;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) X[bitrev(i, log2n)] = f[i];
+    __syncthreads();
+    fft_inplace(X, n, log2n, tw, false);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) X[i] = cmul(sym[i], X[i]);
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int r = bitrev(i, log2n);
+        if (i < r) {
+            const double2 t = X[i];
+            X[i] = X[r];
+            X[r] = t;
+        }
+    }
+    __syncthreads();
+    fft_inplace(X, n, log2n, tw, true);
+    const double sc = 1.0 / (double) n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) o[i] = cscale(X[i], sc) + cscale(f[i], V[i]);
+    __syncthreads();
+    if (threadIdx.x == 0) o[n] = cscale(X[0], sc) + cscale(f[0], V[0]);
+}
+
+// Inverse direct transform for lengths that are not powers of two, written
+// to dst with V src added (pairs with k_dft_forward_mul).
+__global__ __launch_bounds__(256) void k_dft_inverse_add(double2* __restrict__ dst, const double2* __restrict__ src,
+                                                         int grid, int n, const double2* __restrict__ tw,
+                                                         const double2* __restrict__ tmp,
+                                                         const double* __restrict__ V) {
+    extern __shared__ double2 X[];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) X[i] = tmp[(size_t) blockIdx.x * n + i];
+    __syncthreads();
+    double2* o = dst + (size_t) blockIdx.x * grid;
+    const double2* f = src + (size_t) blockIdx.x * grid;
+    const double sc = 1.0 / (double) n;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        double2 acc = make_double2(0, 0);
+        int t = 0;
+        for (int j = 0; j < n; j++) {
+            acc = cfma(X[j], dconj2(tw[t]), acc);
+            t += k;
+            if (t >= n) t -= n;
+        }
+        o[k] = cscale(acc, sc) + cscale(f[k], V[k]);
+        if (k == 0) o[n] = o[0];
+    }
+}
+
+// mats[0..M^2): P[a][b] = <y_a, H_b>; with the Gram part, mats[M^2..2M^2):
+// G[a][b] = <y_a, y_b>.  One block per entry, Simpson weights.
+__global__ __launch_bounds__(256) void k_fs_dots(int M, int n, double dx, const double2* __restrict__ y,
+                                                 const double2* __restrict__ H, double2* __restrict__ mats) {
+    __shared__ double red[16];
+    const int e = blockIdx.x, mm = M * M;
+    const int a = (e % mm) / M, b = e % M;
+    const double2* fa = y + (size_t) a * n;
+    const double2* fb = (e < mm ? H : y) + (size_t) b * n;
+    double v[2] = {0, 0};
+    for (int p = threadIdx.x; p < n; p += blockDim.x) {
+        const double2 t = cconjmul(fa[p], fb[p]);
+        const double w = simpson_w(p, n, dx);
+        v[0] += w * t.x;
+        v[1] += w * t.y;
+    }
+    block_sum<2>(v, red);
+    if (threadIdx.x == 0) mats[e] = make_double2(v[0], v[1]);
+}
+
+// C = P (simple projector) or S^-1 P (robust).  S is the overlap matrix as
+// the reference fills it (orbital_matrices.c:7-20): its j loop starts at 2,
+// so S[0][1] and S[1][0] are never set (quirk Q2; zero here, uninitialised
+// memory in the reference); the inverse reads the lower triangle and the
+// real diagonal (lapack_interface.c:10-59).  One wave.
+__global__ __launch_bounds__(64) void k_fs_coef(int M, int robust, double2* __restrict__ mats) {
+    __shared__ double2 a[30 * 30], inv[30 * 30], fac[64];
+    const int r = threadIdx.x, mm = M * M;
+    const double2 *P = mats, *G = mats + mm;
+    double2* C = mats + 2 * mm;
+    if (!robust) {
+        for (int e = r; e < mm; e += blockDim.x) C[e] = P[e];
+        return;
+    }
+    auto lower = [&](int i, int j) {  // S[i][j], i > j
+        if (j >= 2) return G[i * M + j];
+        if (i >= 2) return dconj2(G[j * M + i]);
+        return make_double2(0, 0);
+    };
+    if (r < M)
+        for (int c = 0; c < M; c++)
+            a[c * M + r] = c == r ? make_double2(G[r * M + r].x, 0) : (r < c ? lower(c, r) : dconj2(lower(r, c)));
+    __syncthreads();
+    gj_inverse_wave(M, a, inv, fac);
+    if (r < M)  // lane r: column r of C = S^-1 P
+        for (int s = 0; s < M; s++) {
+            double2 acc = make_double2(0, 0);
+            for (int l = 0; l < M; l++) acc = acc + cmul(inv[s * M + l], P[l * M + r]);
+            C[s * M + r] = acc;
+        }
+}
+
+// out_k = sum_s y_s C[s][k] (project) or -i tf (H_k - that) (der).
+__global__ __launch_bounds__(256) void k_fs_der(int M, int n, const double2* __restrict__ y,
+                                                const double2* __restrict__ H, const double2* __restrict__ mats,
+                                                double2 tf, int project, double2* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M * n) return;
+    const int k = i / n, x = i % n;
+    const double2* C = mats + 2 * M * M;
+    double2 p = make_double2(0, 0);
+    for (int s = 0; s < M; s++) p = p + cmul(y[(size_t) s * n + x], C[s * M + k]);
+    if (project) {
+        out[i] = p;
+        return;
+    }
+    out[i] = cmul(make_double2(tf.y, -tf.x), H[i] - p);
+}
+
+// P and C for the orbitals y and the functions H (k_fs_dots + k_fs_coef).
+static int fs_projection(OrbWork* w, int robust, const double2* y, const double2* H, hipStream_t st) {
+    const int M = w->eq.norb, n = w->eq.grid_size;
+    k_fs_dots<<<(robust ? 2 : 1) * M * M, 256, 0, st>>>(M, n, w->eq.dx, y, H, w->d_fsmat);
+    k_fs_coef<<<1, 64, 0, st>>>(M, robust, w->d_fsmat);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
+// der = right-hand side of the full equation at y; the A table in d_nl
+// must be current.
+static int fs_rhs(OrbWork* w, int use_fd, int robust, const double2* y, double2* der, hipStream_t st) {
+    const OrbEq& q = w->eq;
+    const int M = q.norb, n = q.grid_size, nf = w->nfft;
+    double2* H = w->d_fsH;
+    if (!use_fd) {
+        if (w->log2n >= 0) {
+            k_fft_apply<<<M, 256, sizeof(double2) * nf, st>>>(y, H, n, nf, w->log2n, w->d_tw, w->d_hsym, w->d_pot);
+        } else {
+            k_dft_forward_mul<<<M, 256, sizeof(double2) * nf, st>>>(y, n, nf, w->d_tw, w->d_hsym, w->d_cnwork);
+            k_dft_inverse_add<<<M, 256, sizeof(double2) * nf, st>>>(H, y, n, nf, w->d_tw, w->d_cnwork, w->d_pot);
+        }
+    }
+    FsLin lin{};
+    lin.fd = use_fd;
+    lin.bounds = q.bounds;
+    const double2 a1 = q.d1coef;
+    const double a2 = q.d2coef, dx = q.dx;
+    lin.up = make_double2(a2 / dx / dx + a1.x / dx / 2, a1.y / dx / 2);
+    lin.lo = make_double2(a2 / dx / dx - a1.x / dx / 2, -a1.y / dx / 2);
+    lin.c0 = -2 * a2 / dx / dx;
+    const dim3 grid(div_up(n, 64), M);
+#define FS_CASE(m)                                                                                  \
+    case m:                                                                                        \
+        k_fs_haction<m><<<grid, 64, 0, st>>>(n, y, lin, w->d_pot, w->d_nl, q.g, H);              \
+        break;
+    switch (M) {
+        FS_CASE(1)
+        FS_CASE(2)
+        FS_CASE(3)
+        FS_CASE(4)
+        FS_CASE(5)
+        FS_CASE(6)
+        FS_CASE(7)
+        FS_CASE(8)
+        default: k_fs_haction_any<<<grid, 64, 0, st>>>(M, n, y, lin, w->d_pot, w->d_nl, q.g, H);
+    }
+#undef FS_CASE
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    int rc = fs_projection(w, robust, y, H, st);
+    if (rc) return rc;
+    k_fs_der<<<div_up(M * n, 256), 256, 0, st>>>(M, n, y, H, w->d_fsmat, q.time_fac, 0, der);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
+int fullstep_rhs(OrbWork* w, int use_fd, int robust, const double2* orb, const double2* rho2, const double2* rinv,
+                 double2* der, hipStream_t st) {
+    k_nl_coef<<<1, 256, 0, st>>>(w->eq.norb, nullptr, nullptr, rho2, rinv, w->d_nl);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return fs_rhs(w, use_fd, robust, orb, der, st);
+}
+
+int projector(OrbWork* w, int robust, const double2* orb, const double2* hact, double2* proj, hipStream_t st) {
+    const int M = w->eq.norb, n = w->eq.grid_size;
+    int rc = fs_projection(w, robust, orb, hact, st);
+    if (rc) return rc;
+    k_fs_der<<<div_up(M * n, 256), 256, 0, st>>>(M, n, orb, hact, w->d_fsmat, w->eq.time_fac, 1, proj);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
+constexpr double kRk4A[4 * 4] = {0, 0, 0, 0, 0.5, 0, 0, 0, 0, 0.5, 0, 0, 0, 0, 1, 0};
+constexpr double kRk4B[4] = {1.0 / 6, 1.0 / 3, 1.0 / 3, 1.0 / 6};
+
+int fullstep(OrbWork* w, int use_fd, int robust, double2* orb, const double2* rho2, const double2* rinv,
+             hipStream_t st) {
+    const OrbEq& q = w->eq;
+    const int M = q.norb, total = M * q.grid_size;
+    const unsigned nb = div_up(total, 256);
+    int rc = kOk;
+    if (q.bounds == kPeriodicBounds && (rc = periodic_bounds(w, orb, st))) return rc;
+    k_nl_coef<<<1, 256, 0, st>>>(M, nullptr, nullptr, rho2, rinv, w->d_nl);
+    int ns = 4;
+    const double *A = kRk4A, *b = kRk4B;
+    if (w->rk_order == 2) ns = 2, A = kRk2A, b = kRk2B;
+    if (w->rk_order == 5) ns = 6, A = kRk5A, b = kRk5B;
+    double2* k = w->d_stage;
+    double2* y = w->d_stage + 6 * (size_t) total;
+    for (int s = 0; s < ns; s++) {
+        const double2* in = orb;
+        if (s > 0) {
+            RkSum r{};
+            for (int j = 0; j < s; j++) {
+                r.k[j] = k + (size_t) j * total;
+                r.a[j] = A[s * ns + j];
+            }
+            r.ns = s;
+            k_rk_sum<<<nb, 256, 0, st>>>(orb, r, q.tstep, y, total);
+            in = y;
+        }
+        if ((rc = fs_rhs(w, use_fd, robust, in, k + (size_t) s * total, st))) return rc;
+    }
+    RkSum r{};
+    for (int j = 0; j < ns; j++) {
+        r.k[j] = k + (size_t) j * total;
+        r.a[j] = b[j];
+    }
+    r.ns = ns;
+    k_rk_sum<<<nb, 256, 0, st>>>(orb, r, q.tstep, orb, total);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    if (w->imagtime) rc = orthonormalize(w, orb, st);
+    return rc;
+}
+
 int splitstep(OrbWork* w, int use_fft, double2* orb, double2* hob, double2* hint, const double2* rho2,
               const double2* rinv, hipStream_t st) {
     const OrbEq& q = w->eq;
@@ -827,6 +1157,56 @@ __device__ void herm_eig_wave(int n, double2* a, double* lam, double2* v, double
     }
 }
 
+// Gauss-Jordan inverse with partial pivoting on one wave (M <= 64): a (LDS,
+// row-major, overwritten) -> inv (LDS).  Lane r owns column r; every lane of
+// the block must call it.  Elimination order and arithmetic per element are
+// those of a serial Gauss-Jordan sweep (lapack_interface.c:10-59 stand-in).
+__device__ void gj_inverse_wave(int M, double2* a, double2* inv, double2* fac) {
+    const int r = threadIdx.x;
+    if (r < M)
+        for (int c = 0; c < M; c++) inv[c * M + r] = make_double2(c == r ? 1.0 : 0.0, 0);
+    __syncthreads();
+    for (int col = 0; col < M; col++) {
+        int piv = col;
+        double best = 0;
+        for (int i = col; i < M; i++) {
+            const double m2 = a[i * M + col].x * a[i * M + col].x + a[i * M + col].y * a[i * M + col].y;
+            if (m2 > best) {
+                best = m2;
+                piv = i;
+            }
+        }
+        if (best == 0) break;
+        __syncthreads();
+        if (piv != col && r < M) {
+            double2 t = a[col * M + r];
+            a[col * M + r] = a[piv * M + r];
+            a[piv * M + r] = t;
+            t = inv[col * M + r];
+            inv[col * M + r] = inv[piv * M + r];
+            inv[piv * M + r] = t;
+        }
+        __syncthreads();
+        const double2 d = cdiv(make_double2(1, 0), a[col * M + col]);
+        __syncthreads();
+        if (r < M) {
+            a[col * M + r] = cmul(a[col * M + r], d);
+            inv[col * M + r] = cmul(inv[col * M + r], d);
+            fac[r] = a[r * M + col];
+        }
+        __syncthreads();
+        if (r < M)
+            for (int i = 0; i < M; i++) {
+                if (i == col) continue;
+                const double2 f = fac[i];
+                if (f.x == 0 && f.y == 0) continue;
+                a[i * M + r] = a[i * M + r] - cmul(f, a[col * M + r]);
+                inv[i * M + r] = inv[i * M + r] - cmul(f, inv[col * M + r]);
+            }
+        __syncthreads();
+    }
+}
+
 // rho1 eigenvalues (synchronize.c:49), regularisation when the smallest
 // natural occupation is below reg_factor times the largest
 // (lapack_interface.c:126-163, synchronize.c:50-54), then the inverse from
@@ -869,51 +1249,11 @@ __global__ __launch_bounds__(64) void k_density_sync_small(int M, double2* __res
     // Gauss-Jordan inverse with partial pivoting of the Hermitian matrix
     // given by the lower triangle and real diagonal
     if (r < M)
-        for (int c = 0; c < M; c++) {
+        for (int c = 0; c < M; c++)
             a[c * M + r] = c == r ? make_double2(rl[r * M + r].x, 0)
                                   : (r < c ? rl[c * M + r] : dconj2(rl[r * M + c]));
-            inv[c * M + r] = make_double2(c == r ? 1.0 : 0.0, 0);
-        }
     __syncthreads();
-    for (int col = 0; col < M; col++) {
-        int piv = col;
-        double best = 0;
-        for (int i = col; i < M; i++) {
-            const double m2 = a[i * M + col].x * a[i * M + col].x + a[i * M + col].y * a[i * M + col].y;
-            if (m2 > best) {
-                best = m2;
-                piv = i;
-            }
-        }
-        if (best == 0) break;
-        __syncthreads();
-        if (piv != col && r < M) {
-            double2 t = a[col * M + r];
-            a[col * M + r] = a[piv * M + r];
-            a[piv * M + r] = t;
-            t = inv[col * M + r];
-            inv[col * M + r] = inv[piv * M + r];
-            inv[piv * M + r] = t;
-        }
-        __syncthreads();
-        const double2 d = cdiv(make_double2(1, 0), a[col * M + col]);
-        __syncthreads();
-        if (r < M) {
-            a[col * M + r] = cmul(a[col * M + r], d);
-            inv[col * M + r] = cmul(inv[col * M + r], d);
-            fac[r] = a[r * M + col];
-        }
-        __syncthreads();
-        if (r < M)
-            for (int i = 0; i < M; i++) {
-                if (i == col) continue;
-                const double2 f = fac[i];
-                if (f.x == 0 && f.y == 0) continue;
-                a[i * M + r] = a[i * M + r] - cmul(f, a[col * M + r]);
-                inv[i * M + r] = inv[i * M + r] - cmul(f, inv[col * M + r]);
-            }
-        __syncthreads();
-    }
+    gj_inverse_wave(M, a, inv, fac);
     if (r < M)
         for (int c = 0; c < M; c++) rinv[c * M + r] = inv[c * M + r];
 }
@@ -996,6 +1336,9 @@ int orbwork_create(const OrbEq& eq, int imagtime, const double* pot_host, OrbWor
     ok = ok && hipMalloc((void**) &w->d_cnwork, sizeof(double2) * 8 * (size_t) M * grid) == hipSuccess;
     ok = ok && hipMalloc((void**) &w->d_nl, sizeof(double2) * ((size_t) M * M * M * (M + 1) / 2 + M * M)) ==
                    hipSuccess;
+    ok = ok && hipMalloc((void**) &w->d_fsH, sizeof(double2) * (size_t) M * grid) == hipSuccess;
+    ok = ok && hipMalloc((void**) &w->d_fsmat, sizeof(double2) * 3 * M * M) == hipSuccess;
+    ok = ok && hipMalloc((void**) &w->d_hsym, sizeof(double2) * n) == hipSuccess;
     if (!ok) {
         orbwork_destroy(w);
         set_error("device allocation failed for the orbital workspace");
@@ -1004,6 +1347,13 @@ int orbwork_create(const OrbEq& eq, int imagtime, const double* pot_host, OrbWor
     MCTDHB_HIP_CHECK(hipMemcpy(w->d_fexp, fexp.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
     MCTDHB_HIP_CHECK(hipMemcpy(w->d_tw, tw.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
     MCTDHB_HIP_CHECK(hipMemcpy(w->d_cn, cn.data(), sizeof(double2) * 3 * grid, hipMemcpyHostToDevice));
+    // spectral symbol of the linear part: i freq d1 - freq^2 d2 (split_linear_orbitals.c:275-278)
+    std::vector<double2> sym(n);
+    for (int i = 0; i < n; i++) {
+        const double freq = (i <= (n - 1) / 2) ? (2 * M_PI * i) / (n * eq.dx) : (2 * M_PI * (i - n)) / (n * eq.dx);
+        sym[i] = make_double2(-freq * eq.d1coef.y - freq * freq * eq.d2coef, freq * eq.d1coef.x);
+    }
+    MCTDHB_HIP_CHECK(hipMemcpy(w->d_hsym, sym.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
     int rc = orbwork_set_potential(w, pot_host);
     if (rc) {
         orbwork_destroy(w);
@@ -1028,6 +1378,9 @@ void orbwork_destroy(OrbWork* w) {
     hipFree(w->d_red);
     hipFree(w->d_cnwork);
     hipFree(w->d_nl);
+    hipFree(w->d_fsH);
+    hipFree(w->d_fsmat);
+    hipFree(w->d_hsym);
     delete w;
 }
 

@@ -21,6 +21,8 @@ struct OrbWork {
     OrbEq eq;
     int imagtime = 0;
     int rk_order = 4;  // nonlinear sub-step: 2 (Heun), 4 (classical), 5 (Butcher, 6 stages)
+    int fullstep = 0;  // whole-step orbital integrator: 0 split step, 1 full-equation RK
+    int robust = 0;    // full-equation RK: projector with the overlap inverse
     int nfft = 0, log2n = -1;  // FFT length grid_size - 1 (log2n < 0: not a power of two)
     double* d_pot = nullptr;     // grid_size
     double2* d_fexp = nullptr;   // nfft: exp(-i dt/2 (i a1 k - a2 k^2))
@@ -30,6 +32,9 @@ struct OrbWork {
     double* d_red = nullptr;     // reduction scratch
     double2* d_cnwork = nullptr; // 8 * norb * grid_size: Crank-Nicolson sweep scratch
     double2* d_nl = nullptr;     // grid-independent factors of the nonlinear term
+    double2* d_fsH = nullptr;    // norb * grid_size: H y of the full-equation step
+    double2* d_fsmat = nullptr;  // 3 norb^2: projections, Gram matrix, projector coefficients
+    double2* d_hsym = nullptr;   // nfft: spectral symbol of the linear part
 };
 
 int orbwork_create(const OrbEq& eq, int imagtime, const double* pot_host, OrbWork** out);
@@ -53,6 +58,16 @@ int orthonormalize(OrbWork* w, double2* orb, hipStream_t st);
 // Gram-Schmidt in imaginary time); hob/hint are rebuilt from the orbitals.
 int splitstep(OrbWork* w, int use_fft, double2* orb, double2* hob, double2* hint, const double2* rho2,
               const double2* rinv, hipStream_t st);
+// Full-equation Runge-Kutta orbital step (orbital_integration.c:283-348):
+// linear part by finite differences (use_fd) or FFT, simple or robust
+// projector, Gram-Schmidt in imaginary time.  rho2 and rinv are held fixed.
+int fullstep(OrbWork* w, int use_fd, int robust, double2* orb, const double2* rho2, const double2* rinv,
+             hipStream_t st);
+// Its right-hand side at orb (orbital_integration.c:158-235).
+int fullstep_rhs(OrbWork* w, int use_fd, int robust, const double2* orb, const double2* rho2, const double2* rinv,
+                 double2* der, hipStream_t st);
+// proj_k = projection of hact_k onto the orbitals (orbital_integration.c:22-129).
+int projector(OrbWork* w, int robust, const double2* orb, const double2* hact, double2* proj, hipStream_t st);
 // rho1 -> eigenvalues, conditional regularisation, inverse (M x M, one wave,
 // LDS work space: no allocation, so the call can be captured in a graph).
 int density_sync_small(int norb, double2* rho1, double2* rinv, double* eigvals, double reg_factor,

@@ -104,6 +104,7 @@ def _declare(L):
     L.set_energy_convergence_digits.argtypes = [C.c_uint8]
     L.set_energy_convergence_eig_residual.argtypes = [C.c_double]
     L.set_overlap_residue_threshold.argtypes = [C.c_double]
+    L.set_overlap_matrix.argtypes = [C.c_uint16, C.c_uint16, C.c_double, _cmat, _cmat]
     L.overlap_residual.argtypes = [C.c_uint16, C.c_uint16, C.c_double, _cmat]
     L.overlap_residual.restype = C.c_double
     L.avg_orbitals_norm.argtypes = [C.c_uint16, C.c_uint16, C.c_double, _cmat]

@@ -112,13 +112,28 @@ void   sync_orbital_matrices(OrbitalEquation eq_desc, ManyBodyState psi);       
 void   sync_density_matrices(MultiConfiguration mc_space, ManyBodyState psi);    /* synchronize.c:41-56 */
 void   sync_equation_params(OrbitalEquation eq_desc);                            /* synchronize.c:58-67 */
 
-/* include/integrator/orbital_integration.h:8, :37.  Orbital integrator
- * SPLITSTEP only; rk_order RK2 / RK4 / RK5.  orb_next is scratch. */
+/* include/integrator/orbital_integration.h:7-37.  rk_order RK2 / RK4 / RK5;
+ * derivatives FINITEDIFF or SPECTRAL (DVR is not provided).  orb_next is
+ * scratch.  The robust projector's overlap matrix has the reference's
+ * unset [0][1] / [1][0] entries (quirk Q2) as zero. */
 void set_periodic_bounds(uint16_t norb, uint16_t grid_size, Cmatrix orb);       /* orbital_integration.c:13-20 */
+void robust_multiorb_projector(OrbitalEquation eq_desc, uint16_t norb, Cmatrix orb, Cmatrix hact_orb,
+                               Cmatrix project);                                 /* orbital_integration.c:22-93 */
+void simple_multiorb_projector(OrbitalEquation eq_desc, uint16_t norb, Cmatrix orb, Cmatrix hact_orb,
+                               Cmatrix project);                                 /* orbital_integration.c:94-129 */
+void propagate_fullstep_orb_rk(MCTDHBDataStruct mctdhb, Carray orb_next);        /* orbital_integration.c:283-348 */
 void propagate_splitstep_orb(MCTDHBDataStruct mctdhb, Carray orb_next);          /* orbital_integration.c:351-434 */
 
+/* include/function_tools/orbital_matrices.h:19-32.  set_overlap_matrix
+ * writes exactly the entries the reference writes (its j loop starts at 2). */
+void set_overlap_matrix(uint16_t norb, uint16_t grid_size, double dx, Cmatrix orb,
+                        Cmatrix overlap);                                        /* orbital_matrices.c:7-20 */
+void set_orbital_hob(OrbitalEquation eq_desc, uint16_t norb, Cmatrix orb, Cmatrix hob); /* orbital_matrices.c:22-70 */
+void set_orbital_hint(OrbitalEquation eq_desc, uint16_t norb, Cmatrix orb, Carray hint); /* orbital_matrices.c:72-287 */
+
 /* include/integrator/driver.h:66-71.  Coefficient integrator LANCZOS only;
- * orb_works and coef_works are scratch. */
+ * orbital integrator SPLITSTEP or FULLSTEP_RUNGEKUTTA; orb_works and
+ * coef_works are scratch. */
 void mctdhb_propagate_step(MCTDHBDataStruct mctdhb, Carray orb_works, Carray coef_works,
                            uint32_t curr_step);                                  /* driver.c:158-202 */
 
@@ -330,6 +345,20 @@ int mctdhb_nonlinear_rhs_dev(mctdhb_orbital* w, const void* orb, const void* hob
 int mctdhb_orthonormalize_dev(mctdhb_orbital* w, void* orb, void* stream);   /* calculus.c:170-191 */
 int mctdhb_splitstep_dev(mctdhb_orbital* w, int use_fft, void* orb, void* hob, void* hint, const void* rho2,
                          const void* rinv, void* stream);                     /* orbital_integration.c:351-434 */
+/* Full-equation Runge-Kutta orbital step: linear part by finite
+ * differences (use_fd) or FFT, simple or robust projector, Gram-Schmidt in
+ * imaginary time; rho2 and rinv held fixed. */
+int mctdhb_fullstep_dev(mctdhb_orbital* w, int use_fd, int robust, void* orb, const void* rho2, const void* rinv,
+                        void* stream);                                        /* orbital_integration.c:283-348 */
+int mctdhb_fullstep_rhs_dev(mctdhb_orbital* w, int use_fd, int robust, const void* orb, const void* rho2,
+                            const void* rinv, void* der, void* stream);       /* orbital_integration.c:158-235 */
+/* proj = projection of hact (norb x grid) onto the orbitals */
+int mctdhb_projector_dev(mctdhb_orbital* w, int robust, const void* orb, const void* hact, void* proj,
+                         void* stream);                                       /* orbital_integration.c:22-129 */
+/* The orbital integrator of mctdhb_propagate_step_dev and the step graph:
+ * orb_method SPLITSTEP (31, default) or FULLSTEP_RUNGEKUTTA (30); with the
+ * full step use_fft = 0 selects finite differences. */
+int mctdhb_orbital_set_integrator(mctdhb_orbital* w, int orb_method, int robust);
 /* rho1, rho2 of coef, then natural occupations (eigvals, M doubles, device,
  * may be NULL), regularisation and rinv = rho1^-1 (synchronize.c:41-56). */
 int mctdhb_density_sync_dev(mctdhb_space* space, const void* coef, void* rho1, void* rho2, void* rinv,

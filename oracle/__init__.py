@@ -118,6 +118,17 @@ def lib():
         L.orc_splitstep_orb.argtypes = [oe, C.c_int, C.c_int, _cp, _cp, _cp, _cp, _cp]
         L.orc_propagate_step.argtypes = [sp, oe, C.c_int, C.c_int, C.c_int, _cp, _cp, _cp,
                                          _cp, _cp, _cp, _cp, _cp]
+        L.orc_orb_full_nonlinear.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, _cp, _cp, _cp, _cp,
+                                             _cp]
+        L.orc_orb_full_nonlinear.restype = _CD
+        L.orc_orb_interacting_part.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, _cp, _cp, _cp]
+        L.orc_orb_interacting_part.restype = _CD
+        L.orc_linear_horb_fd.argtypes = [oe, _cp, _cp]
+        L.orc_linear_horb_fft.argtypes = [oe, _cp, _cp]
+        L.orc_overlap_matrix.argtypes = [C.c_int, C.c_int, C.c_double, _cp, _cp]
+        L.orc_multiorb_projector.argtypes = [oe, C.c_int, _cp, _cp, _cp]
+        L.orc_fullstep_rhs.argtypes = [oe, C.c_int, C.c_int, _cp, _cp, _cp, _cp]
+        L.orc_fullstep_orb.argtypes = [oe, C.c_int, C.c_int, C.c_int, _cp, _cp, _cp]
         _lib = L
     return _lib
 

@@ -123,4 +123,17 @@ void  orc_propagate_step(const orc_space* S, orc_orbeq* eq, int imagtime, int us
                          int lanczos_iter, ocplx* orb, ocplx* coef, ocplx* hob, ocplx* hint,
                          ocplx* rho1, ocplx* rho2, ocplx* rinv, ocplx* lvec_work);
 
+/* fullstep.c: the full-equation RK orbital step (orbital_integration.c:22-348) */
+void  orc_linear_horb_fd(const orc_orbeq* eq, const ocplx* f, ocplx* out);
+void  orc_linear_horb_fft(const orc_orbeq* eq, const ocplx* f, ocplx* out);
+ocplx orc_orb_interacting_part(int k, int n, double g, int m, int grid_size, const ocplx* orb,
+                               const ocplx* rho, const ocplx* rinv);
+void  orc_overlap_matrix(int norb, int grid, double dx, const ocplx* orb, ocplx* overlap);
+void  orc_multiorb_projector(const orc_orbeq* eq, int robust, const ocplx* orb, const ocplx* hact,
+                             ocplx* project);
+void  orc_fullstep_rhs(const orc_orbeq* eq, int use_fd, int robust, const ocplx* orb, const ocplx* rho2,
+                       const ocplx* rinv, ocplx* der);
+void  orc_fullstep_orb(const orc_orbeq* eq, int imagtime, int use_fd, int robust, ocplx* orb,
+                       const ocplx* rho2, const ocplx* rinv);
+
 #endif

@@ -231,3 +231,24 @@ def test_overlap_residual_and_orbital_norm(dropin):
         L = dropin.lib()
         assert L.overlap_residual(4, grid, dx, st.orbitals) == pytest.approx(ov, rel=1e-13)
         assert L.avg_orbitals_norm(4, grid, dx, st.orbitals) == pytest.approx(nrm, rel=1e-13)
+
+
+def test_set_overlap_matrix_writes_what_the_reference_writes(dropin):
+    """orbital_matrices.c:7-20: the j loop starts at 2, so [0][1] and [1][0]
+    keep the caller's values (quirk Q2); the rest is <f_i, f_j>."""
+    M, grid = 4, 41
+    s = dropin.HostState(3, M, grid)
+    orb, _ = _fill(s, 9)
+    sentinel = 7.0 + 7.0j
+    s.set("ob_denmat", np.full((M, M), sentinel))
+    dx = 10.0 / (grid - 1)
+    st = s._p.contents
+    dropin.lib().set_overlap_matrix(M, grid, dx, st.orbitals, st.ob_denmat)
+    S = s.get("ob_denmat")
+    for i in range(M):
+        for j in range(M):
+            if {i, j} == {0, 1}:
+                assert S[i, j] == sentinel
+            else:
+                ref = oracle.csimps(np.conj(orb[i]) * orb[j], dx)
+                assert abs(S[i, j] - ref) <= 1e-13 * abs(ref) + 1e-16

@@ -23,12 +23,12 @@ INTEGRATOR = """# integrator description in the reference's format
 {time}
 # coefficients: 20 Lanczos
 20
-# orbitals: 31 split-step
-31
-# derivatives: 301 FFT, 302 finite differences (Crank-Nicolson)
+# orbitals: 30 full-equation Runge-Kutta, 31 split-step
+{orb}
+# derivatives: 301 FFT, 302 finite differences
 {der}
 # Runge-Kutta order
-4
+{rk}
 # boundaries
 1001
 # Lanczos iterations
@@ -44,13 +44,13 @@ def dropin():
     return d
 
 
-def _write_inputs(tmp_path, prefix, line, npar, norb, grid, xi, xf, time=10, der=301, seed=1):
+def _write_inputs(tmp_path, prefix, line, npar, norb, grid, xi, xf, time=10, der=301, seed=1, orb_method=31, rk=4):
     inp, out = tmp_path / "in", tmp_path / "out"
     inp.mkdir(exist_ok=True)
     out.mkdir(exist_ok=True)
     (inp / f"{prefix}_mctdhb_parameters.dat").write_text(line + "\n")
     conf = tmp_path / "integrator.conf"
-    conf.write_text(INTEGRATOR.format(time=time, der=der))
+    conf.write_text(INTEGRATOR.format(time=time, der=der, orb=orb_method, rk=rk))
     x = np.linspace(xi, xf, grid)
     orb = inputs.hermite_gauss(norb, x, seed=seed, noise=1e-3)
     orb[:, -1] = orb[:, 0]
@@ -185,4 +185,22 @@ def test_real_time_driver_with_time_dependent_trap_matches_oracle(dropin, job_di
     pots = np.loadtxt(out / "rt_job1_obpotential.dat")
     assert pots.shape == (3, grid) and np.allclose(pots[2], 0.5 * omega(0.1) ** 2 * x ** 2)
     assert np.allclose(np.loadtxt(out / "rt_job1_timesteps.dat"), [0.0, 0.05, 0.1])
+    m.close()
+
+
+def test_reference_example_integrator_full_step_reaches_benchmark(dropin, job_dirs):
+    """The integrator of the reference's example file
+    (config_files_example/mctdhb_integrator.txt: full-equation RK, finite
+    differences, RK5) in imaginary time: benchmark.dat:23, N=4, M=3, g=1."""
+    print("driver full step", flush=True)
+    line = "4 3 257 -8.0 8.0 0.005 12.0 -0.5 0.0 linear_ramp 1.0 1.0 0.0 0.0 0.0 harmonic 1.0 1.0 0.0 0.0 0.0"
+    inp, out, conf = _write_inputs(job_dirs, "fs", line, 4, 3, 257, -8.0, 8.0, der=302, orb_method=30, rk=5)
+    dropin.set_directories(str(inp), str(out), str(conf))
+    m = dropin.MCTDHB.from_files("fs")
+    m.run("fs_job1", rec_nsteps=10, monitor_rate=400)
+    m.append_parameters("fs")
+    par = np.loadtxt(out / "fs_mctdhb_parameters.dat")
+    assert abs(par[10] - 0.9685859) < 5e-4, par[10]
+    occ = np.linalg.eigvalsh(m.get("ob_denmat"))
+    assert abs(occ[-1] / 4 - 0.97) < 0.012
     m.close()
