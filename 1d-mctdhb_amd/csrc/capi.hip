@@ -437,7 +437,7 @@ extern "C" int mctdhb_splitstep_dev(mctdhb_orbital* o, int use_fft, void* orb, v
                      (const double2*) rinv, (hipStream_t) stream);
 }
 
-extern "C" int mctdhb_orbital_set_integrator(mctdhb_orbital* o, int orb_method, int robust) {
+extern "C" int mctdhb_orbital_set_integrator(mctdhb_orbital* o, int orb_method, int der_method, int robust) {
     if (!o) {
         set_error("null argument");
         return kInvalidArgument;
@@ -446,14 +446,27 @@ extern "C" int mctdhb_orbital_set_integrator(mctdhb_orbital* o, int orb_method, 
         set_error("orbital integrator must be FULLSTEP_RUNGEKUTTA (30) or SPLITSTEP (31)");
         return kInvalidArgument;
     }
+    if (der_method != 0 && der_method != DVR && der_method != SPECTRAL && der_method != FINITEDIFF) {
+        set_error("derivative method must be 0, DVR (300), SPECTRAL (301) or FINITEDIFF (302)");
+        return kInvalidArgument;
+    }
     o->w->fullstep = orb_method == FULLSTEP_RUNGEKUTTA;
     o->w->robust = robust != 0;
-    return kOk;
+    o->w->dvr = o->w->fullstep && der_method == DVR;
+    return o->w->dvr ? ensure_dvr(o->w) : kOk;
+}
+
+// use_fd of the device entry points: 0 FFT, 1 finite differences, 2 DVR
+static bool der_ok(int d) {
+    if (d >= 0 && d <= 2) return true;
+    set_error("derivative selector must be 0 (FFT), 1 (finite differences) or 2 (DVR)");
+    return false;
 }
 
 extern "C" int mctdhb_fullstep_dev(mctdhb_orbital* o, int use_fd, int robust, void* orb, const void* rho2,
                                    const void* rinv, void* stream) {
     ORB_CHECK(!orb || !rho2 || !rinv);
+    if (!der_ok(use_fd)) return kInvalidArgument;
     return fullstep(o->w, use_fd, robust, (double2*) orb, (const double2*) rho2, (const double2*) rinv,
                     (hipStream_t) stream);
 }
@@ -461,6 +474,7 @@ extern "C" int mctdhb_fullstep_dev(mctdhb_orbital* o, int use_fd, int robust, vo
 extern "C" int mctdhb_fullstep_rhs_dev(mctdhb_orbital* o, int use_fd, int robust, const void* orb, const void* rho2,
                                        const void* rinv, void* der, void* stream) {
     ORB_CHECK(!orb || !rho2 || !rinv || !der);
+    if (!der_ok(use_fd)) return kInvalidArgument;
     return fullstep_rhs(o->w, use_fd, robust, (const double2*) orb, (const double2*) rho2, (const double2*) rinv,
                         (double2*) der, (hipStream_t) stream);
 }
@@ -475,7 +489,7 @@ extern "C" int mctdhb_projector_dev(mctdhb_orbital* o, int robust, const void* o
 // The orbital part of a whole step: the integrator set on the workspace.
 static int orbital_step(OrbWork* w, int use_fft, double2* orb, double2* hob, double2* hint, const double2* rho2,
                         const double2* rinv, hipStream_t st) {
-    if (w->fullstep) return fullstep(w, !use_fft, w->robust, orb, rho2, rinv, st);
+    if (w->fullstep) return fullstep(w, w->dvr ? 2 : (use_fft ? 0 : 1), w->robust, orb, rho2, rinv, st);
     return splitstep(w, use_fft, orb, hob, hint, rho2, rinv, st);
 }
 
@@ -1419,12 +1433,8 @@ static int configure_orbital(MCTDHBDataStruct m, DevOrbital* d) {
         printf("\n\nERROR: unknown orbital integrator %d\n\n", (int) m->orb_integ_method);
         exit(EXIT_FAILURE);
     }
-    if (m->orb_integ_method == FULLSTEP_RUNGEKUTTA && m->orb_der_method == DVR) {
-        printf("\n\nERROR: DVR derivatives are not provided; use FINITEDIFF (302) or SPECTRAL (301)\n\n");
-        exit(EXIT_FAILURE);
-    }
     if (mctdhb_orbital_set_rk_order(d->o, m->rk_order)) fail_stop("Runge-Kutta order");
-    if (mctdhb_orbital_set_integrator(d->o, m->orb_integ_method, m->orb_workspace->impr_ortho))
+    if (mctdhb_orbital_set_integrator(d->o, m->orb_integ_method, m->orb_der_method, m->orb_workspace->impr_ortho))
         fail_stop("orbital integrator");
     return m->orb_der_method != FINITEDIFF;
 }
@@ -1854,7 +1864,8 @@ extern "C" void integration_driver(MCTDHBDataStruct m, uint16_t rec_nsteps, char
                 if (o > g_impr_ortho_thr && !m->orb_workspace->impr_ortho) {
                     printf("\n\n== Start improving orthogonality ==\n\n");
                     m->orb_workspace->impr_ortho = TRUE;  // the full-equation step's robust projector
-                    if (mctdhb_orbital_set_integrator(R.d->o, m->orb_integ_method, 1)) fail_stop("projector");
+                    if (mctdhb_orbital_set_integrator(R.d->o, m->orb_integ_method, m->orb_der_method, 1))
+                        fail_stop("projector");
                 }
                 if (o > REALTIME_OVERLAP_RESIDUE_TOL) {
                     printf("\n\nOverlap residue fell below critical value %.3lf. Aborting\n\n",

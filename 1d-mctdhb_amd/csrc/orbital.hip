@@ -846,6 +846,88 @@ __global__ __launch_bounds__(256) void k_dft_inverse_add(double2* __restrict__ d
     }
 }
 
+// The exponential-DVR matrix of the derivative terms (split_linear_orbitals.c:
+// 82-165), n = grid_size - 1, stored transposed (DT[l * n + j] = D[j][l]) so
+// the product below reads it coalesced.  One thread per entry j >= i;
+// the entry j < i is its (anti-)Hermitian partner, as in the reference.
+__global__ __launch_bounds__(256) void k_dvr_build(int n, double length, double d2coef, double2 d1coef,
+                                                   double2* __restrict__ DT) {
+    const long long e = (long long) blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long) n * n) return;
+    const int i = (int) (e / n), j = (int) (e % n);
+    if (j < i) return;
+    double2 s1 = make_double2(0, 0), s2 = make_double2(0, 0);
+    for (int k = 0; k < n; k++) {
+        const int kmom = k - n / 2;
+        const double w1 = -(2 * M_PI * kmom / length);           // d1 diagonal: i * w1
+        const double w2 = -(2 * M_PI * kmom / length) * (2 * M_PI * kmom / length);
+        double sn = 0, cs = 1;
+        if (j != i) sincos(2 * kmom * (j - i) * M_PI / n, &sn, &cs);
+        // s1 += i w1 e / n, s2 += w2 e / n with e = cos + i sin
+        s1 = s1 + make_double2(-w1 * sn / n, w1 * cs / n);
+        s2 = s2 + make_double2(w2 * cs / n, w2 * sn / n);
+    }
+    const double2 dij = cscale(s2, d2coef) + cmul(d1coef, s1);
+    DT[(size_t) j * n + i] = dij;  // D[i][j]
+    if (j != i) {                  // D[j][i] = d2 conj(s2) - d1 conj(s1)
+        const double2 dji = cscale(make_double2(s2.x, -s2.y), d2coef) - cmul(d1coef, make_double2(s1.x, -s1.y));
+        DT[(size_t) i * n + j] = dji;
+    }
+}
+
+// H_k[j] = sum_l D[j][l] y_k[l] + V[j] y_k[j] (j < n), H_k[n] = H_k[0]
+// (orbital_integration.c:159-169).  One thread per row j, all orbitals.
+template <int M>
+__global__ __launch_bounds__(256) void k_dvr_apply(int n, int grid, const double2* __restrict__ DT,
+                                                   const double2* __restrict__ y, const double* __restrict__ V,
+                                                   double2* __restrict__ H) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double2 acc[M];
+#pragma unroll
+    for (int k = 0; k < M; k++) acc[k] = make_double2(0, 0);
+    for (int l = 0; l < n; l++) {
+        const double2 d = DT[(size_t) l * n + j];
+#pragma unroll
+        for (int k = 0; k < M; k++) acc[k] = acc[k] + cmul(d, y[(size_t) k * grid + l]);
+    }
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        const double2 h = acc[k] + cscale(y[(size_t) k * grid + j], V[j]);
+        H[(size_t) k * grid + j] = h;
+        if (j == 0) H[(size_t) k * grid + n] = h;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dvr_apply_any(int M, int n, int grid, const double2* __restrict__ DT,
+                                                       const double2* __restrict__ y, const double* __restrict__ V,
+                                                       double2* __restrict__ H) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    for (int k = 0; k < M; k++) {
+        double2 acc = make_double2(0, 0);
+        for (int l = 0; l < n; l++) acc = acc + cmul(DT[(size_t) l * n + j], y[(size_t) k * grid + l]);
+        const double2 h = acc + cscale(y[(size_t) k * grid + j], V[j]);
+        H[(size_t) k * grid + j] = h;
+        if (j == 0) H[(size_t) k * grid + n] = h;
+    }
+}
+
+int ensure_dvr(OrbWork* w) {
+    if (w->d_dvrT) return kOk;
+    const int n = w->nfft;
+    if (hipMalloc((void**) &w->d_dvrT, sizeof(double2) * (size_t) n * n) != hipSuccess) {
+        w->d_dvrT = nullptr;
+        set_error("device allocation failed for the DVR matrix");
+        return kOutOfMemory;
+    }
+    const long long total = (long long) n * n;
+    k_dvr_build<<<(unsigned) ((total + 255) / 256), 256>>>(n, n * w->eq.dx, w->eq.d2coef, w->eq.d1coef, w->d_dvrT);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    MCTDHB_HIP_CHECK(hipDeviceSynchronize());
+    return kOk;
+}
+
 // mats[0..M^2): P[a][b] = <y_a, H_b>; with the Gram part, mats[M^2..2M^2):
 // G[a][b] = <y_a, y_b>.  One block per entry, Simpson weights.
 __global__ __launch_bounds__(256) void k_fs_dots(int M, int n, double dx, const double2* __restrict__ y,
@@ -925,12 +1007,35 @@ static int fs_projection(OrbWork* w, int robust, const double2* y, const double2
 }
 
 // der = right-hand side of the full equation at y; the A table in d_nl
-// must be current.
-static int fs_rhs(OrbWork* w, int use_fd, int robust, const double2* y, double2* der, hipStream_t st) {
+// must be current.  der_method: 0 FFT, 1 finite differences, 2 DVR.
+static int fs_rhs(OrbWork* w, int der_method, int robust, const double2* y, double2* der, hipStream_t st) {
     const OrbEq& q = w->eq;
     const int M = q.norb, n = q.grid_size, nf = w->nfft;
+    const int use_fd = der_method == 1;
     double2* H = w->d_fsH;
-    if (!use_fd) {
+    if (der_method == 2) {
+        if (!w->d_dvrT) {
+            set_error("the DVR matrix is not built (ensure_dvr)");
+            return kInvalidArgument;
+        }
+        if (M <= 8) {
+#define DVR_CASE(m) \
+    case m: k_dvr_apply<m><<<div_up(nf, 256), 256, 0, st>>>(nf, n, w->d_dvrT, y, w->d_pot, H); break;
+            switch (M) {
+                DVR_CASE(1)
+                DVR_CASE(2)
+                DVR_CASE(3)
+                DVR_CASE(4)
+                DVR_CASE(5)
+                DVR_CASE(6)
+                DVR_CASE(7)
+                DVR_CASE(8)
+            }
+#undef DVR_CASE
+        } else {
+            k_dvr_apply_any<<<div_up(nf, 256), 256, 0, st>>>(M, nf, n, w->d_dvrT, y, w->d_pot, H);
+        }
+    } else if (!use_fd) {
         if (w->log2n >= 0) {
             k_fft_apply<<<M, 256, sizeof(double2) * nf, st>>>(y, H, n, nf, w->log2n, w->d_tw, w->d_hsym, w->d_pot);
         } else {
@@ -971,11 +1076,13 @@ static int fs_rhs(OrbWork* w, int use_fd, int robust, const double2* y, double2*
     return kOk;
 }
 
-int fullstep_rhs(OrbWork* w, int use_fd, int robust, const double2* orb, const double2* rho2, const double2* rinv,
-                 double2* der, hipStream_t st) {
+int fullstep_rhs(OrbWork* w, int der_method, int robust, const double2* orb, const double2* rho2,
+                 const double2* rinv, double2* der, hipStream_t st) {
+    int rc = der_method == 2 ? ensure_dvr(w) : kOk;
+    if (rc) return rc;
     k_nl_coef<<<1, 256, 0, st>>>(w->eq.norb, nullptr, nullptr, rho2, rinv, w->d_nl);
     MCTDHB_HIP_CHECK(hipGetLastError());
-    return fs_rhs(w, use_fd, robust, orb, der, st);
+    return fs_rhs(w, der_method, robust, orb, der, st);
 }
 
 int projector(OrbWork* w, int robust, const double2* orb, const double2* hact, double2* proj, hipStream_t st) {
@@ -990,12 +1097,13 @@ int projector(OrbWork* w, int robust, const double2* orb, const double2* hact, d
 constexpr double kRk4A[4 * 4] = {0, 0, 0, 0, 0.5, 0, 0, 0, 0, 0.5, 0, 0, 0, 0, 1, 0};
 constexpr double kRk4B[4] = {1.0 / 6, 1.0 / 3, 1.0 / 3, 1.0 / 6};
 
-int fullstep(OrbWork* w, int use_fd, int robust, double2* orb, const double2* rho2, const double2* rinv,
+int fullstep(OrbWork* w, int der_method, int robust, double2* orb, const double2* rho2, const double2* rinv,
              hipStream_t st) {
     const OrbEq& q = w->eq;
     const int M = q.norb, total = M * q.grid_size;
     const unsigned nb = div_up(total, 256);
-    int rc = kOk;
+    int rc = der_method == 2 ? ensure_dvr(w) : kOk;
+    if (rc) return rc;
     if (q.bounds == kPeriodicBounds && (rc = periodic_bounds(w, orb, st))) return rc;
     k_nl_coef<<<1, 256, 0, st>>>(M, nullptr, nullptr, rho2, rinv, w->d_nl);
     int ns = 4;
@@ -1016,7 +1124,7 @@ int fullstep(OrbWork* w, int use_fd, int robust, double2* orb, const double2* rh
             k_rk_sum<<<nb, 256, 0, st>>>(orb, r, q.tstep, y, total);
             in = y;
         }
-        if ((rc = fs_rhs(w, use_fd, robust, in, k + (size_t) s * total, st))) return rc;
+        if ((rc = fs_rhs(w, der_method, robust, in, k + (size_t) s * total, st))) return rc;
     }
     RkSum r{};
     for (int j = 0; j < ns; j++) {
@@ -1381,6 +1489,7 @@ void orbwork_destroy(OrbWork* w) {
     hipFree(w->d_fsH);
     hipFree(w->d_fsmat);
     hipFree(w->d_hsym);
+    hipFree(w->d_dvrT);
     delete w;
 }
 

@@ -23,6 +23,7 @@ struct OrbWork {
     int rk_order = 4;  // nonlinear sub-step: 2 (Heun), 4 (classical), 5 (Butcher, 6 stages)
     int fullstep = 0;  // whole-step orbital integrator: 0 split step, 1 full-equation RK
     int robust = 0;    // full-equation RK: projector with the overlap inverse
+    int dvr = 0;       // full-equation RK: linear part by the exponential DVR
     int nfft = 0, log2n = -1;  // FFT length grid_size - 1 (log2n < 0: not a power of two)
     double* d_pot = nullptr;     // grid_size
     double2* d_fexp = nullptr;   // nfft: exp(-i dt/2 (i a1 k - a2 k^2))
@@ -35,6 +36,7 @@ struct OrbWork {
     double2* d_fsH = nullptr;    // norb * grid_size: H y of the full-equation step
     double2* d_fsmat = nullptr;  // 3 norb^2: projections, Gram matrix, projector coefficients
     double2* d_hsym = nullptr;   // nfft: spectral symbol of the linear part
+    double2* d_dvrT = nullptr;   // nfft^2: exponential-DVR matrix, transposed (built on first use)
 };
 
 int orbwork_create(const OrbEq& eq, int imagtime, const double* pot_host, OrbWork** out);
@@ -59,13 +61,16 @@ int orthonormalize(OrbWork* w, double2* orb, hipStream_t st);
 int splitstep(OrbWork* w, int use_fft, double2* orb, double2* hob, double2* hint, const double2* rho2,
               const double2* rinv, hipStream_t st);
 // Full-equation Runge-Kutta orbital step (orbital_integration.c:283-348):
-// linear part by finite differences (use_fd) or FFT, simple or robust
-// projector, Gram-Schmidt in imaginary time.  rho2 and rinv are held fixed.
-int fullstep(OrbWork* w, int use_fd, int robust, double2* orb, const double2* rho2, const double2* rinv,
+// linear part by FFT (der_method 0), finite differences (1) or the
+// exponential DVR (2), simple or robust projector, Gram-Schmidt in
+// imaginary time.  rho2 and rinv are held fixed.
+int fullstep(OrbWork* w, int der_method, int robust, double2* orb, const double2* rho2, const double2* rinv,
              hipStream_t st);
 // Its right-hand side at orb (orbital_integration.c:158-235).
-int fullstep_rhs(OrbWork* w, int use_fd, int robust, const double2* orb, const double2* rho2, const double2* rinv,
-                 double2* der, hipStream_t st);
+int fullstep_rhs(OrbWork* w, int der_method, int robust, const double2* orb, const double2* rho2,
+                 const double2* rinv, double2* der, hipStream_t st);
+// Builds the DVR matrix on first use (allocates and synchronises).
+int ensure_dvr(OrbWork* w);
 // proj_k = projection of hact_k onto the orbitals (orbital_integration.c:22-129).
 int projector(OrbWork* w, int robust, const double2* orb, const double2* hact, double2* proj, hipStream_t st);
 // rho1 -> eigenvalues, conditional regularisation, inverse (M x M, one wave,

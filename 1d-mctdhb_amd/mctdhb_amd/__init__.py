@@ -119,7 +119,7 @@ def lib():
     L.mctdhb_fullstep_dev.argtypes = [_vp, _i, _i, _vp, _vp, _vp, _vp]
     L.mctdhb_fullstep_rhs_dev.argtypes = [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp]
     L.mctdhb_projector_dev.argtypes = [_vp, _i, _vp, _vp, _vp, _vp]
-    L.mctdhb_orbital_set_integrator.argtypes = [_vp, _i, _i]
+    L.mctdhb_orbital_set_integrator.argtypes = [_vp, _i, _i, _i]
     L.mctdhb_density_sync_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, _i, _vp]
     L.mctdhb_propagate_step_dev.argtypes = [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.mctdhb_step_graph_create.argtypes = [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _cpp]
@@ -322,22 +322,24 @@ class DeviceOrbital:
     def orthonormalize(self, orb, stream=None):
         check(lib().mctdhb_orthonormalize_dev(self._h, _ptr(orb), _stream(stream)))
 
-    def fullstep(self, orb, rho2, rinv, use_fd=True, robust=False, stream=None):
-        check(lib().mctdhb_fullstep_dev(self._h, int(bool(use_fd)), int(bool(robust)), _ptr(orb), _ptr(rho2),
+    def fullstep(self, orb, rho2, rinv, use_fd=1, robust=False, stream=None):
+        """use_fd: 0 FFT, 1 finite differences, 2 DVR."""
+        check(lib().mctdhb_fullstep_dev(self._h, int(use_fd), int(bool(robust)), _ptr(orb), _ptr(rho2),
                                         _ptr(rinv), _stream(stream)))
 
-    def fullstep_rhs(self, orb, rho2, rinv, der, use_fd=True, robust=False, stream=None):
-        check(lib().mctdhb_fullstep_rhs_dev(self._h, int(bool(use_fd)), int(bool(robust)), _ptr(orb), _ptr(rho2),
+    def fullstep_rhs(self, orb, rho2, rinv, der, use_fd=1, robust=False, stream=None):
+        check(lib().mctdhb_fullstep_rhs_dev(self._h, int(use_fd), int(bool(robust)), _ptr(orb), _ptr(rho2),
                                             _ptr(rinv), _ptr(der), _stream(stream)))
 
     def projector(self, orb, hact, proj, robust=False, stream=None):
         check(lib().mctdhb_projector_dev(self._h, int(bool(robust)), _ptr(orb), _ptr(hact), _ptr(proj),
                                          _stream(stream)))
 
-    def set_integrator(self, fullstep: bool, robust: bool = False):
+    def set_integrator(self, fullstep: bool, robust: bool = False, dvr: bool = False):
         """Orbital integrator of propagate_step / StepGraph: FULLSTEP_RUNGEKUTTA
-        (30) or SPLITSTEP (31)."""
-        check(lib().mctdhb_orbital_set_integrator(self._h, 30 if fullstep else 31, int(bool(robust))))
+        (30) or SPLITSTEP (31); dvr selects the DVR linear part of the full step."""
+        check(lib().mctdhb_orbital_set_integrator(self._h, 30 if fullstep else 31, 300 if dvr else 0,
+                                                  int(bool(robust))))
 
     def splitstep(self, orb, hob, hint, rho2, rinv, use_fft=True, stream=None):
         check(lib().mctdhb_splitstep_dev(self._h, int(bool(use_fft)), _ptr(orb), _ptr(hob), _ptr(hint),

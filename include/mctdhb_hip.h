@@ -113,7 +113,7 @@ void   sync_density_matrices(MultiConfiguration mc_space, ManyBodyState psi);   
 void   sync_equation_params(OrbitalEquation eq_desc);                            /* synchronize.c:58-67 */
 
 /* include/integrator/orbital_integration.h:7-37.  rk_order RK2 / RK4 / RK5;
- * derivatives FINITEDIFF or SPECTRAL (DVR is not provided).  orb_next is
+ * derivatives FINITEDIFF, SPECTRAL or DVR (full step).  orb_next is
  * scratch.  The robust projector's overlap matrix has the reference's
  * unset [0][1] / [1][0] entries (quirk Q2) as zero. */
 void set_periodic_bounds(uint16_t norb, uint16_t grid_size, Cmatrix orb);       /* orbital_integration.c:13-20 */
@@ -345,8 +345,9 @@ int mctdhb_nonlinear_rhs_dev(mctdhb_orbital* w, const void* orb, const void* hob
 int mctdhb_orthonormalize_dev(mctdhb_orbital* w, void* orb, void* stream);   /* calculus.c:170-191 */
 int mctdhb_splitstep_dev(mctdhb_orbital* w, int use_fft, void* orb, void* hob, void* hint, const void* rho2,
                          const void* rinv, void* stream);                     /* orbital_integration.c:351-434 */
-/* Full-equation Runge-Kutta orbital step: linear part by finite
- * differences (use_fd) or FFT, simple or robust projector, Gram-Schmidt in
+/* Full-equation Runge-Kutta orbital step: linear part by FFT (use_fd 0),
+ * finite differences (1) or the exponential DVR (2; its n^2 matrix is built
+ * on the GPU on first use), simple or robust projector, Gram-Schmidt in
  * imaginary time; rho2 and rinv held fixed. */
 int mctdhb_fullstep_dev(mctdhb_orbital* w, int use_fd, int robust, void* orb, const void* rho2, const void* rinv,
                         void* stream);                                        /* orbital_integration.c:283-348 */
@@ -356,9 +357,11 @@ int mctdhb_fullstep_rhs_dev(mctdhb_orbital* w, int use_fd, int robust, const voi
 int mctdhb_projector_dev(mctdhb_orbital* w, int robust, const void* orb, const void* hact, void* proj,
                          void* stream);                                       /* orbital_integration.c:22-129 */
 /* The orbital integrator of mctdhb_propagate_step_dev and the step graph:
- * orb_method SPLITSTEP (31, default) or FULLSTEP_RUNGEKUTTA (30); with the
- * full step use_fft = 0 selects finite differences. */
-int mctdhb_orbital_set_integrator(mctdhb_orbital* w, int orb_method, int robust);
+ * orb_method SPLITSTEP (31, default) or FULLSTEP_RUNGEKUTTA (30).  With the
+ * full step, der_method DVR (300) selects the DVR linear part (the matrix is
+ * built here); otherwise use_fft of the step selects FFT or finite
+ * differences.  der_method 0 means "as use_fft says". */
+int mctdhb_orbital_set_integrator(mctdhb_orbital* w, int orb_method, int der_method, int robust);
 /* rho1, rho2 of coef, then natural occupations (eigvals, M doubles, device,
  * may be NULL), regularisation and rinv = rho1^-1 (synchronize.c:41-56). */
 int mctdhb_density_sync_dev(mctdhb_space* space, const void* coef, void* rho1, void* rho2, void* rinv,

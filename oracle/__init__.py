@@ -125,6 +125,8 @@ def lib():
         L.orc_orb_interacting_part.restype = _CD
         L.orc_linear_horb_fd.argtypes = [oe, _cp, _cp]
         L.orc_linear_horb_fft.argtypes = [oe, _cp, _cp]
+        L.orc_expdvr_mat.argtypes = [oe, _cp]
+        L.orc_linear_horb_dvr.argtypes = [oe, _cp, _cp, _cp]
         L.orc_overlap_matrix.argtypes = [C.c_int, C.c_int, C.c_double, _cp, _cp]
         L.orc_multiorb_projector.argtypes = [oe, C.c_int, _cp, _cp, _cp]
         L.orc_fullstep_rhs.argtypes = [oe, C.c_int, C.c_int, _cp, _cp, _cp, _cp]

@@ -69,6 +69,62 @@ orc_linear_horb_fft(const orc_orbeq* eq, const ocplx* f, ocplx* out)
     free(bw);
 }
 
+/* Reference split_linear_orbitals.c:82-165: the exponential-DVR matrix of
+ * the derivative terms, n = grid_size - 1 rows, row-major. */
+void
+orc_expdvr_mat(const orc_orbeq* eq, ocplx* mat)
+{
+    const int    n = eq->grid_size - 1;
+    const double length = n * eq->dx, d2coef = eq->d2coef;
+    const ocplx  d1coef = eq->d1coef_re + I * eq->d1coef_im;
+    ocplx *      d1 = malloc(sizeof(ocplx) * n * n), *d2 = malloc(sizeof(ocplx) * n * n);
+    for (int i = 0; i < n; i++)
+    {
+        for (int j = i + 1; j < n; j++)
+        {
+            ocplx s1 = 0, s2 = 0;
+            for (int k = 0; k < n; k++)
+            {
+                const int   kmom = k - n / 2;
+                const ocplx e = cexp(2 * I * kmom * (j - i) * ORC_PI / n);
+                s1 += -(2 * I * ORC_PI * kmom / length) * e / n;
+                s2 += -(2 * ORC_PI * kmom / length) * (2 * ORC_PI * kmom / length) * e / n;
+            }
+            d1[i * n + j] = s1;
+            d1[j * n + i] = -conj(s1);
+            d2[i * n + j] = s2;
+            d2[j * n + i] = conj(s2);
+        }
+        ocplx s1 = 0, s2 = 0;
+        for (int k = 0; k < n; k++)
+        {
+            const int kmom = k - n / 2;
+            s1 = s1 + -(2 * I * ORC_PI * kmom / length) / n;
+            s2 = s2 + -(2 * ORC_PI * kmom / length) * (2 * ORC_PI * kmom / length) / n;
+        }
+        d1[i * n + i] = s1;
+        d2[i * n + i] = s2;
+    }
+    for (int i = 0; i < n * n; i++) mat[i] = d2coef * d2[i] + d1coef * d1[i];
+    free(d1);
+    free(d2);
+}
+
+/* Reference orbital_integration.c:159-169 (DVR branch): the matrix times
+ * the first n points, plus V, last point = first. */
+void
+orc_linear_horb_dvr(const orc_orbeq* eq, const ocplx* mat, const ocplx* f, ocplx* out)
+{
+    const int n = eq->grid_size - 1;
+    for (int j = 0; j < n; j++)
+    {
+        ocplx acc = 0;
+        for (int l = 0; l < n; l++) acc += mat[j * n + l] * f[l];
+        out[j] = acc + eq->pot[j] * f[j];
+    }
+    out[n] = out[0];
+}
+
 /* Reference split_nonlinear_orbitals.c:4-60 */
 ocplx
 orc_orb_interacting_part(int k, int n, double g, int m, int grid_size, const ocplx* orb, const ocplx* rho,
@@ -166,23 +222,32 @@ orc_multiorb_projector(const orc_orbeq* eq, int robust, const ocplx* orb, const 
 }
 
 /* Reference orbital_integration.c:158-235: der = -i time_fac (H orb - P H orb)
- * with H = linear part (finite differences when use_fd, else FFT) plus the
- * interacting part. */
+ * with H = linear part (der: 0 FFT, 1 finite differences, 2 exponential
+ * DVR) plus the interacting part. */
 void
-orc_fullstep_rhs(const orc_orbeq* eq, int use_fd, int robust, const ocplx* orb, const ocplx* rho2,
+orc_fullstep_rhs(const orc_orbeq* eq, int der_method, int robust, const ocplx* orb, const ocplx* rho2,
                  const ocplx* rinv, ocplx* der)
 {
     const int    norb = eq->norb, npts = eq->grid_size;
     const size_t n = (size_t) norb * npts;
     const ocplx  tf = eq->time_fac_re + I * eq->time_fac_im;
     ocplx *      hact = malloc(sizeof(ocplx) * n), *lin = malloc(sizeof(ocplx) * n), *proj = malloc(sizeof(ocplx) * n);
+    ocplx*       dvr = NULL;
+    if (der_method == 2)
+    {
+        dvr = malloc(sizeof(ocplx) * (npts - 1) * (npts - 1));
+        orc_expdvr_mat(eq, dvr);
+    }
     for (int k = 0; k < norb; k++)
     {
-        if (use_fd)
+        if (der_method == 1)
             orc_linear_horb_fd(eq, &orb[(size_t) k * npts], &lin[(size_t) k * npts]);
+        else if (der_method == 2)
+            orc_linear_horb_dvr(eq, dvr, &orb[(size_t) k * npts], &lin[(size_t) k * npts]);
         else
             orc_linear_horb_fft(eq, &orb[(size_t) k * npts], &lin[(size_t) k * npts]);
     }
+    free(dvr);
     for (int k = 0; k < norb; k++)
         for (int j = 0; j < npts; j++)
             hact[(size_t) k * npts + j] =
@@ -210,7 +275,7 @@ static const double FS_RK5_B[6] = {7.0 / 90, 0, 32.0 / 90, 12.0 / 90, 32.0 / 90,
  * step of the full equation with rho2 and rho1^-1 fixed, Gram-Schmidt in
  * imaginary time. */
 void
-orc_fullstep_orb(const orc_orbeq* eq, int imagtime, int use_fd, int robust, ocplx* orb, const ocplx* rho2,
+orc_fullstep_orb(const orc_orbeq* eq, int imagtime, int der_method, int robust, ocplx* orb, const ocplx* rho2,
                  const ocplx* rinv)
 {
     const int     norb = eq->norb, npts = eq->grid_size;
@@ -229,7 +294,7 @@ orc_fullstep_orb(const orc_orbeq* eq, int imagtime, int use_fd, int robust, ocpl
     {
         if (st == 0)
         {
-            orc_fullstep_rhs(eq, use_fd, robust, orb, rho2, rinv, k);
+            orc_fullstep_rhs(eq, der_method, robust, orb, rho2, rinv, k);
             continue;
         }
         for (size_t i = 0; i < n; i++)
@@ -238,7 +303,7 @@ orc_fullstep_orb(const orc_orbeq* eq, int imagtime, int use_fd, int robust, ocpl
             for (int j = 1; j < st; j++) acc += A[st * s + j] * k[j * n + i];
             y[i] = orb[i] + dt * acc;
         }
-        orc_fullstep_rhs(eq, use_fd, robust, y, rho2, rinv, k + st * n);
+        orc_fullstep_rhs(eq, der_method, robust, y, rho2, rinv, k + st * n);
     }
     for (size_t i = 0; i < n; i++)
     {

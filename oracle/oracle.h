@@ -126,14 +126,16 @@ void  orc_propagate_step(const orc_space* S, orc_orbeq* eq, int imagtime, int us
 /* fullstep.c: the full-equation RK orbital step (orbital_integration.c:22-348) */
 void  orc_linear_horb_fd(const orc_orbeq* eq, const ocplx* f, ocplx* out);
 void  orc_linear_horb_fft(const orc_orbeq* eq, const ocplx* f, ocplx* out);
+void  orc_expdvr_mat(const orc_orbeq* eq, ocplx* mat);
+void  orc_linear_horb_dvr(const orc_orbeq* eq, const ocplx* mat, const ocplx* f, ocplx* out);
 ocplx orc_orb_interacting_part(int k, int n, double g, int m, int grid_size, const ocplx* orb,
                                const ocplx* rho, const ocplx* rinv);
 void  orc_overlap_matrix(int norb, int grid, double dx, const ocplx* orb, ocplx* overlap);
 void  orc_multiorb_projector(const orc_orbeq* eq, int robust, const ocplx* orb, const ocplx* hact,
                              ocplx* project);
-void  orc_fullstep_rhs(const orc_orbeq* eq, int use_fd, int robust, const ocplx* orb, const ocplx* rho2,
+void  orc_fullstep_rhs(const orc_orbeq* eq, int der_method, int robust, const ocplx* orb, const ocplx* rho2,
                        const ocplx* rinv, ocplx* der);
-void  orc_fullstep_orb(const orc_orbeq* eq, int imagtime, int use_fd, int robust, ocplx* orb,
+void  orc_fullstep_orb(const orc_orbeq* eq, int imagtime, int der_method, int robust, ocplx* orb,
                        const ocplx* rho2, const ocplx* rinv);
 
 #endif

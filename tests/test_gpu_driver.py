@@ -204,3 +204,19 @@ def test_reference_example_integrator_full_step_reaches_benchmark(dropin, job_di
     occ = np.linalg.eigvalsh(m.get("ob_denmat"))
     assert abs(occ[-1] / 4 - 0.97) < 0.012
     m.close()
+
+
+def test_reference_default_integrator_full_step_dvr(dropin, job_dirs, tmp_path):
+    """No integrator file: the reference's defaults (mctdhb_types.h:93-99:
+    imaginary time, Lanczos, full-equation RK5 with the DVR linear part,
+    periodic, 5 Lanczos vectors) reach benchmark.dat:23 (N=4, M=3, g=1)."""
+    print("driver defaults (DVR)", flush=True)
+    line = "4 3 129 -8.0 8.0 0.005 12.0 -0.5 0.0 linear_ramp 1.0 1.0 0.0 0.0 0.0 harmonic 1.0 1.0 0.0 0.0 0.0"
+    inp, out, conf = _write_inputs(job_dirs, "df", line, 4, 3, 129, -8.0, 8.0)
+    dropin.set_directories(str(inp), str(out), str(tmp_path / "no_such_integrator.conf"))
+    m = dropin.MCTDHB.from_files("df")
+    m.run("df_job1", rec_nsteps=10, monitor_rate=400)
+    m.append_parameters("df")
+    par = np.loadtxt(out / "df_mctdhb_parameters.dat")
+    assert abs(par[10] - 0.9685859) < 5e-4, par[10]
+    m.close()

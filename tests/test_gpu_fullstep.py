@@ -36,7 +36,7 @@ def _setup(amd, M, grid, N=4, seed=3, bounds=1001, imagtime=False, rk=4, L=10.0,
 
 
 @pytest.mark.parametrize("M,grid", [(3, 129), (9, 65), (4, 101)])
-@pytest.mark.parametrize("use_fd", [1, 0])
+@pytest.mark.parametrize("use_fd", [1, 0, 2])  # finite differences, FFT, DVR
 @pytest.mark.parametrize("robust", [0, 1])
 def test_fullstep_rhs_matches_oracle(amd, M, grid, use_fd, robust):
     print(f"fullstep rhs M={M} grid={grid} fd={use_fd} robust={robust}", flush=True)
@@ -64,7 +64,7 @@ def test_projector_matches_oracle(amd, robust):
 
 @pytest.mark.parametrize("rk", [2, 4, 5])
 @pytest.mark.parametrize("imagtime", [False, True])
-@pytest.mark.parametrize("use_fd,bounds", [(1, 1001), (0, 1001), (1, 1000)])
+@pytest.mark.parametrize("use_fd,bounds", [(1, 1001), (0, 1001), (1, 1000), (2, 1001)])
 def test_fullstep_matches_oracle(amd, rk, imagtime, use_fd, bounds):
     print(f"fullstep rk={rk} imag={imagtime} fd={use_fd} bounds={bounds}", flush=True)
     M, grid = 3, 129

@@ -74,6 +74,32 @@ def _simpson_weights(grid, dx):
     return np.array([oracle.csimps(np.eye(grid)[p].astype(complex), dx).real for p in range(grid)])
 
 
+def test_dvr_linear_part_plane_wave():
+    """The exponential DVR (split_linear_orbitals.c:82-165) is exact on the
+    grid's plane waves: symbol i k d1 - k^2 d2 (+ V), as the FFT form."""
+    grid = 49
+    n = grid - 1
+    x, eq = _eq(1, grid, pot=np.full(grid, 0.4), d1=0.3 + 0.1j)
+    dx = x[1] - x[0]
+    mat = np.zeros(n * n, complex)
+    oracle.lib().orc_expdvr_mat(eq, mat)
+    D1 = mat.reshape(n, n).copy()
+    for m in (0, 1, -5, 11):
+        kk = 2 * np.pi * m / (n * dx)
+        f = np.exp(1j * kk * (x - x[0]))
+        out = np.zeros(grid, complex)
+        oracle.lib().orc_linear_horb_dvr(eq, mat, f, out)
+        lam = 1j * kk * (0.3 + 0.1j) + 0.5 * kk ** 2 + 0.4
+        assert np.allclose(out[:n], lam * f[:n], atol=1e-10 * max(1, abs(lam)))
+        assert out[n] == out[0]
+    # without d1 the matrix is Hermitian (the second derivative); with it, not
+    x, eq = _eq(1, grid, d1=0.0j)
+    oracle.lib().orc_expdvr_mat(eq, mat)
+    D = mat.reshape(n, n)
+    assert np.allclose(D, D.conj().T, atol=1e-12 * np.max(abs(D)))
+    assert not np.allclose(D1, D1.conj().T)
+
+
 def _state(norb, npar, grid, seed, L=10.0):
     """Orbitals orthonormal in the Simpson inner product (classical
     Gram-Schmidt in numpy: the reference's point-by-point variant leaves
@@ -164,7 +190,7 @@ def test_projectors_are_the_orthogonal_projection():
     assert np.allclose(out.reshape(3, grid), ref, atol=1e-11)
 
 
-@pytest.mark.parametrize("use_fd", [1, 0])
+@pytest.mark.parametrize("use_fd", [1, 0, 2])
 @pytest.mark.parametrize("robust", [0, 1])
 def test_rhs_is_orthogonal_to_the_orbitals(use_fd, robust):
     """der_k = -i(H f_k - P H f_k) has no component along the orbitals."""
