@@ -228,6 +228,15 @@ extern "C" int mctdhb_sil_destroy(mctdhb_sil* s) {
     return kOk;
 }
 
+extern "C" int mctdhb_sil_set_small(mctdhb_sil* s, int on) {
+    if (!s) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    s->L->small = on != 0;
+    return kOk;
+}
+
 extern "C" size_t mctdhb_sil_device_bytes(const mctdhb_space* sp, int lanczos_iter, int prec) {
     return sp ? sil_device_bytes(sp->S, lanczos_iter, prec) : 0;
 }

@@ -415,6 +415,18 @@ int hc_apply_cf(const Space* S, const void* coef, const void* cf, void* hc, int 
     return dispatch_fused_plain(S, coef, blk, hc, prec, index, st);
 }
 
+int hc_sil_small(const Space* S, const double2* cf64, const SmallSilArgs& a, int index, hipStream_t st) {
+    switch (S->norb) {
+        case 2: return HcFused<2>::sil_small(S, cf64, a, index, st);
+        case 3: return HcFused<3>::sil_small(S, cf64, a, index, st);
+        case 4: return HcFused<4>::sil_small(S, cf64, a, index, st);
+        case 5: return HcFused<5>::sil_small(S, cf64, a, index, st);
+        case 6: return HcFused<6>::sil_small(S, cf64, a, index, st);
+        case 7: return HcFused<7>::sil_small(S, cf64, a, index, st);
+        default: set_error("the one-workgroup SIL step covers 2 <= norb <= 7"); return kInvalidArgument;
+    }
+}
+
 int hc_lanczos(const Space* S, const double2* cf64, const float2* cf32, const LanczosHcArgs& a, int prec,
                int index, hipStream_t st) {
     int rc = check_mode(S, prec, index);

@@ -105,6 +105,36 @@ inline bool hc_lanczos_half(const Space* S, int prec) {
 // the space's padded staging (allocated on first use); returns element 0.
 const float2* hc_padded_copy(const Space* S, const float2* c, hipStream_t st);
 
+// The whole FP64 SIL step in one workgroup (sil_small.hpp): w[0] is C
+// (updated in place), w[1..lm-1] Krylov storage, out one more vector.
+constexpr uint32_t kSmallSilMaxDim = 8192;
+constexpr int kSmallSilBlock = 512;
+// LDS of that kernel after the W table and the sqrt LUT.
+struct SmallSilLds {
+    double red[(2 * kMaxLanczos + 2) * (kSmallSilBlock / 64)];
+    double tot[2 * kMaxLanczos + 4];
+    double z[64 * (kMaxLanczos + 1)];  // every lane of wave 0 owns a row
+    double2 cre[kMaxLanczos];
+    double2* w[kMaxLanczos];  // the vectors, indexed at run time
+};
+inline size_t sil_small_smem(const Space* S) {
+    const size_t wsz = (size_t) (S->norb - 1) * S->ldw * 4;
+    return ((wsz + 15) & ~(size_t) 15) + ((sizeof(double) * (S->npar + 3) + 15) & ~(size_t) 15) + sizeof(SmallSilLds);
+}
+inline bool sil_small_fits(const Space* S, int lm) {
+    return S->norb >= 2 && S->norb <= 7 && S->dim <= kSmallSilMaxDim && lm <= kMaxLanczos &&
+           sil_small_smem(S) <= 64 * 1024;
+}
+struct SmallSilArgs {
+    SilState* st;
+    void* w[kMaxLanczos];
+    double2* out;
+    int lm;
+    double dt_re, dt_im;
+    int imagtime;
+};
+int hc_sil_small(const Space* S, const double2* cf64, const SmallSilArgs& a, int index, hipStream_t st);
+
 // Per-orbital-count entry points of the fused kernel (hc_m<M>.hip).  The
 // mixed form always uses on-the-fly indices.
 template <int M> struct HcFused {
@@ -112,6 +142,7 @@ template <int M> struct HcFused {
     // mixed form with an explicit tiling (0 per lane, 1 pairs; c padded for 1)
     static int plain_tiled(const Space* S, const void* c, const void* cf, void* out, int tiling, hipStream_t st);
     static int lanczos(const Space* S, const void* cf, const LanczosHcArgs& a, int prec, int index, hipStream_t st);
+    static int sil_small(const Space* S, const void* cf, const SmallSilArgs& a, int index, hipStream_t st);
 };
 
 }  // namespace mctdhb

@@ -15,6 +15,7 @@
 #pragma once
 
 #include "hamiltonian_impl.hpp"
+#include "sil_small.hpp"
 
 namespace mctdhb {
 

@@ -10,6 +10,7 @@ struct Sil {
     const Space* S = nullptr;
     int lm = 0, prec = 0, index = 0;
     bool generic = false;
+    bool small = true;  // small spaces take the one-workgroup step (mctdhb_sil_set_small)
     SilState* d_st = nullptr;
     double2* cf64 = nullptr;  // fused coefficient block, or [hob | hint] for the generic kernel
     float2* cf32 = nullptr;

@@ -98,6 +98,7 @@ def lib():
     L.mctdhb_sil_device_bytes.argtypes = [_vp, _i, _i]
     L.mctdhb_sil_device_bytes.restype = C.c_size_t
     L.mctdhb_sil_step_dev.argtypes = [_vp, _vp, _vp, _vp, C.c_double, C.c_double, _i, _vp]
+    L.mctdhb_sil_set_small.argtypes = [_vp, _i]
     L.mctdhb_lanczos_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
     L.mctdhb_sil_scalars.argtypes = [_vp, C.POINTER(_i), _vp, _vp, _vp]
     L.mctdhb_lowest_state_dev.argtypes = [_vp, _vp, _vp, _vp, C.POINTER(C.c_double), _vp]
@@ -248,6 +249,10 @@ class DeviceSil:
             self._h = None
 
     __del__ = close
+
+    def set_small(self, on: bool):
+        """One-workgroup steps for small FP64 spaces (default on)."""
+        check(lib().mctdhb_sil_set_small(self._h, int(bool(on))))
 
     def step(self, coef, hob, hint, prop_dt: complex, imagtime: bool, stream=None):
         check(lib().mctdhb_sil_step_dev(self._h, _ptr(coef), _ptr(hob), _ptr(hint), float(np.real(prop_dt)),

@@ -289,6 +289,10 @@ int mctdhb_density_dev(mctdhb_space* space, const void* coef, void* rho1, void* 
 int mctdhb_sil_create(const mctdhb_space* space, int lanczos_iter, int prec, int index,
                       mctdhb_sil** out);
 int mctdhb_sil_destroy(mctdhb_sil* sil);
+/* FP64 steps of spaces up to 8192 configurations, 2 <= norb <= 7 and at most
+ * 15 Krylov vectors run as one workgroup (the whole recursion in one
+ * kernel); on = 0 keeps them on the multi-kernel path.  Default on. */
+int mctdhb_sil_set_small(mctdhb_sil* sil, int on);
 size_t mctdhb_sil_device_bytes(const mctdhb_space* space, int lanczos_iter, int prec);
 /* coef (complex128, device) <- exp(-i H prop_dt) coef; renormalised when
  * imagtime != 0 (prop_dt = -i dt in imaginary time, dt in real time). */

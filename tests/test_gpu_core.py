@@ -153,6 +153,35 @@ def test_sil_step_matches_oracle(amd, orc, npar, norb, lm, imag):
     assert gu.rel_err(gu.host(dc), ref) < 1e-10
 
 
+@pytest.mark.parametrize("npar,norb,lm,index", [(10, 4, 5, 1), (10, 4, 5, 0), (2, 2, 5, 1), (30, 2, 15, 1),
+                                                (6, 7, 8, 1), (12, 5, 6, 0), (3, 3, 12, 1)])
+@pytest.mark.parametrize("imag", [False, True])
+def test_one_workgroup_sil_step_matches_multikernel_and_oracle(amd, orc, npar, norb, lm, index, imag):
+    """Small FP64 spaces take the one-workgroup step (mctdhb_sil_set_small);
+    several steps agree with the multi-kernel step and with the oracle,
+    including Krylov breakdown (N=2, M=2: three configurations, 5 vectors)
+    and the table-driven indices."""
+    print(f"small sil N={npar} M={norb} lm={lm} index={index} imag={imag}", flush=True)
+    S = orc.Space(npar, norb)
+    c, h, g = _problem(npar, norb, 41)
+    dt = -0.01j if imag else 0.01
+    ref = c
+    for _ in range(3):
+        _, ref = S.sil_step(ref, h, g, lm, dt, imag)
+    D = amd.DeviceSpace(npar, norb, tables=True)
+    small = amd.DeviceSil(D, lm, prec=amd.FP64, index=index)
+    multi = amd.DeviceSil(D, lm, prec=amd.FP64, index=index)
+    multi.set_small(False)
+    dh, dg = gu.dev(h), gu.dev(g)
+    a, b = gu.dev(c), gu.dev(c)
+    for _ in range(3):
+        small.step(a, dh, dg, dt, imag)
+        multi.step(b, dh, dg, dt, imag)
+    assert small.scalars()[0] == multi.scalars()[0]  # Krylov vectors used
+    assert gu.rel_err(gu.host(a), gu.host(b)) < 1e-12
+    assert gu.rel_err(gu.host(a), ref) < 1e-10
+
+
 @pytest.mark.parametrize("npar,norb,lm,prec", [(6, 4, 8, 0), (10, 5, 12, 0), (10, 5, 12, 1), (3, 9, 6, 0),
                                                (12, 5, 40, 0), (12, 5, 24, 1)])
 def test_lowest_state_matches_oracle(amd, orc, npar, norb, lm, prec):
