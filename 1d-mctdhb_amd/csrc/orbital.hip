@@ -413,9 +413,11 @@ __global__ __launch_bounds__(256) void k_nonlinear(const double2* __restrict__ o
 __global__ __launch_bounds__(256) void k_nl_coef(int M, const double2* __restrict__ hob,
                                                  const double2* __restrict__ hint, const double2* __restrict__ rho,
                                                  const double2* __restrict__ rinv, double2* __restrict__ out) {
+    extern __shared__ double2 lpart[];  // M^2 P partial sums of L (M <= 8)
     const int mm = M * M, mmm = mm * M, P = M * (M + 1) / 2, nA = M * P * M;
-    // without hob (the full-equation step) only the A table is needed
-    for (int e = threadIdx.x; e < nA + (hob ? mm : 0); e += blockDim.x) {
+    const bool split = M <= 8;
+    // A entries; without hob (the full-equation step) only the A table is needed
+    for (int e = threadIdx.x; e < nA + (hob && !split ? mm : 0); e += blockDim.x) {
         if (e < nA) {
             const int s = e % M, ql = (e / M) % P, k = e / (M * P);
             int q = 0, rem = ql;
@@ -444,7 +446,38 @@ __global__ __launch_bounds__(256) void k_nl_coef(int M, const double2* __restric
             out[e] = acc;
         }
     }
+    if (!hob || !split) return;
+    // L with M <= 8: thread (k, j, ql) sums the terms of one pair q <= l over
+    // a (and t > a), then thread (k, j) adds the pairs in order
+    for (int e = threadIdx.x; e < mm * P; e += blockDim.x) {
+        const int ql = e % P, j = (e / P) % M, k = e / (P * M);
+        int q = 0, rem = ql;
+        while (rem >= M - q) rem -= M - q++;
+        const int l = q + rem;
+        const double c = q == l ? 1.0 : 2.0;
+        double2 acc = make_double2(0, 0);
+        for (int a = 0; a < M; a++) {
+            const double2 rka = cscale(rinv[k * M + a], c);
+            acc = acc + cmul(cmul(rka, rho[a + M * a + mm * q + mmm * l]), hint[j + a * M + l * mm + q * mmm]);
+            for (int t = a + 1; t < M; t++) {
+                const double2 r = rho[a + M * t + mm * q + mmm * l];
+                acc = acc + cmul(cmul(rka, r), hint[j + t * M + l * mm + q * mmm]);
+                acc = acc + cmul(cmul(cscale(rinv[k * M + t], c), r), hint[j + a * M + l * mm + q * mmm]);
+            }
+        }
+        lpart[e] = acc;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < mm; e += blockDim.x) {
+        const int k = e / M, j = e % M;
+        double2 acc = hob[j * M + k];
+        for (int ql = 0; ql < P; ql++) acc = acc + lpart[(k * M + j) * P + ql];
+        out[nA + e] = acc;
+    }
 }
+
+// LDS of k_nl_coef (the split L sums for M <= 8).
+static size_t nl_coef_smem(int M) { return M <= 8 ? sizeof(double2) * M * M * M * (M + 1) / 2 : 0; }
 
 // out = y0 + dt (a_0 k_0 + a_1 k_1 + ...), the stage sums of an explicit
 // tableau in ascending stage order.
@@ -510,7 +543,7 @@ constexpr int kNlFusedMaxOrb = 8;
 static int nonlinear_prepare(OrbWork* w, const double2* hob, const double2* hint, const double2* rho2,
                              const double2* rinv, hipStream_t st) {
     if (w->eq.norb > kNlFusedMaxOrb) return kOk;
-    k_nl_coef<<<1, 256, 0, st>>>(w->eq.norb, hob, hint, rho2, rinv, w->d_nl);
+    k_nl_coef<<<1, 256, nl_coef_smem(w->eq.norb), st>>>(w->eq.norb, hob, hint, rho2, rinv, w->d_nl);
     MCTDHB_HIP_CHECK(hipGetLastError());
     return kOk;
 }
@@ -593,6 +626,122 @@ constexpr double kRk5A[6 * 6] = {
     -3.0 / 7, 2.0 / 7, 12.0 / 7, -12.0 / 7, 8.0 / 7, 0,
 };
 constexpr double kRk5B[6] = {7.0 / 90, 0, 32.0 / 90, 12.0 / 90, 32.0 / 90, 7.0 / 90};
+
+// The whole nonlinear Runge-Kutta sub-step at one grid point: the
+// right-hand side at x involves only the orbitals at x (the A and L tables
+// of k_nl_coef, V(x)), so the stage values stay on chip and one pass
+// replaces the stage kernels.  Stage input and final sum as k_rk_sum forms
+// them.
+struct RkTab {
+    double a[6 * 6], b[6];
+};
+
+// Block: 64 grid points x M orbitals; thread (k, x) forms orbital k's
+// right-hand side at x, the stage values of all orbitals at x are shared in
+// LDS between stages.
+template <int M, int NS>
+__global__ __launch_bounds__(64 * M) void k_nl_rk_point(double2* __restrict__ orb, int n, RkTab tab, double dt,
+                                                        double g, double2 tf, const double* __restrict__ V,
+                                                        const double2* __restrict__ coef) {
+    constexpr int P = M * (M + 1) / 2;
+    __shared__ double2 y0[M][64], ks[NS][M][64];
+    // k is uniform across the wave: the table reads below become scalar loads
+    const int xl = threadIdx.x & 63, k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int x = blockIdx.x * 64 + xl;
+    const bool in = x < n;
+    y0[k][xl] = in ? orb[(size_t) k * n + x] : make_double2(0, 0);
+    const double vx = in ? V[x] : 0.0;
+    const double2 mi_tf = make_double2(tf.y, -tf.x);
+    const double2* A = coef + (size_t) k * P * M;
+    const double2* Lk = coef + (size_t) M * P * M + k * M;
+    __syncthreads();
+#pragma unroll 1  // stages in a loop: the table loads are not hoisted across stages
+    for (int s = 0; s < NS; s++) {
+        double2 f[M];
+#pragma unroll
+        for (int a = 0; a < M; a++) {
+            if (s == 0) {
+                f[a] = y0[a][xl];
+            } else {
+                double2 acc = cscale(ks[0][a][xl], tab.a[s * NS]);
+#pragma unroll
+                for (int j = 1; j < s; j++) acc = acc + cscale(ks[j][a][xl], tab.a[s * NS + j]);
+                f[a] = y0[a][xl] + cscale(acc, dt);
+            }
+        }
+        double2 acc[M];
+#pragma unroll
+        for (int t = 0; t < M; t++) acc[t] = make_double2(0, 0);
+        int ql = 0;
+#pragma unroll
+        for (int q = 0; q < M; q++)
+#pragma unroll
+            for (int l = q; l < M; l++, ql++) {
+                const double2 pql = cmul(f[q], f[l]);
+#pragma unroll
+                for (int t = 0; t < M; t++) acc[t] = acc[t] + cmul(A[ql * M + t], pql);
+            }
+        double2 cubic = make_double2(0, 0), fk = f[0];
+#pragma unroll
+        for (int t = 0; t < M; t++) {
+            cubic = cubic + cconjmul(f[t], acc[t]);
+            if (t == k) fk = f[t];  // constant indices: f stays in registers
+        }
+        double2 total = cscale(cubic, g) + cscale(fk, vx);
+#pragma unroll
+        for (int j = 0; j < M; j++) total = total - cmul(Lk[j], f[j]);
+        ks[s][k][xl] = cmul(mi_tf, total);
+        __syncthreads();
+    }
+    if (!in) return;
+    double2 acc = cscale(ks[0][k][xl], tab.b[0]);
+#pragma unroll
+    for (int j = 1; j < NS; j++) acc = acc + cscale(ks[j][k][xl], tab.b[j]);
+    orb[(size_t) k * n + x] = y0[k][xl] + cscale(acc, dt);
+}
+
+constexpr double kRk4A_[4 * 4] = {0, 0, 0, 0, 0.5, 0, 0, 0, 0, 0.5, 0, 0, 0, 0, 1, 0};
+constexpr double kRk4B_[4] = {1.0 / 6, 1.0 / 3, 1.0 / 3, 1.0 / 6};
+
+// Pointwise RK sub-step for 1 <= norb <= 8 (k_nl_rk_point); returns
+// kInvalidArgument (no error message) when the orbital count is not covered.
+static int nl_rk_pointwise(OrbWork* w, double2* orb, const double2* hob, const double2* hint, const double2* rho2,
+                           const double2* rinv, hipStream_t st) {
+    const OrbEq& q = w->eq;
+    const int M = q.norb, n = q.grid_size;
+    if (M > 8) return kInvalidArgument;
+    RkTab tab{};
+    int ns = 4;
+    const double *A = kRk4A_, *b = kRk4B_;
+    if (w->rk_order == 2) ns = 2, A = kRk2A, b = kRk2B;
+    if (w->rk_order == 5) ns = 6, A = kRk5A, b = kRk5B;
+    for (int i = 0; i < ns * ns; i++) tab.a[i] = A[i];
+    for (int i = 0; i < ns; i++) tab.b[i] = b[i];
+    int rc = nonlinear_prepare(w, hob, hint, rho2, rinv, st);
+    if (rc) return rc;
+    const unsigned nb = div_up(n, 64);
+#define RKP(m, s) k_nl_rk_point<m, s><<<nb, 64 * m, 0, st>>>(orb, n, tab, q.tstep, q.g, q.time_fac, w->d_pot, w->d_nl)
+#define RKP_M(m)                     \
+    case m:                          \
+        if (ns == 2) RKP(m, 2);      \
+        else if (ns == 4) RKP(m, 4); \
+        else RKP(m, 6);              \
+        break;
+    switch (M) {
+        RKP_M(1)
+        RKP_M(2)
+        RKP_M(3)
+        RKP_M(4)
+        RKP_M(5)
+        RKP_M(6)
+        RKP_M(7)
+        RKP_M(8)
+    }
+#undef RKP_M
+#undef RKP
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
 
 static int rk_tableau(OrbWork* w, int ns, const double* A, const double* b, double2* orb, const double2* hob,
                       const double2* hint, const double2* rho2, const double2* rinv, hipStream_t st) {
@@ -1148,7 +1297,9 @@ int splitstep(OrbWork* w, int use_fft, double2* orb, double2* hob, double2* hint
     if (rc) return rc;
     rc = orbital_matrices(w, orb, hob, hint, st);
     if (rc) return rc;
-    if (w->rk_order == 2) {
+    if (q.norb <= 8) {
+        rc = nl_rk_pointwise(w, orb, hob, hint, rho2, rinv, st);
+    } else if (w->rk_order == 2) {
         rc = rk_tableau(w, 2, kRk2A, kRk2B, orb, hob, hint, rho2, rinv, st);
     } else if (w->rk_order == 5) {
         rc = rk_tableau(w, 6, kRk5A, kRk5B, orb, hob, hint, rho2, rinv, st);
