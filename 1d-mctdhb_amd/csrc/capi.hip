@@ -496,10 +496,16 @@ extern "C" int mctdhb_projector_dev(mctdhb_orbital* o, int robust, const void* o
 }
 
 // The orbital part of a whole step: the integrator set on the workspace.
+// periodic_tail: end with the periodic copy of the first grid point onto
+// the last (driver.c:179-182), folded into the split step's last kernel.
 static int orbital_step(OrbWork* w, int use_fft, double2* orb, double2* hob, double2* hint, const double2* rho2,
-                        const double2* rinv, hipStream_t st) {
-    if (w->fullstep) return fullstep(w, w->dvr ? 2 : (use_fft ? 0 : 1), w->robust, orb, rho2, rinv, st);
-    return splitstep(w, use_fft, orb, hob, hint, rho2, rinv, st);
+                        const double2* rinv, hipStream_t st, bool periodic_tail) {
+    if (w->fullstep) {
+        const int rc = fullstep(w, w->dvr ? 2 : (use_fft ? 0 : 1), w->robust, orb, rho2, rinv, st);
+        if (rc || !periodic_tail || w->eq.bounds != kPeriodicBounds) return rc;
+        return periodic_bounds(w, orb, st);
+    }
+    return splitstep(w, use_fft, orb, hob, hint, rho2, rinv, st, periodic_tail);
 }
 
 extern "C" int mctdhb_density_sync_dev(mctdhb_space* sp, const void* coef, void* rho1, void* rho2, void* rinv,
@@ -508,10 +514,14 @@ extern "C" int mctdhb_density_sync_dev(mctdhb_space* sp, const void* coef, void*
         set_error("null argument");
         return kInvalidArgument;
     }
-    int rc = mctdhb_density_dev(sp, coef, rho1, rho2, prec, kIndexOnTheFly, stream);
-    if (rc) return rc;
-    return density_sync_small(sp->S->norb, (double2*) rho1, (double2*) rinv, (double*) eigvals, reg_factor,
-                              (hipStream_t) stream);
+    if (prec != kDmFP64 && prec != kDmMixed) {
+        set_error("unknown precision mode");
+        return kInvalidArgument;
+    }
+    if (!sp->dm_work) MCTDHB_HIP_CHECK(hipMalloc(&sp->dm_work, dm_workspace_bytes(sp->S)));
+    const DmSync sy{(double2*) rinv, (double*) eigvals, reg_factor};
+    return density_matrices(sp->S, coef, (double2*) rho1, (double2*) rho2, prec, kIndexOnTheFly, sp->dm_work,
+                            (hipStream_t) stream, &sy);
 }
 
 extern "C" int mctdhb_propagate_step_dev(mctdhb_space* sp, mctdhb_sil* s, mctdhb_orbital* o, int use_fft,
@@ -523,11 +533,11 @@ extern "C" int mctdhb_propagate_step_dev(mctdhb_space* sp, mctdhb_sil* s, mctdhb
     }
     hipStream_t st = (hipStream_t) stream;
     OrbWork* w = o->w;
+    // driver.c:179-186: orbital step and periodic bounds, then h and g
+    // from the new orbitals
     int rc = orbital_step(w, use_fft, (double2*) orb, (double2*) hob, (double2*) hint, (const double2*) rho2,
-                          (const double2*) rinv, st);
+                          (const double2*) rinv, st, true);
     if (rc) return rc;
-    // driver.c:179-186: periodic bounds, then h and g from the new orbitals
-    if (w->eq.bounds == kPeriodicBounds && (rc = periodic_bounds(w, (double2*) orb, st))) return rc;
     rc = orbital_matrices(w, (const double2*) orb, (double2*) hob, (double2*) hint, st);
     if (rc) return rc;
     if (sp->S->norb == 1) return kOk;
@@ -1458,7 +1468,7 @@ static void orbital_step_from_host(MCTDHBDataStruct m, DevOrbital* d) {
     orbitals_h2d(d->d_orb, psi->orbitals, M, psi->grid_size);
     h2d(d->d_rho2, psi->tb_denmat, sizeof(dcomplex) * M * M * M * M);
     cmatrix_h2d(d->d_rinv, psi->inv_ob_denmat, M);
-    if (orbital_step(d->o->w, use_fft, d->d_orb, d->d_hob, d->d_hint, d->d_rho2, d->d_rinv, nullptr))
+    if (orbital_step(d->o->w, use_fft, d->d_orb, d->d_hob, d->d_hint, d->d_rho2, d->d_rinv, nullptr, false))
         fail_stop("orbital step");
 }
 
@@ -1700,7 +1710,9 @@ struct RunState {
     int step(uint32_t curr_step, bool params, hipStream_t st) {
         OrbitalEquation eq = m->orb_eq;
         const int M = m->state->norb;
-        int rc = orbital_step(d->o->w, use_fft, d->d_orb, d->d_hob, d->d_hint, d->d_rho2, d->d_rinv, st);
+        // the periodic copy (driver.c:179-182) is folded into the orbital
+        // step; the trap and g below do not touch the orbitals
+        int rc = orbital_step(d->o->w, use_fft, d->d_orb, d->d_hob, d->d_hint, d->d_rho2, d->d_rinv, st, true);
         if (rc) return rc;
         if (params) {
             eq->t = (curr_step + 1) * eq->tstep;
@@ -1709,7 +1721,6 @@ struct RunState {
                 (rc = mctdhb_orbital_set_interaction(d->o, eq->g)))
                 return rc;
         }
-        if (eq->bounds == PERIODIC_BOUNDS && (rc = periodic_bounds(d->o->w, d->d_orb, st))) return rc;
         if ((rc = mctdhb_orbital_matrices_dev(d->o, d->d_orb, d->d_hob, d->d_hint, st))) return rc;
         if (M == 1) return kOk;
         const double2 dt = d->o->w->eq.prop_dt;

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "density_impl.hpp"
+#include "mxm_wave.hpp"
 
 namespace mctdhb {
 
@@ -160,8 +161,8 @@ __device__ __forceinline__ double2 dconj(double2 a) { return make_double2(a.x, -
 // over the threads of one block with no ordering: work item t is a diagonal
 // index k (t < M), an orbital pair k < l (Rules 1-5, 7 and 3), a (k, q < l)
 // triple of Rule 6 or a (k < s, q < l) quadruple of Rule 8.
-__global__ __launch_bounds__(256) void k_dm_assemble(int M, const double2* __restrict__ tot,
-                                                     double2* __restrict__ rho1, double2* __restrict__ rho2) {
+__device__ __forceinline__ void dm_assemble_block(int M, const double2* __restrict__ tot, double2* __restrict__ rho1,
+                                                  double2* __restrict__ rho2) {
     const DmLayout L{M};
     const int M2 = M * M, M3 = M2 * M, np = c2(M);
     auto R2 = [&](int a, int b, int cc, int d) -> double2& { return rho2[a + b * M + cc * M2 + d * M3]; };
@@ -239,6 +240,33 @@ __global__ __launch_bounds__(256) void k_dm_assemble(int M, const double2* __res
     }
 }
 
+__global__ __launch_bounds__(256) void k_dm_assemble(int M, const double2* __restrict__ tot,
+                                                     double2* __restrict__ rho1, double2* __restrict__ rho2) {
+    dm_assemble_block(M, tot, rho1, rho2);
+}
+
+// With fill set (few configuration blocks): the second-stage sums (each
+// element by one thread, over the blocks in order) and the fill; then, with
+// sy.rinv set, the density synchronisation of rho1, all in one block.
+__global__ __launch_bounds__(256) void k_dm_finish(int M, const double2* __restrict__ part, int nblocks, int E,
+                                                   double2* __restrict__ tot, double2* __restrict__ rho1,
+                                                   double2* __restrict__ rho2, DmSync sy, int fill) {
+    extern __shared__ double2 work[];  // 4 M^2 for the synchronisation
+    if (fill) {
+        for (int e = threadIdx.x; e < E; e += blockDim.x) {
+            double2 v = make_double2(0, 0);
+            for (int b = 0; b < nblocks; b++) v = v + part[(size_t) b * E + e];
+            tot[e] = v;
+        }
+        __syncthreads();
+        dm_assemble_block(M, tot, rho1, rho2);
+    }
+    if (sy.rinv) {
+        __syncthreads();
+        density_sync_body(M, rho1, sy.rinv, sy.eig, sy.reg, work);
+    }
+}
+
 static int dispatch_dm(const Space* S, const void* c, int prec, int index, double2* part, unsigned nb,
                        hipStream_t st) {
     switch (S->norb) {
@@ -259,7 +287,11 @@ size_t dm_workspace_bytes(const Space* S) {
 }
 
 int density_matrices(const Space* S, const void* coef, double2* rho1, double2* rho2, int prec, int index,
-                     void* work, hipStream_t st) {
+                     void* work, hipStream_t st, const DmSync* sync) {
+    if (sync && sync->rinv && S->norb > 30) {
+        set_error("norb exceeds 30");
+        return kInvalidArgument;
+    }
     const int M = S->norb;
     const bool generic = M == 1 || M > 7;
     if (prec == kDmMixed && generic) {
@@ -289,9 +321,20 @@ int density_matrices(const Space* S, const void* coef, double2* rho1, double2* r
         rc = dispatch_dm(S, coef, prec, index, part, nb, st);
     }
     if (rc) return rc;
+    const DmSync sy = sync ? *sync : DmSync{nullptr, nullptr, 0.0};
+    if (nb <= kDmFinishBlocks) {
+        k_dm_finish<<<1, 256, sy.rinv ? sizeof(double2) * 4 * M * M : 0, st>>>(M, part, (int) nb, E, tot, rho1, rho2,
+                                                                               sy, 1);
+        MCTDHB_HIP_CHECK(hipGetLastError());
+        return kOk;
+    }
     k_dm_reduce<<<E, 256, 0, st>>>(part, (int) nb, E, tot);
     k_dm_assemble<<<1, 256, 0, st>>>(M, tot, rho1, rho2);
     MCTDHB_HIP_CHECK(hipGetLastError());
+    if (sy.rinv) {
+        k_dm_finish<<<1, 256, sizeof(double2) * 4 * M * M, st>>>(M, nullptr, 0, 0, nullptr, rho1, nullptr, sy, 0);
+        MCTDHB_HIP_CHECK(hipGetLastError());
+    }
     return kOk;
 }
 

@@ -20,7 +20,17 @@ template <int M> struct DmFused {
     static int run(const Space* S, const void* c, int prec, int index, double2* part, unsigned nb, hipStream_t st);
 };
 
+// With sync->rinv set, also the synchronisation of rho1 (eigenvalues into
+// sync->eig when set, regularisation by sync->reg, rinv = rho1^-1), as the
+// standalone density_sync_small does.
+struct DmSync {
+    double2* rinv;
+    double* eig;
+    double reg;
+};
+// Up to this many configuration blocks, one block finishes the reduction.
+constexpr unsigned kDmFinishBlocks = 64;
 int density_matrices(const Space* S, const void* coef, double2* rho1, double2* rho2, int prec, int index,
-                     void* work, hipStream_t st);
+                     void* work, hipStream_t st, const DmSync* sync = nullptr);
 
 }  // namespace mctdhb

@@ -113,13 +113,20 @@ constexpr int kSmallSilBlock = 512;
 struct SmallSilLds {
     double red[(2 * kMaxLanczos + 2) * (kSmallSilBlock / 64)];
     double tot[2 * kMaxLanczos + 4];
-    double z[64 * (kMaxLanczos + 1)];  // every lane of wave 0 owns a row
+    double z[64 * (kMaxLanczos + 2)];  // every lane of wave 0 owns a row (odd stride: no bank conflicts)
     double2 cre[kMaxLanczos];
     double2* w[kMaxLanczos];  // the vectors, indexed at run time
+    SilState state;           // the scalars during the step, copied out after the eigensolve
 };
 inline size_t sil_small_smem(const Space* S) {
     const size_t wsz = (size_t) (S->norb - 1) * S->ldw * 4;
     return ((wsz + 15) & ~(size_t) 15) + ((sizeof(double) * (S->npar + 3) + 15) & ~(size_t) 15) + sizeof(SmallSilLds);
+}
+// The staged form keeps the current vector and the occupation rows in LDS
+// when everything fits in 64 KiB.
+inline size_t sil_small_stage_bytes(const Space* S) { return (size_t) S->dim * (sizeof(double2) + 8); }
+inline bool sil_small_stage(const Space* S) {
+    return sil_small_smem(S) + sil_small_stage_bytes(S) <= 64 * 1024;
 }
 inline bool sil_small_fits(const Space* S, int lm) {
     return S->norb >= 2 && S->norb <= 7 && S->dim <= kSmallSilMaxDim && lm <= kMaxLanczos &&

@@ -13,17 +13,12 @@
 #include <cmath>
 #include <vector>
 
+#include "mxm_wave.hpp"
 #include "orbital.hpp"
 
 namespace mctdhb {
 
 // ------------------------------------------------------------- helpers
-
-__device__ __forceinline__ double2 dconj2(double2 a) { return make_double2(a.x, -a.y); }
-__device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
-    const double d = b.x * b.x + b.y * b.y;
-    return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
-}
 
 // Weight of point p in the reference's Simpson rule on n points
 // (calculus.c:8-39: composite 1/3 rule, 3/8 rule on the last 4 points when
@@ -283,11 +278,9 @@ int linear_cn(OrbWork* w, double2* orb, hipStream_t st) {
 
 // hob[i][j] for i <= j (one block each), conj fill, real diagonal
 // (orbital_matrices.c:21-70).
-__global__ __launch_bounds__(256) void k_hob(const double2* __restrict__ orb, int M, int n, double dx, double a2,
-                                             double2 a1, const double* __restrict__ V, double2* __restrict__ hob) {
-    __shared__ double red[8];
-    const int i = blockIdx.x / M, j = blockIdx.x % M;
-    if (j < i) return;
+__device__ __forceinline__ void hob_block(int i, int j, const double2* __restrict__ orb, int M, int n, double dx,
+                                          double a2, double2 a1, const double* __restrict__ V,
+                                          double2* __restrict__ hob, double* red) {
     const double2 *fi = orb + (size_t) i * n, *fj = orb + (size_t) j * n;
     const double r = 1.0 / (12 * dx);
     double v[2] = {0, 0};
@@ -312,10 +305,8 @@ __global__ __launch_bounds__(256) void k_hob(const double2* __restrict__ orb, in
 
 // hint[a + bM + cM^2 + dM^3] = g int conj(phi_a phi_b) phi_c phi_d, one block
 // per element (orbital_matrices.c:72-287 evaluates the same integrals).
-__global__ __launch_bounds__(256) void k_hint(const double2* __restrict__ orb, int M, int n, double dx, double g,
-                                              double2* __restrict__ hint) {
-    __shared__ double red[8];
-    const int e = blockIdx.x;
+__device__ __forceinline__ void hint_block(int e, const double2* __restrict__ orb, int M, int n, double dx, double g,
+                                           double2* __restrict__ hint, double* red) {
     const int a = e % M, b = (e / M) % M, c = (e / (M * M)) % M, d = e / (M * M * M);
     const double2 *fa = orb + (size_t) a * n, *fb = orb + (size_t) b * n;
     const double2 *fc = orb + (size_t) c * n, *fd = orb + (size_t) d * n;
@@ -330,11 +321,26 @@ __global__ __launch_bounds__(256) void k_hint(const double2* __restrict__ orb, i
     if (threadIdx.x == 0) hint[e] = make_double2(g * v[0], g * v[1]);
 }
 
+// Both in one launch: blocks [0, M^2) are the hob elements (i = b / M,
+// j = b % M, only i <= j work), the next M^4 blocks the hint elements.
+__global__ __launch_bounds__(256) void k_orbital_matrices(const double2* __restrict__ orb, int M, int n, double dx,
+                                                          double a2, double2 a1, const double* __restrict__ V,
+                                                          double g, double2* __restrict__ hob,
+                                                          double2* __restrict__ hint) {
+    __shared__ double red[8];
+    const int b = blockIdx.x, mm = M * M;
+    if (b < mm) {
+        if (b % M >= b / M) hob_block(b / M, b % M, orb, M, n, dx, a2, a1, V, hob, red);
+    } else {
+        hint_block(b - mm, orb, M, n, dx, g, hint, red);
+    }
+}
+
 int orbital_matrices(OrbWork* w, const double2* orb, double2* hob, double2* hint, hipStream_t st) {
     const OrbEq& q = w->eq;
     const int M = q.norb;
-    k_hob<<<M * M, 256, 0, st>>>(orb, M, q.grid_size, q.dx, q.d2coef, q.d1coef, w->d_pot, hob);
-    k_hint<<<M * M * M * M, 256, 0, st>>>(orb, M, q.grid_size, q.dx, q.g, hint);
+    k_orbital_matrices<<<M * M + M * M * M * M, 256, 0, st>>>(orb, M, q.grid_size, q.dx, q.d2coef, q.d1coef,
+                                                              w->d_pot, q.g, hob, hint);
     MCTDHB_HIP_CHECK(hipGetLastError());
     return kOk;
 }
@@ -778,7 +784,7 @@ static int rk_tableau(OrbWork* w, int ns, const double* A, const double* b, doub
 // S_p = <F_j, F_i> evaluated after points < p were updated,
 // S_{p+1} = S_p (1 - w_p |F_j[p]|^2), so S_p = S_0 prod_{q<p} (1 - w_q |F_j[q]|^2):
 // one reduction and one multiplicative scan per (i, j).
-__global__ __launch_bounds__(256) void k_gram_schmidt(double2* __restrict__ F, int M, int n, double dx) {
+__device__ __forceinline__ void gram_schmidt_block(double2* __restrict__ F, int M, int n, double dx) {
     __shared__ double red[64];
     __shared__ double wave_prod[4];
     const int T = blockDim.x;
@@ -833,8 +839,37 @@ __global__ __launch_bounds__(256) void k_gram_schmidt(double2* __restrict__ F, i
     }
 }
 
-int orthonormalize(OrbWork* w, double2* orb, hipStream_t st) {
-    k_gram_schmidt<<<1, 256, 0, st>>>(orb, w->eq.norb, w->eq.grid_size, w->eq.dx);
+// The orbitals are staged in LDS when they fit (every pass above then reads
+// LDS, not the cache hierarchy).  With tail set, the last grid point of
+// every orbital is then set to the first (the periodic copy the whole step
+// applies after the orbitals).
+constexpr size_t kGsStageBytes = 64 * 1024;
+__global__ __launch_bounds__(256) void k_gram_schmidt(double2* __restrict__ Fg, int M, int n, double dx, int tail,
+                                                      int staged) {
+    extern __shared__ double2 Fs[];
+    const int tot = M * n;
+    double2* F = Fg;
+    if (staged) {
+        for (int t = threadIdx.x; t < tot; t += blockDim.x) Fs[t] = Fg[t];
+        __syncthreads();
+        gram_schmidt_block(Fs, M, n, dx);  // ends with a barrier
+        F = Fs;
+    } else {
+        gram_schmidt_block(Fg, M, n, dx);
+    }
+    if (tail)
+        for (int k = threadIdx.x; k < M; k += blockDim.x) F[(size_t) k * n + n - 1] = F[(size_t) k * n];
+    if (staged) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < tot; t += blockDim.x) Fg[t] = Fs[t];
+    }
+}
+
+int orthonormalize(OrbWork* w, double2* orb, hipStream_t st, bool periodic_tail) {
+    const size_t bytes = sizeof(double2) * w->eq.norb * w->eq.grid_size;
+    const bool staged = bytes <= kGsStageBytes;
+    k_gram_schmidt<<<1, 256, staged ? bytes : 0, st>>>(orb, w->eq.norb, w->eq.grid_size, w->eq.dx,
+                                                       periodic_tail ? 1 : 0, staged ? 1 : 0);
     MCTDHB_HIP_CHECK(hipGetLastError());
     return kOk;
 }
@@ -860,7 +895,6 @@ int periodic_bounds(OrbWork* w, double2* orb, hipStream_t st) {
 // L is the reference's finite-difference operator (split_linear_orbitals.c:
 // 209-255) or its spectral one (:257-286) plus V.
 
-__device__ void gj_inverse_wave(int M, double2* a, double2* inv, double2* fac);
 
 // L y_k at point x by finite differences (n = grid size).
 __device__ __forceinline__ double2 horb_fd_at(const double2* f, int n, int x, int bounds, double2 up, double2 lo,
@@ -1287,13 +1321,16 @@ int fullstep(OrbWork* w, int der_method, int robust, double2* orb, const double2
     return rc;
 }
 
+// The reference first copies the first grid point onto the last
+// (orbital_integration.c:370-373); with periodic bounds both linear half
+// steps neither read the last point nor keep it (they write it from the
+// first, split_linear_orbitals.c:364-365 and :397), so that copy changes
+// nothing and is not made here.
 int splitstep(OrbWork* w, int use_fft, double2* orb, double2* hob, double2* hint, const double2* rho2,
-              const double2* rinv, hipStream_t st) {
+              const double2* rinv, hipStream_t st, bool periodic_tail) {
     const OrbEq& q = w->eq;
     const int total = q.norb * q.grid_size;
-    int rc = kOk;
-    if (q.bounds == kPeriodicBounds && (rc = periodic_bounds(w, orb, st))) return rc;
-    rc = use_fft ? linear_fft(w, orb, st) : linear_cn(w, orb, st);
+    int rc = use_fft ? linear_fft(w, orb, st) : linear_cn(w, orb, st);
     if (rc) return rc;
     rc = orbital_matrices(w, orb, hob, hint, st);
     if (rc) return rc;
@@ -1325,196 +1362,18 @@ int splitstep(OrbWork* w, int use_fft, double2* orb, double2* hob, double2* hint
     if (rc) return rc;
     rc = use_fft ? linear_fft(w, orb, st) : linear_cn(w, orb, st);
     if (rc) return rc;
-    if (w->imagtime) rc = orthonormalize(w, orb, st);
+    // in real time the linear step already made the periodic copy
+    if (w->imagtime) rc = orthonormalize(w, orb, st, periodic_tail && q.bounds == kPeriodicBounds);
     return rc;
 }
 
 // ---------------------------------------------------- M x M algebra
 
-// Hermitian eigen-decomposition by cyclic complex Jacobi rotations on one
-// wave (n <= 64; a is overwritten, all in LDS): rotations run in the cyclic
-// order, lane r applies each one to row/column r.  Eigenvalues ascending in
-// lam (LDS), eigenvectors in the columns of v.  Every lane of the block
-// must call it.
-__device__ void herm_eig_wave(int n, double2* a, double* lam, double2* v, double* rowsum) {
-    const int r = threadIdx.x;
-    if (r < n)
-        for (int c = 0; c < n; c++) v[r * n + c] = make_double2(r == c ? 1.0 : 0.0, 0.0);
-    __syncthreads();
-    for (int sweep = 0; sweep < 100; sweep++) {
-        if (r < n) {
-            double off = 0, tot = 0;
-            for (int c = 0; c < n; c++) {
-                const double m2 = a[r * n + c].x * a[r * n + c].x + a[r * n + c].y * a[r * n + c].y;
-                tot += m2;
-                if (r != c) off += m2;
-            }
-            rowsum[2 * r] = off;
-            rowsum[2 * r + 1] = tot;
-        }
-        __syncthreads();
-        double off = 0, tot = 0;
-        for (int c = 0; c < n; c++) {
-            off += rowsum[2 * c];
-            tot += rowsum[2 * c + 1];
-        }
-        __syncthreads();
-        if (off <= 1e-30 * tot || off == 0) break;
-        for (int p = 0; p < n - 1; p++)
-            for (int q = p + 1; q < n; q++) {
-                const double2 apq = a[p * n + q];
-                const double mag = sqrt(apq.x * apq.x + apq.y * apq.y);
-                if (mag == 0) continue;
-                const double app = a[p * n + p].x, aqq = a[q * n + q].x;
-                const double2 ph = cscale(apq, 1.0 / mag);
-                const double tau = (aqq - app) / (2 * mag);
-                const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1 + tau * tau));
-                const double c = 1 / sqrt(1 + t * t), s = t * c;
-                __syncthreads();
-                if (r < n) {
-                    const double2 arp = a[r * n + p], arq = a[r * n + q];
-                    a[r * n + p] = cscale(arp, c) - cmul(cscale(dconj2(ph), s), arq);
-                    a[r * n + q] = cmul(cscale(ph, s), arp) + cscale(arq, c);
-                    const double2 vrp = v[r * n + p], vrq = v[r * n + q];
-                    v[r * n + p] = cscale(vrp, c) - cmul(cscale(dconj2(ph), s), vrq);
-                    v[r * n + q] = cmul(cscale(ph, s), vrp) + cscale(vrq, c);
-                }
-                __syncthreads();
-                if (r < n) {
-                    const double2 apr = a[p * n + r], aqr = a[q * n + r];
-                    a[p * n + r] = cscale(apr, c) - cmul(cscale(ph, s), aqr);
-                    a[q * n + r] = cmul(cscale(dconj2(ph), s), apr) + cscale(aqr, c);
-                }
-                __syncthreads();
-                if (r == 0) {
-                    a[p * n + q] = make_double2(0, 0);
-                    a[q * n + p] = make_double2(0, 0);
-                }
-                __syncthreads();
-            }
-    }
-    if (r < n) lam[r] = a[r * n + r].x;
-    __syncthreads();
-    for (int x = 0; x < n - 1; x++) {
-        int best = x;
-        for (int y = x + 1; y < n; y++)
-            if (lam[y] < lam[best]) best = y;
-        __syncthreads();
-        if (best != x) {
-            if (r == 0) {
-                const double t = lam[x];
-                lam[x] = lam[best];
-                lam[best] = t;
-            }
-            if (r < n) {
-                const double2 tv = v[r * n + x];
-                v[r * n + x] = v[r * n + best];
-                v[r * n + best] = tv;
-            }
-        }
-        __syncthreads();
-    }
-}
-
-// Gauss-Jordan inverse with partial pivoting on one wave (M <= 64): a (LDS,
-// row-major, overwritten) -> inv (LDS).  Lane r owns column r; every lane of
-// the block must call it.  Elimination order and arithmetic per element are
-// those of a serial Gauss-Jordan sweep (lapack_interface.c:10-59 stand-in).
-__device__ void gj_inverse_wave(int M, double2* a, double2* inv, double2* fac) {
-    const int r = threadIdx.x;
-    if (r < M)
-        for (int c = 0; c < M; c++) inv[c * M + r] = make_double2(c == r ? 1.0 : 0.0, 0);
-    __syncthreads();
-    for (int col = 0; col < M; col++) {
-        int piv = col;
-        double best = 0;
-        for (int i = col; i < M; i++) {
-            const double m2 = a[i * M + col].x * a[i * M + col].x + a[i * M + col].y * a[i * M + col].y;
-            if (m2 > best) {
-                best = m2;
-                piv = i;
-            }
-        }
-        if (best == 0) break;
-        __syncthreads();
-        if (piv != col && r < M) {
-            double2 t = a[col * M + r];
-            a[col * M + r] = a[piv * M + r];
-            a[piv * M + r] = t;
-            t = inv[col * M + r];
-            inv[col * M + r] = inv[piv * M + r];
-            inv[piv * M + r] = t;
-        }
-        __syncthreads();
-        const double2 d = cdiv(make_double2(1, 0), a[col * M + col]);
-        __syncthreads();
-        if (r < M) {
-            a[col * M + r] = cmul(a[col * M + r], d);
-            inv[col * M + r] = cmul(inv[col * M + r], d);
-            fac[r] = a[r * M + col];
-        }
-        __syncthreads();
-        if (r < M)
-            for (int i = 0; i < M; i++) {
-                if (i == col) continue;
-                const double2 f = fac[i];
-                if (f.x == 0 && f.y == 0) continue;
-                a[i * M + r] = a[i * M + r] - cmul(f, a[col * M + r]);
-                inv[i * M + r] = inv[i * M + r] - cmul(f, inv[col * M + r]);
-            }
-        __syncthreads();
-    }
-}
-
-// rho1 eigenvalues (synchronize.c:49), regularisation when the smallest
-// natural occupation is below reg_factor times the largest
-// (lapack_interface.c:126-163, synchronize.c:50-54), then the inverse from
-// the lower triangle (lapack_interface.c:10-59).  One wave, M <= 30; lane
-// r owns row r in the eigen-solve and column r in the regularisation and
-// the Gauss-Jordan elimination.
 __global__ __launch_bounds__(64) void k_density_sync_small(int M, double2* __restrict__ rho1,
                                                            double2* __restrict__ rinv, double* __restrict__ eig_out,
                                                            double reg_factor) {
     extern __shared__ double2 work[];  // 4 * M * M
-    __shared__ double lam[64], rowsum[128];
-    __shared__ double2 fac[64];
-    const int r = threadIdx.x;
-    double2* a = work;              // M*M
-    double2* v = work + M * M;      // M*M
-    double2* inv = work + 2 * M * M;
-    double2* rl = work + 3 * M * M;  // rho1 as regularised
-    // zheev reads the upper triangle with a real diagonal
-    if (r < M)
-        for (int c = 0; c < M; c++) {
-            a[r * M + c] = r == c ? make_double2(rho1[r * M + r].x, 0)
-                                  : (c > r ? rho1[r * M + c] : dconj2(rho1[c * M + r]));
-            rl[r * M + c] = rho1[r * M + c];
-        }
-    __syncthreads();
-    herm_eig_wave(M, a, lam, v, rowsum);
-    if (eig_out && r < M) eig_out[r] = lam[r];
-    if (lam[0] / lam[M - 1] < reg_factor) {
-        const double x = lam[M - 1] * reg_factor;
-        if (r < M)
-            for (int i = 0; i < M; i++) {
-                double2 z = make_double2(0, 0);
-                for (int k = 0; k < M; k++)
-                    z = z + cmul(cscale(v[i * M + k], x * exp(-lam[k] / x)), dconj2(v[r * M + k]));
-                rl[i * M + r] = rl[i * M + r] + z;
-                rho1[i * M + r] = rl[i * M + r];
-            }
-    }
-    __syncthreads();
-    // Gauss-Jordan inverse with partial pivoting of the Hermitian matrix
-    // given by the lower triangle and real diagonal
-    if (r < M)
-        for (int c = 0; c < M; c++)
-            a[c * M + r] = c == r ? make_double2(rl[r * M + r].x, 0)
-                                  : (r < c ? rl[c * M + r] : dconj2(rl[r * M + c]));
-    __syncthreads();
-    gj_inverse_wave(M, a, inv, fac);
-    if (r < M)
-        for (int c = 0; c < M; c++) rinv[c * M + r] = inv[c * M + r];
+    density_sync_body(M, rho1, rinv, eig_out, reg_factor, work);
 }
 
 int density_sync_small(int norb, double2* rho1, double2* rinv, double* eigvals, double reg_factor, hipStream_t st) {

@@ -54,12 +54,15 @@ int nonlinear_rhs(OrbWork* w, const double2* orb, const double2* hob, const doub
                   const double2* rinv, double2* der, hipStream_t st);
 // orb[k][grid - 1] = orb[k][0] (orbital_integration.c:13-20).
 int periodic_bounds(OrbWork* w, double2* orb, hipStream_t st);
-// Gram-Schmidt with the reference's point-by-point projection update.
-int orthonormalize(OrbWork* w, double2* orb, hipStream_t st);
+// Gram-Schmidt with the reference's point-by-point projection update;
+// periodic_tail also copies the first grid point onto the last.
+int orthonormalize(OrbWork* w, double2* orb, hipStream_t st, bool periodic_tail = false);
 // One split step (linear half, h/g rebuild, RK4 nonlinear, linear half,
 // Gram-Schmidt in imaginary time); hob/hint are rebuilt from the orbitals.
+// periodic_tail: leave the orbitals with the periodic copy of the first
+// grid point on the last (with periodic bounds), as the whole step needs.
 int splitstep(OrbWork* w, int use_fft, double2* orb, double2* hob, double2* hint, const double2* rho2,
-              const double2* rinv, hipStream_t st);
+              const double2* rinv, hipStream_t st, bool periodic_tail = false);
 // Full-equation Runge-Kutta orbital step (orbital_integration.c:283-348):
 // linear part by FFT (der_method 0), finite differences (1) or the
 // exponential DVR (2), simple or robust projector, Gram-Schmidt in

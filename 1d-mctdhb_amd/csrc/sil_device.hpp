@@ -110,15 +110,18 @@ __device__ inline int tridiag_eig_wave(int n, double& d, double& e, double* zr) 
             if (m == l) break;
             if (++sweeps > 60) return 1;
             const double dl = rlane(d, l), el = rlane(e, l);
+            // |theta| <= 1 / (2 eps) after the test above and the entries
+            // of T are far from overflow: plain square roots of sums of
+            // squares in place of hypot, one reciprocal per rotation
             const double theta = (rlane(d, l + 1) - dl) / (2.0 * el);
-            const double rho = hypot(theta, 1.0);
+            const double rho = sqrt(theta * theta + 1.0);
             double g = rlane(d, m) - dl + el / (theta + (theta >= 0 ? rho : -rho));
             double sn = 1.0, cs = 1.0, shift = 0.0;
             bool early = false;
             for (int i = m - 1; i >= l; i--) {
                 const double ei = rlane(e, i);
                 const double f = sn * ei, b = cs * ei;
-                double r = hypot(f, g);
+                double r = sqrt(f * f + g * g);
                 if (lane == i + 1) e = r;
                 if (r == 0.0) {
                     if (lane == i + 1) d -= shift;
@@ -126,8 +129,9 @@ __device__ inline int tridiag_eig_wave(int n, double& d, double& e, double* zr) 
                     early = true;
                     break;
                 }
-                sn = f / r;
-                cs = g / r;
+                const double ir = 1.0 / r;
+                sn = f * ir;
+                cs = g * ir;
                 g = rlane(d, i + 1) - shift;
                 r = (rlane(d, i) - g) * sn + 2.0 * cs * b;
                 shift = sn * r;

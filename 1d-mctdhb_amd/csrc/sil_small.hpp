@@ -40,7 +40,10 @@ __device__ __forceinline__ void block_sum_slots(const double (&acc)[kSmallSlots]
     __syncthreads();
 }
 
-template <int M, int IDX>
+// STAGE: the vector H acts on (and the occupation rows) are kept in LDS, so
+// the row gathers read LDS; w_{it+1} is written there by the pass that
+// forms it, while the pass over w_it has finished.
+template <int M, int IDX, bool STAGE>
 __global__ __launch_bounds__(kSmallSilBlock) void k_sil_small(RowCtx ctx, const double2* __restrict__ cf,
                                                               const uint32_t* __restrict__ Wg, int ldw,
                                                               SmallSilArgs a) {
@@ -48,13 +51,30 @@ __global__ __launch_bounds__(kSmallSilBlock) void k_sil_small(RowCtx ctx, const 
     const int wsz = (M - 1) * ldw;
     uint32_t* Wl = (uint32_t*) smem;
     double* sql = (double*) (smem + ((wsz * 4 + 15) & ~15));
-    SmallSilLds& L = *(SmallSilLds*) (smem + ((wsz * 4 + 15) & ~15) + ((sizeof(double) * (ctx.npar + 3) + 15) & ~15));
+    unsigned char* lds_l = smem + ((wsz * 4 + 15) & ~15) + ((sizeof(double) * (ctx.npar + 3) + 15) & ~15);
+    SmallSilLds& L = *(SmallSilLds*) lds_l;
+    double2* Ws = (double2*) (lds_l + sizeof(SmallSilLds));  // STAGE only: dim entries
+    uint2* Os = (uint2*) (Ws + ctx.dim);                       // STAGE only: dim occupation rows
     if constexpr (IDX == kIdxFly)
         for (int t = threadIdx.x; t < wsz; t += blockDim.x) Wl[t] = Wg[t];
     for (int t = threadIdx.x; t < ctx.npar + 3; t += blockDim.x) sql[t] = sqrt((double) t);
-    SilState* st = a.st;
-    if (threadIdx.x == 0) sil_init_state(st, a.lm);
+    RowCtx rc = ctx;
+    if constexpr (STAGE) {
+        const double2* c0 = (const double2*) a.w[0];
+        for (uint32_t t = threadIdx.x; t < ctx.dim; t += blockDim.x) Ws[t] = c0[t];
+        if (ctx.occ8) {
+            for (uint32_t t = threadIdx.x; t < ctx.dim; t += blockDim.x) Os[t] = ((const uint2*) ctx.occ8)[t];
+            rc.occ8 = (const uint8_t*) Os;
+        }
+    }
+    // the scalars live in LDS (the one-thread updates then make no global
+    // round trips) and are copied to a.st once the eigensolve is done
+    SilState* st = &L.state;
+    for (uint32_t t = threadIdx.x; t < sizeof(SilState) / sizeof(double); t += blockDim.x) ((double*) st)[t] = 0;
+    __syncthreads();
     if (threadIdx.x == 0) {
+        st->s[0] = 1.0;
+        st->iter_done = a.lm;
 #pragma unroll
         for (int k = 0; k < kMaxLanczos; k++) L.w[k] = (double2*) a.w[k];  // constant indices: no private copy
     }
@@ -74,7 +94,8 @@ __global__ __launch_bounds__(kSmallSilBlock) void k_sil_small(RowCtx ctx, const 
         for (int t = 0; t < kSmallSlots; t++) acc[t] = 0;
         for (uint32_t i = threadIdx.x; i < n; i += T) {
             double2 own;
-            double2 o = cscale(hc_row_fused<M, double, IDX, double2, false>(i, ctx, wit, cf, Wl, ldw, sql, own), sit);
+            double2 o = cscale(
+                hc_row_fused<M, double, IDX, double2, false>(i, rc, STAGE ? Ws : wit, cf, Wl, ldw, sql, own), sit);
             if (it > 0) o = o - cscale(wpr[i], bprev);
             out[i] = o;
 #pragma unroll
@@ -108,6 +129,7 @@ __global__ __launch_bounds__(kSmallSilBlock) void k_sil_small(RowCtx ctx, const 
             double2 w = out[i];
             for (int k = 0; k <= it; k++) w = w - cmul(L.cre[k], L.w[k][i]);
             wn[i] = w;
+            if constexpr (STAGE) Ws[i] = w;
 #pragma unroll
             for (int k = 0; k < kMaxLanczos; k++) {
                 if (k <= it) {  // the second read of w_k[i] hits the cache
@@ -127,8 +149,10 @@ __global__ __launch_bounds__(kSmallSilBlock) void k_sil_small(RowCtx ctx, const 
         if (st->stop) break;
     }
     // exp(-i T prop_dt) e_1 on wave 0, then C' = sum_k y_k w_k
-    if (threadIdx.x < 64) sil_tridiag_wave(st, a.dt_re, a.dt_im, L.z, kMaxLanczos + 1);
+    if (threadIdx.x < 64) sil_tridiag_wave(st, a.dt_re, a.dt_im, L.z, kMaxLanczos + 2);
     __syncthreads();
+    for (uint32_t t = threadIdx.x; t < sizeof(SilState) / sizeof(double); t += blockDim.x)
+        ((double*) a.st)[t] = ((const double*) st)[t];
     if (st->eig_fail) return;
     const int nd = st->iter_done;
     double2* C = L.w[0];
@@ -148,16 +172,27 @@ __global__ __launch_bounds__(kSmallSilBlock) void k_sil_small(RowCtx ctx, const 
 
 template <int M>
 int HcFused<M>::sil_small(const Space* S, const void* cf, const SmallSilArgs& a, int index, hipStream_t st) {
-    const size_t smem = sil_small_smem(S);
     if (!sil_small_fits(S, a.lm)) {
         set_error("space too large for the one-workgroup SIL step");
         return kInvalidArgument;
     }
+    const bool stage = sil_small_stage(S);
+    const size_t smem = sil_small_smem(S) + (stage ? sil_small_stage_bytes(S) : 0);
     const RowCtx ctx = row_ctx(S);
-    if (index == kIndexOnTheFly)
-        k_sil_small<M, kIdxFly><<<1, kSmallSilBlock, smem, st>>>(ctx, (const double2*) cf, S->d_W, S->ldw, a);
-    else
-        k_sil_small<M, kIdxTables><<<1, kSmallSilBlock, smem, st>>>(ctx, (const double2*) cf, S->d_W, S->ldw, a);
+    if (index == kIndexOnTheFly) {
+        if (stage)
+            k_sil_small<M, kIdxFly, true><<<1, kSmallSilBlock, smem, st>>>(ctx, (const double2*) cf, S->d_W, S->ldw, a);
+        else
+            k_sil_small<M, kIdxFly, false><<<1, kSmallSilBlock, smem, st>>>(ctx, (const double2*) cf, S->d_W, S->ldw,
+                                                                           a);
+    } else {
+        if (stage)
+            k_sil_small<M, kIdxTables, true><<<1, kSmallSilBlock, smem, st>>>(ctx, (const double2*) cf, S->d_W,
+                                                                             S->ldw, a);
+        else
+            k_sil_small<M, kIdxTables, false><<<1, kSmallSilBlock, smem, st>>>(ctx, (const double2*) cf, S->d_W,
+                                                                              S->ldw, a);
+    }
     MCTDHB_HIP_CHECK(hipGetLastError());
     return kOk;
 }

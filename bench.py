@@ -12,8 +12,9 @@ All inputs are resident in HBM before timing starts.
 Prints one JSON line.  `value` is the mixed-precision form (complex64 H.C and
 density sums, FP64 Lanczos scalars); the FP64 forms, their ratio, the error
 of the mixed path against FP64, p50/p99 step latency, the H.C kernel's
-roofline and a CPU baseline (the oracle's restatement of the reference,
-timed on this host) ride along.
+roofline, the whole MCTDHB step at N=10, M=4, mixed against FP64 on
+(10,4), (200,4), (30,7) and (100,5) with m = 8, and a CPU baseline (the
+oracle's restatement of the reference, timed on this host) ride along.
 """
 from __future__ import annotations
 
@@ -188,6 +189,62 @@ def full_step_bench(args, stream):
     return out
 
 
+MIXED_SHAPES = ((10, 4), (200, 4), (30, 7), (100, 5))
+
+
+def mixed_shapes_bench(args, stream, steps=50, lm=8, dt=0.01):
+    """Mixed precision against FP64 on several shapes: from the same seeded
+    coefficients, h and g (scale 0.1), `steps` real-time SIL steps with
+    Lanczos dimension `lm`, once with complex64 H.C (FP64 Lanczos scalars)
+    and once all-FP64, both with on-the-fly jump indices.  Reports the
+    per-step p50 of each and the relative error of the energy per particle,
+    of the natural occupations and of rho1 (Frobenius) at the end."""
+    import torch
+    import mctdhb_amd as amd
+    from mctdhb_amd import inputs
+
+    out = []
+    for N, M in MIXED_SHAPES:
+        nc = math.comb(N + M - 1, M - 1)
+        space = amd.DeviceSpace(N, M, tables=False)
+        C0 = torch.from_numpy(inputs.coefficients(nc, args.seed)).cuda()
+        H = torch.from_numpy(np.ascontiguousarray(inputs.onebody(M, args.seed))).cuda()
+        G = torch.from_numpy(inputs.twobody(M, args.seed, 0.1)).cuda()
+        res = {"npar": N, "norb": M, "dim": nc, "lanczos_iter": lm, "dt": dt, "steps": steps}
+        finals = {}
+        for name, prec in (("mixed", amd.MIXED), ("fp64", amd.FP64)):
+            sil = amd.DeviceSil(space, lm, prec, amd.INDEX_FLY)
+            C = C0.clone()
+            evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+            evs[0].record(stream)
+            for k in range(steps):
+                sil.step(C, H, G, dt, False, stream=stream)
+                evs[k + 1].record(stream)
+            evs[-1].synchronize()
+            per = [evs[k].elapsed_time(evs[k + 1]) for k in range(steps)]
+            sil.close()
+            res[f"{name}_p50_ms"] = percentile(per, 50)
+            res[f"{name}_steps_per_s"] = 1e3 / percentile(per, 50)
+            hc = torch.empty_like(C)
+            space.apply_hamiltonian(C, H, G, hc, amd.FP64, amd.INDEX_FLY, stream=stream)
+            r1 = torch.empty(M * M, dtype=torch.complex128, device="cuda")
+            space.density(C, r1, None, amd.FP64, amd.INDEX_FLY, stream=stream)
+            torch.cuda.synchronize()
+            rho1 = r1.cpu().numpy().reshape(M, M)
+            finals[name] = (torch.vdot(C, hc).real.item() / N, rho1, np.sort(np.linalg.eigvalsh(rho1)) / N)
+        (e32, r32, o32), (e64, r64, o64) = finals["mixed"], finals["fp64"]
+        res["energy_rel"] = abs(e32 - e64) / abs(e64)
+        res["natural_occupations_rel"] = float(np.max(np.abs(o32 - o64) / np.maximum(np.abs(o64), 1e-300)))
+        res["rho1_rel_fro"] = float(np.linalg.norm(r32 - r64) / np.linalg.norm(r64))
+        res["mixed_vs_fp64"] = res["mixed_steps_per_s"] / res["fp64_steps_per_s"]
+        log(f"mixed vs FP64 N={N} M={M}: {res['mixed_p50_ms']:.3f} / {res['fp64_p50_ms']:.3f} ms, "
+            f"E rel {res['energy_rel']:.2e}, rho1 rel {res['rho1_rel_fro']:.2e}")
+        out.append(res)
+        del space, C0
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,6 +260,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-full-step", action="store_true",
                     help="skip the whole-step (orbitals + SIL + densities) eager/graph timing")
+    ap.add_argument("--no-mixed-shapes", action="store_true",
+                    help="skip the mixed-vs-FP64 comparison on (10,4), (200,4), (30,7), (100,5) with m=8")
     ap.add_argument("--full-step-only", action="store_true",
                     help="only the whole-step timing (prints its JSON object)")
     ap.add_argument("--probe-hc", action="store_true",
@@ -359,6 +418,7 @@ def main():
     }
 
     full = full_step_bench(args, stream) if not args.no_full_step else None
+    shapes = mixed_shapes_bench(args, stream) if not args.no_mixed_shapes else None
 
     cpu = None
     if not args.no_cpu:
@@ -404,6 +464,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "full_step": full,
+        "mixed_vs_fp64_shapes": shapes,
     }
     print(json.dumps(line), flush=True)
 
