@@ -1,0 +1,150 @@
+"""Many independent MCTDHB solves on one GPU at once.
+
+Each instance owns its configuration space, SIL workspace, orbital workspace
+and state (orbitals, coefficients, h, g, rho1, rho2, rho1^-1) in HBM, and its
+whole time step (reference driver.c:159-202) captured once as a HIP graph.
+A batch replays the instances' graphs round-robin over a few streams, so the
+small kernels of different instances run side by side on the compute units;
+instances share no buffers, so each result is the same alone and in the mix.
+
+`draw_ground_states` draws the uneven ground-state mix: N uniform in
+[4, 64], M uniform in {2..6} (rejecting NC above a cap), Ng in {256, 512,
+1024} grid intervals (periodic, grid_size Ng + 1), a double-well trap
+V = a (x^2 - b^2)^2 with a ~ U[0.5, 2], b ~ U[1, 3], g log-uniform in
+[0.01, 10], Hermite-Gauss orbitals plus 1e-3 noise and complex-normal
+coefficients, imaginary time, dt = 0.005, Lanczos dimension 5.
+
+The box is [-L, L) with V(L) = 200, not [-10, 10): the split step
+integrates V inside its Runge-Kutta sub-step (orbital_integration.c:243-282),
+which is stable only for V dt below about 2.8, and V(10) dt is 20 or more
+for every draw, so on [-10, 10) the edge points blow up (the oracle's own
+split step gives E/N in the thousands after 60 steps).  With V(L) dt = 1
+the wells and their decaying tails lie well inside the box.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import inputs
+
+
+@dataclass
+class Draw:
+    npar: int
+    norb: int
+    ngrid: int
+    a: float
+    b: float
+    g: float
+    seed: int
+
+    @property
+    def dim(self) -> int:
+        return math.comb(self.npar + self.norb - 1, self.norb - 1)
+
+    @property
+    def half_width(self) -> float:
+        """L with V(L) = a (L^2 - b^2)^2 = 200, at most 10."""
+        return min(10.0, math.sqrt(self.b ** 2 + math.sqrt(200.0 / self.a)))
+
+
+def draw_ground_states(count: int, seed: int = 11, max_dim: int = 200_000) -> list[Draw]:
+    r = inputs.rng(seed, 4)
+    out = []
+    while len(out) < count:
+        n = int(r.integers(4, 65))
+        m = int(r.integers(2, 7))
+        ng = int(r.choice([256, 512, 1024]))
+        a = float(r.uniform(0.5, 2.0))
+        b = float(r.uniform(1.0, 3.0))
+        g = float(10 ** r.uniform(-2.0, 1.0))
+        if math.comb(n + m - 1, m - 1) > max_dim:
+            continue
+        out.append(Draw(n, m, ng, a, b, g, seed * 100003 + len(out)))
+    return out
+
+
+class Instance:
+    """One imaginary-time solve, state resident on the GPU, step captured
+    as a graph.  Spaces up to the one-workgroup SIL size run the FP64 step
+    (its one-workgroup form is the faster one there); larger ones the mixed
+    form (complex64 H.C, FP64 Lanczos scalars)."""
+
+    def __init__(self, d: Draw, dt: float = 0.005, lanczos: int = 5, small_fp64_dim: int = 8192):
+        import torch
+        import mctdhb_amd as amd
+
+        self.draw = d
+        M, N = d.norb, d.npar
+        grid = d.ngrid + 1
+        L = d.half_width
+        x = np.linspace(-L, L, grid)
+        pot = d.a * (x ** 2 - d.b ** 2) ** 2
+        self.prec = amd.FP64 if d.dim <= small_fp64_dim else amd.MIXED
+        self.space = amd.DeviceSpace(N, M, tables=False)
+        self.sil = amd.DeviceSil(self.space, lanczos, self.prec, amd.INDEX_FLY)
+        self.orbital = amd.DeviceOrbital(M, grid, x[0], x[-1], dt, d.g, pot, imagtime=True)
+        orb = inputs.hermite_gauss(M, x, seed=d.seed, noise=1e-3)
+        orb[:, -1] = orb[:, 0]
+        dev = "cuda"
+        self.t = {"orb": torch.from_numpy(np.ascontiguousarray(orb.ravel())).to(dev),
+                  "coef": torch.from_numpy(inputs.coefficients(d.dim, d.seed)).to(dev)}
+        for k, n in (("hob", M * M), ("hint", M ** 4), ("r1", M * M), ("r2", M ** 4), ("rinv", M * M)):
+            self.t[k] = torch.zeros(n, dtype=torch.complex128, device=dev)
+        t = self.t
+        self.orbital.orthonormalize(t["orb"])
+        self.orbital.orbital_matrices(t["orb"], t["hob"], t["hint"])
+        amd.density_sync(self.space, t["coef"], t["r1"], t["r2"], t["rinv"])
+        torch.cuda.synchronize()
+        self.graph = amd.StepGraph(self.space, self.sil, self.orbital, t["orb"], t["coef"], t["hob"], t["hint"],
+                                   t["r1"], t["r2"], t["rinv"])
+
+    def step(self, stream=None):
+        self.graph.launch(stream)
+
+    def energy_per_particle(self) -> float:
+        """E / N from rho1, rho2, h and g (reference total_energy,
+        integrator_monitor.c:56-88); call after a synchronisation."""
+        M = self.draw.norb
+        r1 = self.t["r1"].cpu().numpy().reshape(M, M)
+        h = self.t["hob"].cpu().numpy().reshape(M, M)
+        # flat k + l M + s M^2 + q M^3 -> numpy axes (q, s, l, k)
+        r2 = self.t["r2"].cpu().numpy().reshape(M, M, M, M)
+        g = self.t["hint"].cpu().numpy().reshape(M, M, M, M)
+        # rho2[k,l,s,q] g[k,l,q,s]: swap the two leading axes of g
+        e = np.sum(r1 * h) + 0.5 * np.sum(r2 * g.transpose(1, 0, 2, 3))
+        return float(e.real) / self.draw.npar
+
+    def occupations(self) -> np.ndarray:
+        M = self.draw.norb
+        r1 = self.t["r1"].cpu().numpy().reshape(M, M)
+        return np.sort(np.linalg.eigvalsh(r1))[::-1] / self.draw.npar
+
+    def close(self):
+        self.graph.close()
+        self.sil.close()
+        self.space.close()
+
+
+class Batch:
+    """Instances stepped together: step k of every instance is enqueued on
+    stream (instance index mod len(streams)) before step k + 1 of any."""
+
+    def __init__(self, instances: list[Instance], nstreams: int = 4):
+        import torch
+
+        self.instances = instances
+        self.streams = [torch.cuda.Stream() for _ in range(max(1, nstreams))]
+
+    def run(self, steps: int):
+        S = self.streams
+        for _ in range(steps):
+            for i, inst in enumerate(self.instances):
+                inst.step(S[i % len(S)])
+
+    def synchronize(self):
+        for s in self.streams:
+            s.synchronize()

@@ -245,6 +245,62 @@ def mixed_shapes_bench(args, stream, steps=50, lm=8, dt=0.01):
     return out
 
 
+def batched_bench(args):
+    """Uneven imaginary-time ground states solved at once: N in [4, 64], M in
+    {2..6} (NC <= 2e5), Ng in {256, 512, 1024} on [-L, L) with V(L) = 200,
+    double-well traps, g log-uniform in [0.01, 10], dt = 0.005, m = 5
+    (mctdhb_amd.batch).
+    Every instance's step is a HIP graph; the batch replays them round-robin
+    over a few streams.  For comparison the same instances are also stepped
+    one after another, and together, for min(K, 10) steps each."""
+    import torch
+    from mctdhb_amd import batch
+
+    draws = batch.draw_ground_states(args.batch_instances, seed=args.seed)
+    t0 = time.perf_counter()
+    insts = [batch.Instance(d) for d in draws]
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    b = batch.Batch(insts, nstreams=args.batch_streams)
+    K = args.batch_steps
+    b.run(1)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.run(K)
+    b.synchronize()
+    wall = time.perf_counter() - t0
+    log(f"batched: {len(insts)} instances x {K} steps in {wall:.2f} s (setup {setup_s:.1f} s)")
+    # the same instances one after another on one stream, and together
+    K2 = min(K, 10)
+    st = b.streams[0]
+    t0 = time.perf_counter()
+    for inst in insts:
+        for _ in range(K2):
+            inst.step(st)
+    st.synchronize()
+    serial = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.run(K2)
+    b.synchronize()
+    together = time.perf_counter() - t0
+    log(f"{K2} steps each: one after another {serial:.2f} s, together {together:.2f} s")
+    e = [inst.energy_per_particle() for inst in insts]
+    finite = [v for v in e if math.isfinite(v)]
+    dims = np.array([d.dim for d in draws])
+    out = {"workload": f"{len(insts)} uneven imaginary-time ground states x {K} steps (dt=0.005, m=5)",
+           "instances": len(insts), "steps_each": K, "streams": args.batch_streams,
+           "instance_steps_per_s": len(insts) * K / wall, "wall_s": wall, "setup_s": setup_s,
+           "comparison_steps_each": K2, "one_after_another_s": serial, "together_s": together,
+           "concurrency_gain": serial / together,
+           "dim_min_median_max": [int(dims.min()), int(np.median(dims)), int(dims.max())],
+           "mixed_instances": int(sum(1 for i in insts if i.prec != 0)),
+           "instances_with_finite_energy": len(finite),
+           "energy_per_particle_first8": e[:8]}
+    for inst in insts:
+        inst.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -262,6 +318,11 @@ def main():
                     help="skip the whole-step (orbitals + SIL + densities) eager/graph timing")
     ap.add_argument("--no-mixed-shapes", action="store_true",
                     help="skip the mixed-vs-FP64 comparison on (10,4), (200,4), (30,7), (100,5) with m=8")
+    ap.add_argument("--batch-instances", type=int, default=128)
+    ap.add_argument("--batch-steps", type=int, default=50)
+    ap.add_argument("--batch-streams", type=int, default=4)
+    ap.add_argument("--no-batched", action="store_true", help="skip the many-instance timing")
+    ap.add_argument("--batched-only", action="store_true", help="only the many-instance timing (its JSON object)")
     ap.add_argument("--full-step-only", action="store_true",
                     help="only the whole-step timing (prints its JSON object)")
     ap.add_argument("--probe-hc", action="store_true",
@@ -279,6 +340,9 @@ def main():
 
     if args.full_step_only:
         print(json.dumps(full_step_bench(args, torch.cuda.current_stream())), flush=True)
+        return
+    if args.batched_only:
+        print(json.dumps(batched_bench(args)), flush=True)
         return
 
     N, M, lm, dt = args.npar, args.norb, args.lanczos, args.dt
@@ -419,6 +483,7 @@ def main():
 
     full = full_step_bench(args, stream) if not args.no_full_step else None
     shapes = mixed_shapes_bench(args, stream) if not args.no_mixed_shapes else None
+    batched = batched_bench(args) if not args.no_batched else None
 
     cpu = None
     if not args.no_cpu:
@@ -465,6 +530,7 @@ def main():
         "cpu_baseline": cpu,
         "full_step": full,
         "mixed_vs_fp64_shapes": shapes,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
 
