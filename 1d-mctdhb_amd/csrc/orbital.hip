@@ -779,63 +779,82 @@ static int rk_tableau(OrbWork* w, int ns, const double* A, const double* b, doub
 
 // --------------------------------------------------------- Gram-Schmidt
 
-// One block: renormalise F_0, then for i >= 1 and j < i apply the
-// reference's point-by-point projection (calculus.c:170-191).  With
-// S_p = <F_j, F_i> evaluated after points < p were updated,
-// S_{p+1} = S_p (1 - w_p |F_j[p]|^2), so S_p = S_0 prod_{q<p} (1 - w_q |F_j[q]|^2):
-// one reduction and one multiplicative scan per (i, j).
+// One block.  The reference (calculus.c:170-191) renormalises F_0, then for
+// i >= 1 projects F_i point by point on each F_j, j < i, and renormalises
+// it.  Its quirk: the coefficient at point p is S_p = <F_j, F_i> evaluated
+// after points < p were updated, and S_{p+1} = S_p (1 - w_p |F_j[p]|^2), so
+// S_p = S_0 prod_{q<p} (1 - w_q |F_j[q]|^2).  The prefix products depend on
+// F_j alone, so the loop runs j-major: finish F_j (renormalise), form its
+// prefix products once, then project every later F_i on it (each F_i still
+// sees its projections in the order j = 0, 1, ..., i - 1).  Thread t owns
+// the points [t per, (t + 1) per): it alone reads and writes them, so the
+// only barriers are those of the block sums (double-buffered, one each).
 __device__ __forceinline__ void gram_schmidt_block(double2* __restrict__ F, int M, int n, double dx) {
-    __shared__ double red[64];
+    __shared__ double red[2][16];
     __shared__ double wave_prod[4];
     const int T = blockDim.x;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = T >> 6;
     const int per = (n + T - 1) / T;  // contiguous points per thread
-    const int p0 = threadIdx.x * per;
-    auto renorm = [&](double2* f) {
-        double v[1] = {0};
-        for (int p = threadIdx.x; p < n; p += T) v[0] += simpson_w(p, n, dx) * (f[p].x * f[p].x + f[p].y * f[p].y);
-        block_sum_bcast<1>(v, red);
-        const double s = sqrt(1.0 / v[0]);
-        for (int p = threadIdx.x; p < n; p += T) f[p] = cscale(f[p], s);
+    const int p0 = min((int) threadIdx.x * per, n), p1 = min(p0 + per, n);
+    int buf = 0;
+    // (a, b) summed over the block, the same value in every thread
+    auto sum2 = [&](double a, double b, double& sa, double& sb) {
+        a = wave_sum(a);
+        b = wave_sum(b);
+        double* r = red[buf];
+        buf ^= 1;
+        if (lane == 0) {
+            r[wave] = a;
+            r[8 + wave] = b;
+        }
         __syncthreads();
+        sa = 0;
+        sb = 0;
+        for (int w = 0; w < nw; w++) {
+            sa += r[w];
+            sb += r[8 + w];
+        }
     };
-    renorm(F);
-    for (int i = 1; i < M; i++) {
-        double2* Fi = F + (size_t) i * n;
-        for (int j = 0; j < i; j++) {
-            const double2* Fj = F + (size_t) j * n;
-            double v[2] = {0, 0};
-            for (int p = threadIdx.x; p < n; p += T) {
+    for (int j = 0; j < M; j++) {
+        double2* Fj = F + (size_t) j * n;
+        double nr = 0, tot, unused;
+        for (int p = p0; p < p1; p++) nr += simpson_w(p, n, dx) * (Fj[p].x * Fj[p].x + Fj[p].y * Fj[p].y);
+        sum2(nr, 0.0, tot, unused);
+        const double sc = sqrt(1.0 / tot);
+        for (int p = p0; p < p1; p++) Fj[p] = cscale(Fj[p], sc);
+        if (j == M - 1) break;
+        // exclusive product of (1 - w_q |F_j[q]|^2) over the points before
+        // this thread's first point: own chunk, wave scan, then the waves
+        double prod = 1.0;
+        for (int p = p0; p < p1; p++) prod *= 1.0 - simpson_w(p, n, dx) * (Fj[p].x * Fj[p].x + Fj[p].y * Fj[p].y);
+        double incl = prod;
+        for (int off = 1; off < 64; off <<= 1) {
+            const double o = __shfl_up(incl, off, 64);
+            if (lane >= off) incl *= o;
+        }
+        if (lane == 63) wave_prod[wave] = incl;
+        double run0 = __shfl_up(incl, 1, 64);
+        if (lane == 0) run0 = 1.0;
+        __syncthreads();
+        for (int u = 0; u < wave; u++) run0 *= wave_prod[u];
+        for (int i = j + 1; i < M; i++) {
+            double2* Fi = F + (size_t) i * n;
+            double a = 0, b = 0;
+            for (int p = p0; p < p1; p++) {
                 const double2 t = cconjmul(Fj[p], Fi[p]);
                 const double wp = simpson_w(p, n, dx);
-                v[0] += wp * t.x;
-                v[1] += wp * t.y;
+                a += wp * t.x;
+                b += wp * t.y;
             }
-            block_sum_bcast<2>(v, red);
-            const double2 S0 = make_double2(v[0], v[1]);
-            // exclusive multiplicative scan of (1 - w_p |F_j[p]|^2)
-            double prod = 1.0;
-            for (int p = p0; p < p0 + per && p < n; p++)
-                prod *= 1.0 - simpson_w(p, n, dx) * (Fj[p].x * Fj[p].x + Fj[p].y * Fj[p].y);
-            // inclusive product scan within the wave, then over the waves
-            double incl = prod;
-            for (int off = 1; off < 64; off <<= 1) {
-                const double o = __shfl_up(incl, off, 64);
-                if (lane >= off) incl *= o;
-            }
-            if (lane == 63) wave_prod[wave] = incl;
-            double run = __shfl_up(incl, 1, 64);
-            if (lane == 0) run = 1.0;
-            __syncthreads();
-            for (int u = 0; u < wave; u++) run *= wave_prod[u];
-            for (int p = p0; p < p0 + per && p < n; p++) {
-                const double2 coef = cscale(S0, run);
-                Fi[p] = Fi[p] - cmul(coef, Fj[p]);
+            double sa, sb;
+            sum2(a, b, sa, sb);
+            const double2 S0 = make_double2(sa, sb);
+            double run = run0;
+            for (int p = p0; p < p1; p++) {
+                Fi[p] = Fi[p] - cmul(cscale(S0, run), Fj[p]);
                 run *= 1.0 - simpson_w(p, n, dx) * (Fj[p].x * Fj[p].x + Fj[p].y * Fj[p].y);
             }
-            __syncthreads();
         }
-        renorm(Fi);
     }
 }
 
@@ -852,11 +871,12 @@ __global__ __launch_bounds__(256) void k_gram_schmidt(double2* __restrict__ Fg, 
     if (staged) {
         for (int t = threadIdx.x; t < tot; t += blockDim.x) Fs[t] = Fg[t];
         __syncthreads();
-        gram_schmidt_block(Fs, M, n, dx);  // ends with a barrier
+        gram_schmidt_block(Fs, M, n, dx);
         F = Fs;
     } else {
         gram_schmidt_block(Fg, M, n, dx);
     }
+    __syncthreads();  // every thread's points are final
     if (tail)
         for (int k = threadIdx.x; k < M; k += blockDim.x) F[(size_t) k * n + n - 1] = F[(size_t) k * n];
     if (staged) {

@@ -67,13 +67,25 @@ def draw_ground_states(count: int, seed: int = 11, max_dim: int = 200_000) -> li
     return out
 
 
+def footprint_bytes(dim: int, norb: int, grid: int, lanczos: int = 5) -> int:
+    """Device memory of one instance, upper estimate: coefficients and the
+    Krylov vectors (complex128), the H.C output, occupation rows, the
+    density partial sums, orbitals with their stage and FFT buffers, and a
+    fixed allowance for the small tables and matrices."""
+    return dim * (16 * (lanczos + 3) + 8) + (dim // 4096 + 2) * 16 * norb ** 4 + 16 * norb * grid * 24 + (1 << 20)
+
+
 class Instance:
     """One imaginary-time solve, state resident on the GPU, step captured
     as a graph.  Spaces up to the one-workgroup SIL size run the FP64 step
     (its one-workgroup form is the faster one there); larger ones the mixed
     form (complex64 H.C, FP64 Lanczos scalars)."""
 
-    def __init__(self, d: Draw, dt: float = 0.005, lanczos: int = 5, small_fp64_dim: int = 8192):
+    def __init__(self, d: Draw, dt: float = 0.005, lanczos: int = 5, small_fp64_dim: int = 8192,
+                 quench_g: float | None = None):
+        """quench_g: also capture a real-time step with interaction
+        quench_g on the same state (`step_real`), for an interaction quench
+        after the imaginary-time relaxation."""
         import torch
         import mctdhb_amd as amd
 
@@ -101,9 +113,20 @@ class Instance:
         torch.cuda.synchronize()
         self.graph = amd.StepGraph(self.space, self.sil, self.orbital, t["orb"], t["coef"], t["hob"], t["hint"],
                                    t["r1"], t["r2"], t["rinv"])
+        self.orbital_real = self.graph_real = None
+        if quench_g is not None:
+            self.orbital_real = amd.DeviceOrbital(M, grid, x[0], x[-1], dt, quench_g, pot, imagtime=False)
+            self.graph_real = amd.StepGraph(self.space, self.sil, self.orbital_real, t["orb"], t["coef"], t["hob"],
+                                            t["hint"], t["r1"], t["r2"], t["rinv"])
 
     def step(self, stream=None):
         self.graph.launch(stream)
+
+    def step_real(self, stream=None):
+        self.graph_real.launch(stream)
+
+    def footprint_bytes(self) -> int:
+        return footprint_bytes(self.draw.dim, self.draw.norb, self.draw.ngrid + 1)
 
     def energy_per_particle(self) -> float:
         """E / N from rho1, rho2, h and g (reference total_energy,
@@ -125,6 +148,8 @@ class Instance:
 
     def close(self):
         self.graph.close()
+        if self.graph_real is not None:
+            self.graph_real.close()
         self.sil.close()
         self.space.close()
 

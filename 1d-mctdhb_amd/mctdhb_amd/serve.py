@@ -1,0 +1,192 @@
+"""A request loop over many MCTDHB solves on one GPU (continuous batching).
+
+Requests arrive over time.  A request is admitted as soon as it has arrived
+and its device footprint fits in what is left of a fixed HBM budget; from
+then on its time steps are replayed (captured graphs, mctdhb_amd.batch)
+together with those of every other admitted request, a few steps per
+request per pass, round-robin over a few streams.  A request that has
+enqueued its last step is retired when the event recorded after that step
+has completed: its result is read, its memory returned to the budget, and
+its time to solution (completion minus arrival) recorded.  Requests share no
+buffers, so each result is the same as when the request runs alone.
+
+Two kinds, as in the served mix: a ground state (imaginary-time steps) and
+an interaction quench (imaginary-time relaxation, then g scaled by a factor
+and real-time steps to a final time).
+"""
+from __future__ import annotations
+
+import math
+import time
+from collections import deque
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import batch, inputs
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float          # seconds after the start of the loop
+    kind: str               # "ground" or "quench"
+    draw: batch.Draw
+    imag_steps: int
+    real_steps: int = 0
+    g_factor: float = 1.0
+
+    @property
+    def footprint(self) -> int:
+        return batch.footprint_bytes(self.draw.dim, self.draw.norb, self.draw.ngrid + 1)
+
+
+def npar_for_dim(target: float, norb: int, max_npar: int = 10_000) -> int:
+    """Smallest N with C(N + M - 1, M - 1) >= target (at most max_npar)."""
+    lo, hi = 1, max_npar
+    while lo < hi:
+        mid = (lo + hi) // 2
+        if math.comb(mid + norb - 1, norb - 1) >= target:
+            hi = mid
+        else:
+            lo = mid + 1
+    return lo
+
+
+def draw_requests(rate: float, window: float, seed: int = 13, time_scale: float = 1.0, dt: float = 0.005,
+                  dim_range=(1e2, 5e6)) -> list[Request]:
+    """Poisson arrivals at `rate` per second during `window` seconds; 70 %
+    ground states (500 imaginary-time steps), 30 % quenches (200
+    imaginary-time steps, then g times U[0.5, 2] and real time to
+    T ~ U[1, 10]); NC log-uniform in dim_range with M in {2..7} and N chosen
+    to reach it (N <= 4000: the fused H.C kernel keeps rank and square-root
+    tables of N + 5 entries in LDS); Ng in {512, 1024, 2048}; traps and g as in
+    batch.draw_ground_states.  time_scale multiplies every step count (a
+    shortened sample keeps the same mix)."""
+    r = inputs.rng(seed, 5)
+    out, t = [], 0.0
+    while True:
+        t += float(r.exponential(1.0 / rate))
+        if t >= window:
+            return out
+        kind = "ground" if r.random() < 0.7 else "quench"
+        target = math.exp(r.uniform(math.log(dim_range[0]), math.log(dim_range[1])))
+        m = int(r.integers(2, 8))
+        n = npar_for_dim(target, m, max_npar=4000)
+        ng = int(r.choice([512, 1024, 2048]))
+        a, b = float(r.uniform(0.5, 2.0)), float(r.uniform(1.0, 3.0))
+        g = float(10 ** r.uniform(-2.0, 1.0))
+        f, T = float(r.uniform(0.5, 2.0)), float(r.uniform(1.0, 10.0))
+        d = batch.Draw(n, m, ng, a, b, g, seed * 1000003 + len(out))
+        if kind == "ground":
+            req = Request(len(out), t, kind, d, max(1, round(500 * time_scale)))
+        else:
+            req = Request(len(out), t, kind, d, max(1, round(200 * time_scale)),
+                          max(1, round(T / dt * time_scale)), f)
+        out.append(req)
+
+
+class _Active:
+    def __init__(self, req: Request, admitted: float):
+        import torch
+
+        self.req = req
+        self.admitted = admitted
+        quench_g = req.draw.g * req.g_factor if req.kind == "quench" else None
+        self.inst = batch.Instance(req.draw, quench_g=quench_g)
+        self.done_steps = 0
+        self.total = req.imag_steps + req.real_steps
+        self.event = torch.cuda.Event()
+        self.enqueued_all = False
+
+    def advance(self, stream, k: int):
+        for _ in range(min(k, self.total - self.done_steps)):
+            if self.done_steps < self.req.imag_steps:
+                self.inst.step(stream)
+            else:
+                self.inst.step_real(stream)
+            self.done_steps += 1
+        if self.done_steps == self.total and not self.enqueued_all:
+            self.event.record(stream)
+            self.enqueued_all = True
+
+
+class Server:
+    """budget_bytes: HBM for the admitted requests' state; chunk: steps per
+    request per pass."""
+
+    def __init__(self, budget_bytes: int, nstreams: int = 4, chunk: int = 8):
+        import torch
+
+        self.budget = budget_bytes
+        self.streams = [torch.cuda.Stream() for _ in range(max(1, nstreams))]
+        self.chunk = chunk
+
+    def run(self, requests: list[Request], keep_state: bool = False) -> list[dict]:
+        """Serve `requests` (arrival times relative to the call); returns one
+        record per request, in completion order."""
+        import torch
+
+        pending = deque(sorted(requests, key=lambda q: q.arrival))
+        for q in pending:
+            if q.footprint > self.budget:
+                raise ValueError(f"request {q.rid} needs {q.footprint} bytes, above the whole budget")
+        active: list[_Active] = []
+        used, peak_active, out = 0, 0, []
+        t0 = time.perf_counter()
+        while pending or active:
+            now = time.perf_counter() - t0
+            while pending and pending[0].arrival <= now and used + pending[0].footprint <= self.budget:
+                q = pending.popleft()
+                active.append(_Active(q, time.perf_counter() - t0))
+                used += q.footprint
+            peak_active = max(peak_active, len(active))
+            if not active:
+                time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
+                continue
+            for i, a in enumerate(active):
+                if not a.enqueued_all:
+                    a.advance(self.streams[i % len(self.streams)], self.chunk)
+            still = []
+            for a in active:
+                if a.enqueued_all and a.event.query():
+                    done = time.perf_counter() - t0
+                    rec = {"rid": a.req.rid, "kind": a.req.kind, "dim": a.req.draw.dim, "norb": a.req.draw.norb,
+                           "steps": a.total, "arrival": a.req.arrival, "admitted": a.admitted, "done": done,
+                           "time_to_solution": done - a.req.arrival,
+                           "energy_per_particle": a.inst.energy_per_particle()}
+                    if keep_state:
+                        rec["state"] = {k: v.cpu().numpy().copy() for k, v in a.inst.t.items()}
+                    out.append(rec)
+                    a.inst.close()
+                    used -= a.req.footprint
+                else:
+                    still.append(a)
+            if len(still) == len(active) and all(a.enqueued_all for a in active):
+                torch.cuda.synchronize()  # only completions are left to wait for
+            active = still
+        self.peak_active = peak_active
+        return out
+
+
+def run_alone(req: Request) -> dict:
+    """The same request on its own (default stream), for comparisons."""
+    import torch
+
+    quench_g = req.draw.g * req.g_factor if req.kind == "quench" else None
+    inst = batch.Instance(req.draw, quench_g=quench_g)
+    for _ in range(req.imag_steps):
+        inst.step()
+    for _ in range(req.real_steps):
+        inst.step_real()
+    torch.cuda.synchronize()
+    st = {k: v.cpu().numpy().copy() for k, v in inst.t.items()}
+    e = inst.energy_per_particle()
+    inst.close()
+    return {"state": st, "energy_per_particle": e}
+
+
+def latency_summary(records: list[dict]) -> dict:
+    tts = np.array([r["time_to_solution"] for r in records])
+    return {"p50_s": float(np.percentile(tts, 50)), "p99_s": float(np.percentile(tts, 99)),
+            "max_s": float(tts.max())}

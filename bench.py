@@ -301,6 +301,43 @@ def batched_bench(args):
     return out
 
 
+def serve_bench(args):
+    """The request loop (mctdhb_amd.serve): Poisson arrivals at 200/s for a
+    window, 70 % ground states and 30 % interaction quenches, NC
+    log-uniform in [1e2, 5e6], admitted against a fixed HBM budget and run
+    together; every step count is scaled by --serve-time-scale."""
+    from mctdhb_amd import serve
+
+    reqs = serve.draw_requests(args.serve_rate, args.serve_window, seed=args.seed,
+                               time_scale=args.serve_time_scale)
+    srv = serve.Server(int(args.serve_budget_gb * 1e9), nstreams=4, chunk=8)
+    t0 = time.perf_counter()
+    recs = srv.run(reqs)
+    wall = time.perf_counter() - t0
+    steps = sum(r["steps"] for r in recs)
+    lat = serve.latency_summary(recs)
+    log(f"served {len(recs)} requests in {wall:.2f} s: time to solution p50 {lat['p50_s']:.3f} s, "
+        f"p99 {lat['p99_s']:.3f} s, at most {srv.peak_active} at once")
+    dims = np.array([r["dim"] for r in recs])
+    # interaction scale g (N - 1) of the requests whose energy stayed finite
+    # and of those whose did not (the split step's Runge-Kutta sub-step
+    # holds the interaction term, stable only for g (N - 1) |phi|^2 dt small)
+    by_id = {q.rid: q for q in reqs}
+    gn = {True: [], False: []}
+    for r in recs:
+        d = by_id[r["rid"]].draw
+        gn[bool(np.isfinite(r["energy_per_particle"]))].append(d.g * (d.npar - 1))
+    return {"workload": f"{len(recs)} requests, Poisson {args.serve_rate}/s for {args.serve_window} s, "
+                        f"step counts x{args.serve_time_scale}",
+            "requests": len(recs), "quenches": sum(r["kind"] == "quench" for r in recs),
+            "wall_s": wall, "solves_per_s": len(recs) / wall, "instance_steps": steps,
+            "instance_steps_per_s": steps / wall, "time_to_solution": lat, "peak_admitted": srv.peak_active,
+            "budget_gb": args.serve_budget_gb,
+            "dim_min_median_max": [int(dims.min()), int(np.median(dims)), int(dims.max())],
+            "finite_energies": int(sum(np.isfinite(r["energy_per_particle"]) for r in recs)),
+            "g_times_n_max_finite": max(gn[True], default=None), "g_times_n_min_not_finite": min(gn[False], default=None)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -323,6 +360,12 @@ def main():
     ap.add_argument("--batch-streams", type=int, default=4)
     ap.add_argument("--no-batched", action="store_true", help="skip the many-instance timing")
     ap.add_argument("--batched-only", action="store_true", help="only the many-instance timing (its JSON object)")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-window", type=float, default=1.0)
+    ap.add_argument("--serve-time-scale", type=float, default=0.02)
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
+    ap.add_argument("--no-serve", action="store_true", help="skip the request-loop timing")
+    ap.add_argument("--serve-only", action="store_true", help="only the request loop (its JSON object)")
     ap.add_argument("--full-step-only", action="store_true",
                     help="only the whole-step timing (prints its JSON object)")
     ap.add_argument("--probe-hc", action="store_true",
@@ -343,6 +386,9 @@ def main():
         return
     if args.batched_only:
         print(json.dumps(batched_bench(args)), flush=True)
+        return
+    if args.serve_only:
+        print(json.dumps(serve_bench(args)), flush=True)
         return
 
     N, M, lm, dt = args.npar, args.norb, args.lanczos, args.dt
@@ -484,6 +530,7 @@ def main():
     full = full_step_bench(args, stream) if not args.no_full_step else None
     shapes = mixed_shapes_bench(args, stream) if not args.no_mixed_shapes else None
     batched = batched_bench(args) if not args.no_batched else None
+    served = serve_bench(args) if not args.no_serve else None
 
     cpu = None
     if not args.no_cpu:
@@ -531,6 +578,7 @@ def main():
         "full_step": full,
         "mixed_vs_fp64_shapes": shapes,
         "batched": batched,
+        "served": served,
     }
     print(json.dumps(line), flush=True)
 
