@@ -92,9 +92,9 @@ __global__ __launch_bounds__(kReduceBlock) void k_reduce_scalars(const double* _
 
 
 // exp(-i T prop_dt) e_1 (reference coefficients_integration.c:104-145).  One wave.
-__global__ __launch_bounds__(64) void k_sil_tridiag(SilState* st, double dt_re, double dt_im) {
+__global__ __launch_bounds__(64) void k_sil_tridiag(SilState* st, double dt_re, double dt_im, int renorm) {
     __shared__ double z[kMaxKrylov * (kMaxKrylov + 1)];
-    sil_tridiag_wave(st, dt_re, dt_im, z, kMaxKrylov + 1);
+    sil_tridiag_wave(st, dt_re, dt_im, z, kMaxKrylov + 1, renorm);
 }
 
 // Ground-state mode (reference lowest_state_lanczos, multiconfig_lanczos.c:
@@ -494,7 +494,7 @@ int sil_step(Sil* L, double2* C, const double2* hob, const double2* hint, double
     }
     int rc = sil_run_lanczos(L, C, hob, hint, st);
     if (rc) return rc;
-    k_sil_tridiag<<<1, 64, 0, st>>>(L->d_st, dt_re, dt_im);
+    k_sil_tridiag<<<1, 64, 0, st>>>(L->d_st, dt_re, dt_im, imagtime ? 1 : 0);
     if (L->prec == kPrecFP64) recombine_dispatch<double2>(L, C, st);
     else recombine_dispatch<float2>(L, C, st);
     if (imagtime) {

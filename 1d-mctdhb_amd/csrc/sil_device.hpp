@@ -158,7 +158,15 @@ __device__ inline int tridiag_eig_wave(int n, double& d, double& e, double* zr) 
 // T = (Re alpha, beta), as reference coefficients_integration.c:104-145:
 // y = Z exp(-i D prop_dt) Z^T e_1.  Called by one whole wave (lanes 0..63);
 // z is LDS scratch of n rows of zstride doubles.
-__device__ inline void sil_tridiag_wave(SilState* st, double dt_re, double dt_im, double* z, int zstride) {
+//
+// renorm: the caller renormalises the result (imaginary time).  Then the
+// exponentials are taken relative to the dominant one, exp((d_k - d*) b)
+// with d* the eigenvalue where d b is largest: a common factor that the
+// renormalisation removes, and that keeps exp(d b) from underflowing to 0
+// for every k (then C' = 0 and the renormalisation would give NaN) when
+// |E| dt exceeds about 745.
+__device__ inline void sil_tridiag_wave(SilState* st, double dt_re, double dt_im, double* z, int zstride,
+                                        int renorm) {
     const int lane = threadIdx.x;
     const int n = st->iter_done;
     double d = lane < n ? st->alpha[lane].x : 0.0;
@@ -170,10 +178,13 @@ __device__ inline void sil_tridiag_wave(SilState* st, double dt_re, double dt_im
     }
     // lane k: aux_k = Z[0][k] exp(-i d_k prop_dt); exp(-i d dt) for complex
     // dt = a + ib is exp(d b) (cos(d a) - i sin(d a))
+    double shift = -INFINITY;  // max over k of d_k b
+    for (int j = 0; j < n; j++) shift = fmax(shift, rlane(d, j) * dt_im);
+    if (!renorm) shift = 0.0;
     double ax = 0, ay = 0;
     if (lane < n) {
         const double z0k = z[lane];  // row 0, column lane
-        const double mag = exp(d * dt_im), ph = -d * dt_re;
+        const double mag = exp(d * dt_im - shift), ph = -d * dt_re;
         ax = z0k * mag * cos(ph);
         ay = z0k * mag * sin(ph);
     }

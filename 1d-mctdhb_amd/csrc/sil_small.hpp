@@ -149,7 +149,7 @@ __global__ __launch_bounds__(kSmallSilBlock) void k_sil_small(RowCtx ctx, const 
         if (st->stop) break;
     }
     // exp(-i T prop_dt) e_1 on wave 0, then C' = sum_k y_k w_k
-    if (threadIdx.x < 64) sil_tridiag_wave(st, a.dt_re, a.dt_im, L.z, kMaxLanczos + 2);
+    if (threadIdx.x < 64) sil_tridiag_wave(st, a.dt_re, a.dt_im, L.z, kMaxLanczos + 2, a.imagtime);
     __syncthreads();
     for (uint32_t t = threadIdx.x; t < sizeof(SilState) / sizeof(double); t += blockDim.x)
         ((double*) a.st)[t] = ((const double*) st)[t];

@@ -182,6 +182,37 @@ def test_one_workgroup_sil_step_matches_multikernel_and_oracle(amd, orc, npar, n
     assert gu.rel_err(gu.host(a), ref) < 1e-10
 
 
+@pytest.mark.parametrize("npar,norb,small", [(10, 4, True), (10, 4, False), (40, 5, False)])
+def test_imaginary_time_sil_invariant_under_energy_shift(amd, orc, npar, norb, small):
+    """In imaginary time the renormalised SIL step is unchanged by H -> H + c
+    (h -> h + c/N I adds c to every diagonal element).  With c dt = 900 the
+    plain exponentials exp(-E dt) underflow for every Krylov component and
+    the oracle's step (the reference's formula) gives no finite result; the
+    device takes them relative to the dominant one and must still agree with
+    the unshifted step, which agrees with the oracle."""
+    print(f"sil shift N={npar} M={norb} small={small}", flush=True)
+    lm, dt = 6, -0.005j
+    c, h, g = _problem(npar, norb, 43)
+    S = orc.Space(npar, norb)
+    _, ref = S.sil_step(c, h, g, lm, dt, True)
+    shift = 900.0 / 0.005  # total energy shift
+    hs = h + np.eye(norb) * (shift / npar)
+    _, bad = S.sil_step(c, hs, g, lm, dt, True)
+    assert not np.all(np.isfinite(bad))  # the reference formula breaks down here
+    D = amd.DeviceSpace(npar, norb, tables=False)
+    out = {}
+    for name, hh in (("plain", h), ("shifted", hs)):
+        sil = amd.DeviceSil(D, lm, prec=amd.FP64, index=amd.INDEX_FLY)
+        sil.set_small(small)
+        a = gu.dev(c)
+        sil.step(a, gu.dev(np.ascontiguousarray(hh)), gu.dev(g), dt, True)
+        out[name] = gu.host(a)
+        sil.close()
+    assert gu.rel_err(out["plain"], ref) < 1e-10
+    assert np.all(np.isfinite(out["shifted"]))
+    assert gu.rel_err(out["shifted"], out["plain"]) < 1e-8
+
+
 @pytest.mark.parametrize("npar,norb,lm,prec", [(6, 4, 8, 0), (10, 5, 12, 0), (10, 5, 12, 1), (3, 9, 6, 0),
                                                (12, 5, 40, 0), (12, 5, 24, 1)])
 def test_lowest_state_matches_oracle(amd, orc, npar, norb, lm, prec):
