@@ -113,14 +113,16 @@ class _Active:
 
 class Server:
     """budget_bytes: HBM for the admitted requests' state; chunk: steps per
-    request per pass."""
+    request per pass; max_active: at most this many admitted at once (None:
+    only the budget limits), which trades sharing for earlier completions."""
 
-    def __init__(self, budget_bytes: int, nstreams: int = 4, chunk: int = 8):
+    def __init__(self, budget_bytes: int, nstreams: int = 4, chunk: int = 8, max_active: int | None = None):
         import torch
 
         self.budget = budget_bytes
         self.streams = [torch.cuda.Stream() for _ in range(max(1, nstreams))]
         self.chunk = chunk
+        self.max_active = max_active
 
     def run(self, requests: list[Request], keep_state: bool = False) -> list[dict]:
         """Serve `requests` (arrival times relative to the call); returns one
@@ -136,7 +138,8 @@ class Server:
         t0 = time.perf_counter()
         while pending or active:
             now = time.perf_counter() - t0
-            while pending and pending[0].arrival <= now and used + pending[0].footprint <= self.budget:
+            while (pending and pending[0].arrival <= now and used + pending[0].footprint <= self.budget
+                   and (self.max_active is None or len(active) < self.max_active)):
                 q = pending.popleft()
                 active.append(_Active(q, time.perf_counter() - t0))
                 used += q.footprint

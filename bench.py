@@ -310,7 +310,8 @@ def serve_bench(args):
 
     reqs = serve.draw_requests(args.serve_rate, args.serve_window, seed=args.seed,
                                time_scale=args.serve_time_scale)
-    srv = serve.Server(int(args.serve_budget_gb * 1e9), nstreams=4, chunk=8)
+    srv = serve.Server(int(args.serve_budget_gb * 1e9), nstreams=4, chunk=8,
+                       max_active=args.serve_max_active or None)
     t0 = time.perf_counter()
     recs = srv.run(reqs)
     wall = time.perf_counter() - t0
@@ -332,7 +333,7 @@ def serve_bench(args):
             "requests": len(recs), "quenches": sum(r["kind"] == "quench" for r in recs),
             "wall_s": wall, "solves_per_s": len(recs) / wall, "instance_steps": steps,
             "instance_steps_per_s": steps / wall, "time_to_solution": lat, "peak_admitted": srv.peak_active,
-            "budget_gb": args.serve_budget_gb,
+            "budget_gb": args.serve_budget_gb, "max_active": args.serve_max_active or None,
             "dim_min_median_max": [int(dims.min()), int(np.median(dims)), int(dims.max())],
             "finite_energies": int(sum(np.isfinite(r["energy_per_particle"]) for r in recs)),
             "g_times_n_max_finite": max(gn[True], default=None), "g_times_n_min_not_finite": min(gn[False], default=None)}
@@ -364,6 +365,7 @@ def main():
     ap.add_argument("--serve-window", type=float, default=1.0)
     ap.add_argument("--serve-time-scale", type=float, default=0.02)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
+    ap.add_argument("--serve-max-active", type=int, default=0, help="admitted at once (0: budget only)")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-loop timing")
     ap.add_argument("--serve-only", action="store_true", help="only the request loop (its JSON object)")
     ap.add_argument("--full-step-only", action="store_true",
