@@ -147,9 +147,12 @@ class Instance:
         return np.sort(np.linalg.eigvalsh(r1))[::-1] / self.draw.npar
 
     def close(self):
+        """Frees the graphs first, then the workspaces they refer to."""
         self.graph.close()
         if self.graph_real is not None:
             self.graph_real.close()
+            self.orbital_real.close()
+        self.orbital.close()
         self.sil.close()
         self.space.close()
 
