@@ -86,17 +86,37 @@ def draw_requests(rate: float, window: float, seed: int = 13, time_scale: float 
         out.append(req)
 
 
-class _Active:
-    def __init__(self, req: Request, admitted: float):
-        import torch
+def _gpu_instance(req: Request):
+    quench_g = req.draw.g * req.g_factor if req.kind == "quench" else None
+    return batch.Instance(req.draw, quench_g=quench_g)
 
+
+def _gpu_event():
+    import torch
+
+    return torch.cuda.Event()
+
+
+def _gpu_stream():
+    import torch
+
+    return torch.cuda.Stream()
+
+
+def _gpu_wait_all():
+    import torch
+
+    torch.cuda.synchronize()
+
+
+class _Active:
+    def __init__(self, req: Request, admitted: float, make_instance, make_event):
         self.req = req
         self.admitted = admitted
-        quench_g = req.draw.g * req.g_factor if req.kind == "quench" else None
-        self.inst = batch.Instance(req.draw, quench_g=quench_g)
+        self.inst = make_instance(req)
         self.done_steps = 0
         self.total = req.imag_steps + req.real_steps
-        self.event = torch.cuda.Event()
+        self.event = make_event()
         self.enqueued_all = False
 
     def advance(self, stream, k: int):
@@ -116,33 +136,35 @@ class Server:
     request per pass; max_active: at most this many admitted at once (None:
     only the budget limits), which trades sharing for earlier completions."""
 
-    def __init__(self, budget_bytes: int, nstreams: int = 4, chunk: int = 8, max_active: int | None = None):
-        import torch
-
+    def __init__(self, budget_bytes: int, nstreams: int = 4, chunk: int = 8, max_active: int | None = None,
+                 make_instance=_gpu_instance, make_event=_gpu_event, make_stream=_gpu_stream,
+                 wait_all=_gpu_wait_all):
+        """make_instance / make_event / make_stream / wait_all: the GPU
+        objects by default; a test of the loop's host logic passes stand-ins."""
         self.budget = budget_bytes
-        self.streams = [torch.cuda.Stream() for _ in range(max(1, nstreams))]
+        self.streams = [make_stream() for _ in range(max(1, nstreams))]
         self.chunk = chunk
         self.max_active = max_active
+        self._make_instance, self._make_event, self._wait_all = make_instance, make_event, wait_all
 
     def run(self, requests: list[Request], keep_state: bool = False) -> list[dict]:
         """Serve `requests` (arrival times relative to the call); returns one
         record per request, in completion order."""
-        import torch
-
         pending = deque(sorted(requests, key=lambda q: q.arrival))
         for q in pending:
             if q.footprint > self.budget:
                 raise ValueError(f"request {q.rid} needs {q.footprint} bytes, above the whole budget")
         active: list[_Active] = []
-        used, peak_active, out = 0, 0, []
+        used, peak_active, peak_used, out = 0, 0, 0, []
         t0 = time.perf_counter()
         while pending or active:
             now = time.perf_counter() - t0
             while (pending and pending[0].arrival <= now and used + pending[0].footprint <= self.budget
                    and (self.max_active is None or len(active) < self.max_active)):
                 q = pending.popleft()
-                active.append(_Active(q, time.perf_counter() - t0))
+                active.append(_Active(q, time.perf_counter() - t0, self._make_instance, self._make_event))
                 used += q.footprint
+                peak_used = max(peak_used, used)
             peak_active = max(peak_active, len(active))
             if not active:
                 time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
@@ -166,9 +188,9 @@ class Server:
                 else:
                     still.append(a)
             if len(still) == len(active) and all(a.enqueued_all for a in active):
-                torch.cuda.synchronize()  # only completions are left to wait for
+                self._wait_all()  # only completions are left to wait for
             active = still
-        self.peak_active = peak_active
+        self.peak_active, self.peak_used = peak_active, peak_used
         return out
 
 

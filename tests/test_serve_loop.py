@@ -1,0 +1,95 @@
+"""Host logic of the request loop (mctdhb_amd.serve.Server) with stand-in
+instances, events and streams: admission only after arrival and within the
+budget and the cap, every step enqueued in order (imaginary then real),
+retirement only once the last step's event reports done."""
+import time
+
+from mctdhb_amd import batch, serve
+
+
+class FakeEvent:
+    def __init__(self):
+        self.recorded = False
+        self.polls = 0
+
+    def record(self, stream):
+        self.recorded = True
+
+    def query(self):
+        # done on the second poll after recording (work takes a moment)
+        if self.recorded:
+            self.polls += 1
+        return self.recorded and self.polls >= 2
+
+
+class FakeInstance:
+    log = []
+
+    def __init__(self, req):
+        self.req = req
+        self.steps = []
+        FakeInstance.log.append(("admit", req.rid, time.perf_counter()))
+
+    def step(self, stream=None):
+        self.steps.append(("imag", stream))
+
+    def step_real(self, stream=None):
+        self.steps.append(("real", stream))
+
+    def energy_per_particle(self):
+        return float(len(self.steps))
+
+    def close(self):
+        FakeInstance.log.append(("close", self.req.rid, time.perf_counter()))
+
+
+def _requests():
+    out = []
+    for i, (t, kind, fp_dim) in enumerate([(0.0, "ground", 1000), (0.0, "quench", 1000), (0.01, "ground", 1000),
+                                           (0.02, "quench", 50000), (0.03, "ground", 1000)]):
+        d = batch.Draw(10, 3, 256, 1.0, 2.0, 0.5, i)
+        q = serve.Request(i, t, kind, d, 5, 7 if kind == "quench" else 0, 1.5)
+        out.append(q)
+    return out
+
+
+def test_loop_admits_within_budget_and_cap_and_runs_every_step():
+    FakeInstance.log.clear()
+    reqs = _requests()
+    fps = [q.footprint for q in reqs]
+    budget = sorted(fps)[-1] + sorted(fps)[-2]  # at most two at once
+    instances = {}
+
+    def make(req):
+        inst = FakeInstance(req)
+        instances[req.rid] = inst
+        return inst
+
+    srv = serve.Server(budget, nstreams=2, chunk=3, max_active=2, make_instance=make, make_event=FakeEvent,
+                       make_stream=object, wait_all=lambda: None)
+    recs = srv.run(reqs)
+    assert sorted(r["rid"] for r in recs) == [0, 1, 2, 3, 4]
+    assert srv.peak_active <= 2 and srv.peak_used <= budget
+    for q in reqs:
+        kinds = [k for k, _ in instances[q.rid].steps]
+        assert kinds == ["imag"] * q.imag_steps + ["real"] * q.real_steps
+    for r in recs:
+        assert r["energy_per_particle"] == r["steps"]
+        assert r["admitted"] >= r["arrival"] - 1e-9 and r["done"] >= r["admitted"]
+        assert r["time_to_solution"] == r["done"] - r["arrival"]
+    # every admission happened after its arrival and each instance was closed once
+    t_admit = {rid: t for ev, rid, t in FakeInstance.log if ev == "admit"}
+    closes = [rid for ev, rid, _ in FakeInstance.log if ev == "close"]
+    assert sorted(closes) == [0, 1, 2, 3, 4] and len(t_admit) == 5
+
+
+def test_loop_rejects_a_request_above_the_whole_budget():
+    reqs = _requests()
+    srv = serve.Server(min(q.footprint for q in reqs) - 1, make_instance=FakeInstance, make_event=FakeEvent,
+                       make_stream=object, wait_all=lambda: None)
+    try:
+        srv.run(reqs)
+    except ValueError as e:
+        assert "above the whole budget" in str(e)
+    else:
+        raise AssertionError("expected a ValueError")
