@@ -416,10 +416,11 @@ __global__ __launch_bounds__(256) void k_nonlinear(const double2* __restrict__ o
 //             coefficient * hint[...] as in the loops above.
 // ql runs over q = 0..M-1, l = q..M-1.  Layout: A (M * P * M entries,
 // P = M (M + 1) / 2), then L (M * M).  One block.
-__global__ __launch_bounds__(256) void k_nl_coef(int M, const double2* __restrict__ hob,
-                                                 const double2* __restrict__ hint, const double2* __restrict__ rho,
-                                                 const double2* __restrict__ rinv, double2* __restrict__ out) {
-    extern __shared__ double2 lpart[];  // M^2 P partial sums of L (M <= 8)
+// Every thread of the block calls it; lpart: LDS of M^2 P entries (M <= 8),
+// out: the tables (global memory or LDS).
+__device__ __forceinline__ void nl_coef_block(int M, const double2* __restrict__ hob, const double2* __restrict__ hint,
+                                              const double2* __restrict__ rho, const double2* __restrict__ rinv,
+                                              double2* __restrict__ out, double2* __restrict__ lpart) {
     const int mm = M * M, mmm = mm * M, P = M * (M + 1) / 2, nA = M * P * M;
     const bool split = M <= 8;
     // A entries; without hob (the full-equation step) only the A table is needed
@@ -480,6 +481,13 @@ __global__ __launch_bounds__(256) void k_nl_coef(int M, const double2* __restric
         for (int ql = 0; ql < P; ql++) acc = acc + lpart[(k * M + j) * P + ql];
         out[nA + e] = acc;
     }
+}
+
+__global__ __launch_bounds__(256) void k_nl_coef(int M, const double2* __restrict__ hob,
+                                                 const double2* __restrict__ hint, const double2* __restrict__ rho,
+                                                 const double2* __restrict__ rinv, double2* __restrict__ out) {
+    extern __shared__ double2 lpart[];  // M^2 P partial sums of L (M <= 8)
+    nl_coef_block(M, hob, hint, rho, rinv, out, lpart);
 }
 
 // LDS of k_nl_coef (the split L sums for M <= 8).
@@ -645,12 +653,31 @@ struct RkTab {
 // Block: 64 grid points x M orbitals; thread (k, x) forms orbital k's
 // right-hand side at x, the stage values of all orbitals at x are shared in
 // LDS between stages.
+// FOLD: each block first forms the tables of k_nl_coef itself (from hob,
+// hint, rho2, rinv) in LDS, in place of reading them from coef; used when
+// they fit beside the stage values.
+template <int M, int NS> constexpr bool nl_fold() {
+    constexpr int P = M * (M + 1) / 2;  // stage values, tables (A and L), L partial sums
+    return sizeof(double2) * (M * 64 * (NS + 1) + M * P * M + M * M + M * M * P) <= 60 * 1024;
+}
+
 template <int M, int NS>
 __global__ __launch_bounds__(64 * M) void k_nl_rk_point(double2* __restrict__ orb, int n, RkTab tab, double dt,
                                                         double g, double2 tf, const double* __restrict__ V,
-                                                        const double2* __restrict__ coef) {
+                                                        const double2* __restrict__ coef_g,
+                                                        const double2* __restrict__ hob,
+                                                        const double2* __restrict__ hint,
+                                                        const double2* __restrict__ rho2,
+                                                        const double2* __restrict__ rinv) {
     constexpr int P = M * (M + 1) / 2;
+    constexpr bool FOLD = nl_fold<M, NS>();
     __shared__ double2 y0[M][64], ks[NS][M][64];
+    __shared__ double2 tables[FOLD ? M * P * M + M * M : 1], lpart[FOLD ? M * M * P : 1];
+    const double2* coef = coef_g;
+    if constexpr (FOLD) {
+        nl_coef_block(M, hob, hint, rho2, rinv, tables, lpart);
+        coef = tables;  // the barrier below orders the table writes before use
+    }
     // k is uniform across the wave: the table reads below become scalar loads
     const int xl = threadIdx.x & 63, k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int x = blockIdx.x * 64 + xl;
@@ -723,10 +750,25 @@ static int nl_rk_pointwise(OrbWork* w, double2* orb, const double2* hob, const d
     if (w->rk_order == 5) ns = 6, A = kRk5A, b = kRk5B;
     for (int i = 0; i < ns * ns; i++) tab.a[i] = A[i];
     for (int i = 0; i < ns; i++) tab.b[i] = b[i];
-    int rc = nonlinear_prepare(w, hob, hint, rho2, rinv, st);
-    if (rc) return rc;
+    // the tables in global memory only where the kernel does not fold them
+    bool fold = false;
+    switch (M * 8 + ns) {
+#define FOLD_CASE(m)                                                  \
+    case m * 8 + 2: fold = nl_fold<m, 2>(); break;                      \
+    case m * 8 + 4: fold = nl_fold<m, 4>(); break;                      \
+    case m * 8 + 6: fold = nl_fold<m, 6>(); break;
+        FOLD_CASE(1) FOLD_CASE(2) FOLD_CASE(3) FOLD_CASE(4) FOLD_CASE(5) FOLD_CASE(6) FOLD_CASE(7) FOLD_CASE(8)
+#undef FOLD_CASE
+        default: break;
+    }
+    if (!fold) {
+        const int rc = nonlinear_prepare(w, hob, hint, rho2, rinv, st);
+        if (rc) return rc;
+    }
     const unsigned nb = div_up(n, 64);
-#define RKP(m, s) k_nl_rk_point<m, s><<<nb, 64 * m, 0, st>>>(orb, n, tab, q.tstep, q.g, q.time_fac, w->d_pot, w->d_nl)
+#define RKP(m, s)                                                                                                 \
+    k_nl_rk_point<m, s><<<nb, 64 * m, 0, st>>>(orb, n, tab, q.tstep, q.g, q.time_fac, w->d_pot, w->d_nl, hob, hint, \
+                                               rho2, rinv)
 #define RKP_M(m)                     \
     case m:                          \
         if (ns == 2) RKP(m, 2);      \
