@@ -35,7 +35,7 @@ int hc_tune_tiling(const Space* S, hipStream_t st) {
     }
     const int M = S->norb;
     float2 *cf = nullptr, *out = nullptr;
-    const size_t cfb = sizeof(float2) * cf_size(M);
+    const size_t cfb = sizeof(float2) * cf32_size(M);
     if (!hc_padded_copy(S, nullptr, st)) {
         set_error("padded staging of the coefficient vector failed");
         return kDeviceError;
@@ -94,9 +94,14 @@ __global__ void k_hc_coefficients(int M, int N, const double2* __restrict__ hob,
     const int t = threadIdx.x;
     const int M2 = M * M, M3 = M2 * M;
     auto G = [&](int a, int b, int c, int d) { return hint[a + b * M + c * M2 + d * M3]; };
+    double2* d64 = cf32 ? (double2*) (cf32 + cf_f64_off(M)) : nullptr;
     auto put = [&](int idx, double2 v) {
         cf64[idx] = v;
-        if (cf32) cf32[idx] = make_float2((float) v.x, (float) v.y);
+        if (cf32) {
+            cf32[idx] = make_float2((float) v.x, (float) v.y);
+            // the one-body and diagonal sections once more in FP64
+            if (idx < cf_f64_count(M)) d64[idx] = v;
+        }
     };
     const int total = cf_size(M);
     for (int idx = t; idx < total; idx += blockDim.x) {
@@ -367,7 +372,7 @@ int hc_apply(const Space* S, const void* coef, const double2* hob, const double2
     double2* cf64 = nullptr;
     float2* cf32 = nullptr;
     MCTDHB_HIP_CHECK(hipMallocAsync((void**) &cf64, sizeof(double2) * cf_size(M), st));
-    if (prec == kPrecMixed) MCTDHB_HIP_CHECK(hipMallocAsync((void**) &cf32, sizeof(float2) * cf_size(M), st));
+    if (prec == kPrecMixed) MCTDHB_HIP_CHECK(hipMallocAsync((void**) &cf32, sizeof(float2) * cf32_size(M), st));
     rc = hc_coefficients(S, hob, hint, cf64, cf32, st);
     if (!rc) rc = dispatch_fused_plain(S, coef, prec == kPrecFP64 ? (const void*) cf64 : (const void*) cf32, hc,
                                        prec, index, st);
@@ -381,7 +386,7 @@ size_t hc_coef_bytes(const Space* S, int prec) {
     if (use_generic(S)) return sizeof(double2) * ((size_t) M * M + (size_t) M * M * M * M);
     // FP64 block, followed by the complex64 copy for the mixed form
     size_t b = sizeof(double2) * cf_size(M);
-    if (prec == kPrecMixed) b += sizeof(float2) * cf_size(M);
+    if (prec == kPrecMixed) b += sizeof(float2) * cf32_size(M);
     return b;
 }
 

@@ -46,11 +46,19 @@ __host__ __device__ constexpr int cf_off_d3(int M) { return cf_off_d2(M) + M; }
 __host__ __device__ constexpr int cf_off_ot(int M) { return cf_off_d3(M) + M * M; }
 __host__ __device__ constexpr int cf_off_tt(int M) { return cf_off_ot(M) + M * M * M; }
 __host__ __device__ constexpr int cf_size(int M) { return cf_off_tt(M) + M * M * M * M; }
+// The mixed-precision block: the complex64 copy of the block above, then
+// (16-byte aligned) its one-body and diagonal sections A, B, d1, d2, d3 again
+// in complex128, for the terms the mixed rows form in FP64.  Sizes in
+// complex64 elements.
+__host__ __device__ constexpr int cf_f64_off(int M) { return (cf_size(M) + 1) & ~1; }
+__host__ __device__ constexpr int cf_f64_count(int M) { return cf_off_ot(M); }
+__host__ __device__ constexpr int cf32_size(int M) { return cf_f64_off(M) + 2 * cf_f64_count(M); }
 
 constexpr int kFusedMaxOrb = 7;  // fused kernels instantiated for 2 <= M <= 7
 
 // Build the coefficient block from hob (M x M row-major) and hint (M^4),
-// both complex128 device arrays, into cf64 (and cf32 if non-null).
+// both complex128 device arrays, into cf64 (and the mixed block cf32, of
+// cf32_size(M) complex64 elements, if non-null).
 int hc_coefficients(const Space* S, const double2* hob, const double2* hint, double2* cf64, float2* cf32,
                     hipStream_t st);
 

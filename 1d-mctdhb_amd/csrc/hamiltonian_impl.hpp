@@ -49,6 +49,29 @@ template <typename R> __device__ __forceinline__ void cmac(cplx<R>& acc, const c
     acc.y += G.y * t.x;
 }
 
+// Diagonal coefficient E(n) = sum_k n_k (d1_k + n_k d2_k) + sum_{k<s} n_k n_s d3_ks
+// in the compute type T, from a block whose d1, d2, d3 sections start at
+// d (complex T entries, the layout of cf_off_d1 onwards).
+template <int M, typename T, typename ZT>
+__device__ __forceinline__ ZT diag_energy(const int* n, const ZT* __restrict__ d) {
+    ZT e = mk<T>(0, 0);
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        const T rk = (T) n[k];
+        const ZT d1 = d[k], d2 = d[M + k];
+        e.x += rk * (d1.x + rk * d2.x);
+        e.y += rk * (d1.y + rk * d2.y);
+#pragma unroll
+        for (int s = k + 1; s < M; s++) {
+            const ZT d3 = d[2 * M + k * M + s];
+            const T ns = rk * (T) n[s];
+            e.x += ns * d3.x;
+            e.y += ns * d3.y;
+        }
+    }
+    return e;
+}
+
 // Row i of H.C.  With HALF, the row's share of the Rayleigh quotient
 // <c, H c> instead: H is Hermitian and the jump types come in inverse pairs
 // (remove R, add A) / (remove A, add R), so sum_i conj(c_i) (H c)_i =
@@ -57,11 +80,17 @@ template <typename R> __device__ __forceinline__ void cmac(cplx<R>& acc, const c
 // (l < k for k -> l, q < k for the double jumps: this keeps the jumps into
 // orbitals 0 and 1, whose targets pair up for 16-byte loads).  The real
 // result is returned in .x.
+//
+// The row value is returned in FP64.  In the complex64 form (R = float) the
+// gathers, products and off-diagonal sums are FP32; the single-jump
+// coefficients F_kl(n) and the diagonal term E(n) c_i are formed in FP64 from
+// the block's FP64 sections (F rounded to FP32 once).
 template <int M, typename R, int IDX, typename Zin, bool HALF = false>
-__device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, const Zin* __restrict__ c,
+__device__ __forceinline__ double2 hc_row_fused(uint32_t i, const RowCtx& ctx, const Zin* __restrict__ c,
                                                 const cplx<R>* __restrict__ cf, const uint32_t* __restrict__ Wl,
                                                 int ldw, const R* __restrict__ sql, cplx<R>& ci_out) {
     using Z = cplx<R>;
+    constexpr bool F32 = std::is_same<R, float>::value;
     Cfg<M, R, IDX, coef_shift<Zin>()> x;
     x.template load_impl<true>(i, ctx, Wl, ldw, sql);
     const int* n = x.n;
@@ -69,24 +98,11 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
 #pragma unroll
     for (int k = 0; k < M; k++) rn[k] = (R) n[k];
 
-    // diagonal E(n)
-    Z e = mk<R>(0, 0);
-#pragma unroll
-    for (int k = 0; k < M; k++) {
-        const Z d1 = cf[cf_off_d1(M) + k], d2 = cf[cf_off_d2(M) + k];
-        e.x += rn[k] * (d1.x + rn[k] * d2.x);
-        e.y += rn[k] * (d1.y + rn[k] * d2.y);
-#pragma unroll
-        for (int s = k + 1; s < M; s++) {
-            const Z d3 = cf[cf_off_d3(M) + k * M + s];
-            const R ns = rn[k] * rn[s];
-            e.x += ns * d3.x;
-            e.y += ns * d3.y;
-        }
-    }
     const Z ci = gather_b<R, Zin>(c, x.is);
     ci_out = ci;
-    Z acc = HALF ? mk<R>(0, 0) : cmul(e, ci);
+    Z acc = mk<R>(0, 0);
+    if constexpr (!F32 && !HALF) acc = cmul(diag_energy<M, R, Z>(n, cf + cf_off_d1(M)), ci);
+    const double2* d64 = F32 ? (const double2*) (cf + cf_f64_off(M)) : nullptr;  // FP64 A, B, d1, d2, d3
 
     // Targets that differ only in whether one added particle lands in orbital
     // 0 or 1 have consecutive ranks (rank(n + e_1) = rank(n + e_0) + 1 when
@@ -110,18 +126,33 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
 #pragma unroll
             for (int l = 0; l < M; l++) {
                 if (l == k || (HALF && l > k)) continue;
-                Z F = cf[cf_off_A(M) + k * M + l];
-#pragma unroll
-                for (int m = 0; m < M - 1; m++) {
-                    const Z B = cf[cf_off_B(M) + (k * M + l) * M + m];
-                    F.x += rn[m] * B.x;
-                    F.y += rn[m] * B.y;
-                }
                 Z cj;
                 if (ADJ && k >= 2 && l <= 1) cj = l == 0 ? c_to0 : c_to1;
                 else cj = gather_b<R, Zin>(c, x.single(k, l));
-                const Z t = cscale(cj, x.s0[k] * x.s1[l]);
-                cmac<R>(acc, F, t);
+                Z F;
+                if constexpr (F32) {
+                    // F_kl(n) in FP64, rounded once: in FP32 the folded sum
+                    // cancels terms of size N |B| and its rounding (~1e-7 of
+                    // F, worse where F is small) moves the energy of a long
+                    // real-time run by ~1e-6 relative
+                    double2 Fd = d64[cf_off_A(M) + k * M + l];
+#pragma unroll
+                    for (int m = 0; m < M - 1; m++) {
+                        const double2 B = d64[cf_off_B(M) + (k * M + l) * M + m];
+                        Fd.x += (double) n[m] * B.x;
+                        Fd.y += (double) n[m] * B.y;
+                    }
+                    F = mk<R>((R) Fd.x, (R) Fd.y);
+                } else {
+                    F = cf[cf_off_A(M) + k * M + l];
+#pragma unroll
+                    for (int m = 0; m < M - 1; m++) {
+                        const Z B = cf[cf_off_B(M) + (k * M + l) * M + m];
+                        F.x += rn[m] * B.x;
+                        F.y += rn[m] * B.y;
+                    }
+                }
+                cmac<R>(acc, F, cscale(cj, x.s0[k] * x.s1[l]));
             }
         }
     }
@@ -193,12 +224,24 @@ __device__ __forceinline__ cplx<R> hc_row_fused(uint32_t i, const RowCtx& ctx, c
             }
         }
     }
-    if constexpr (HALF) {
-        // E_i |c_i|^2 + 2 Re(conj(c_i) acc)
-        const R n2 = ci.x * ci.x + ci.y * ci.y;
-        return mk<R>(e.x * n2 + 2 * (ci.x * acc.x + ci.y * acc.y), 0);
+    if constexpr (F32) {
+        const double2 e = diag_energy<M, double, double2>(n, d64 + cf_off_d1(M));
+        const double2 cd = make_double2(ci.x, ci.y);
+        const double2 off = make_double2(acc.x, acc.y);
+        if constexpr (HALF) {
+            // E_i |c_i|^2 + 2 Re(conj(c_i) acc)
+            const double n2 = cd.x * cd.x + cd.y * cd.y;
+            return make_double2(e.x * n2 + 2 * (cd.x * off.x + cd.y * off.y), 0);
+        }
+        return cmul(e, cd) + off;
+    } else {
+        if constexpr (HALF) {
+            const Z e = diag_energy<M, R, Z>(n, cf + cf_off_d1(M));
+            const R n2 = ci.x * ci.x + ci.y * ci.y;
+            return make_double2(e.x * n2 + 2 * (ci.x * acc.x + ci.y * acc.y), 0);
+        }
+        return acc;
     }
-    return acc;
 }
 
 // --------------------------------------------------------------- epilogues
@@ -321,7 +364,7 @@ __global__ __launch_bounds__(kHcBlock) void k_hc_fused(RowCtx ctx, const Zin* __
     double2 v = make_double2(0, 0), ci = make_double2(0, 0);
     if (valid) {
         cplx<R> own;
-        v = to_d2(hc_row_fused<M, R, IDX, Zin, HALF>(i, ctx, c, cf, Wl, ldw, sql, own));
+        v = hc_row_fused<M, R, IDX, Zin, HALF>(i, ctx, c, cf, Wl, ldw, sql, own);
         if constexpr (std::is_same<Zin, float2>::value) ci = to_d2(own);
     }
     epi(i, valid, v, ci, red);

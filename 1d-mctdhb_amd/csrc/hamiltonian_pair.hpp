@@ -40,8 +40,8 @@ __device__ __forceinline__ float2 hi2(const float4& v) { return make_float2(v.z,
 template <int M>
 __device__ __forceinline__ void hc_pair_fused(uint32_t i0, const RowCtx& ctx, __amdgpu_buffer_rsrc_t c,
                                               const float2* __restrict__ cf, const uint32_t* __restrict__ Wl,
-                                              int ldw, const float* __restrict__ sql, float2& out0,
-                                              float2& out1) {
+                                              int ldw, const float* __restrict__ sql, double2& out0,
+                                              double2& out1) {
     using Z = float2;
     // occupations of i0; those of i0 + 1 follow
     int na[M], nb[M];
@@ -57,12 +57,6 @@ __device__ __forceinline__ void hc_pair_fused(uint32_t i0, const RowCtx& ctx, __
     for (int k = 0; k < M; k++) nb[k] = na[k];
     nb[0] -= 1;
     nb[1] += 1;
-    float ra[M], rb[M];
-#pragma unroll
-    for (int k = 0; k < M; k++) {
-        ra[k] = (float) na[k];
-        rb[k] = (float) nb[k];
-    }
     // square roots sqrt(n-1), sqrt(n), sqrt(n+1), sqrt(n+2) (LDS table)
     float am[M], a0[M], a1[M], a2[M], bm[M], b0[M], b1[M], b2[M];
 #pragma unroll
@@ -113,27 +107,10 @@ __device__ __forceinline__ void hc_pair_fused(uint32_t i0, const RowCtx& ctx, __
         return j;
     };
 
-    // diagonal
-    Z ea = mk<float>(0, 0), eb = mk<float>(0, 0);
-#pragma unroll
-    for (int k = 0; k < M; k++) {
-        const Z d1 = cf[cf_off_d1(M) + k], d2 = cf[cf_off_d2(M) + k];
-        ea.x += ra[k] * (d1.x + ra[k] * d2.x);
-        ea.y += ra[k] * (d1.y + ra[k] * d2.y);
-        eb.x += rb[k] * (d1.x + rb[k] * d2.x);
-        eb.y += rb[k] * (d1.y + rb[k] * d2.y);
-#pragma unroll
-        for (int s = k + 1; s < M; s++) {
-            const Z d3 = cf[cf_off_d3(M) + k * M + s];
-            const float nsa = ra[k] * ra[s], nsb = rb[k] * rb[s];
-            ea.x += nsa * d3.x;
-            ea.y += nsa * d3.y;
-            eb.x += nsb * d3.x;
-            eb.y += nsb * d3.y;
-        }
-    }
-    const float4 cc = ld16(c, is);
-    Z acc0 = cmul(ea, lo2(cc)), acc1 = cmul(eb, hi2(cc));
+    // the off-diagonal sums in FP32; the diagonal terms are added in FP64 at
+    // the end (as in hc_row_fused)
+    Z acc0 = mk<float>(0, 0), acc1 = mk<float>(0, 0);
+    const double2* f64 = (const double2*) (cf + cf_f64_off(M));  // FP64 A, B, d1, d2, d3
 
     // single jumps k -> l; F_kl(i0 + 1) = F_kl(i0) - B'_kl0 + B'_kl1
 #pragma unroll
@@ -143,24 +120,26 @@ __device__ __forceinline__ void hc_pair_fused(uint32_t i0, const RowCtx& ctx, __
 #pragma unroll
             for (int l = 0; l < M; l++) {
                 if (l == k) continue;
-                Z F = cf[cf_off_A(M) + k * M + l];
+                // F in FP64, rounded once (as in hc_row_fused)
+                double2 Fd = f64[cf_off_A(M) + k * M + l];
 #pragma unroll
                 for (int m = 0; m < M - 1; m++) {
-                    const Z B = cf[cf_off_B(M) + (k * M + l) * M + m];
-                    F.x += ra[m] * B.x;
-                    F.y += ra[m] * B.y;
+                    const double2 B = f64[cf_off_B(M) + (k * M + l) * M + m];
+                    Fd.x += (double) na[m] * B.x;
+                    Fd.y += (double) na[m] * B.y;
                 }
-                Z Fb = F;
+                double2 Fbd = Fd;
                 {
-                    const Z B0 = cf[cf_off_B(M) + (k * M + l) * M + 0];
-                    Fb.x -= B0.x;
-                    Fb.y -= B0.y;
+                    const double2 B0 = f64[cf_off_B(M) + (k * M + l) * M + 0];
+                    Fbd.x -= B0.x;
+                    Fbd.y -= B0.y;
                     if (M > 2) {
-                        const Z B1 = cf[cf_off_B(M) + (k * M + l) * M + 1];
-                        Fb.x += B1.x;
-                        Fb.y += B1.y;
+                        const double2 B1 = f64[cf_off_B(M) + (k * M + l) * M + 1];
+                        Fbd.x += B1.x;
+                        Fbd.y += B1.y;
                     }
                 }
+                const Z F = mk<float>((float) Fd.x, (float) Fd.y), Fb = mk<float>((float) Fbd.x, (float) Fbd.y);
                 const uint32_t off = fly(k, -1, l, -1);
                 const float4 v = ld16(c, off);
                 cmac<float>(acc0, F, cscale(lo2(v), a0[k] * a1[l]));
@@ -228,8 +207,11 @@ __device__ __forceinline__ void hc_pair_fused(uint32_t i0, const RowCtx& ctx, __
             }
         }
     }
-    out0 = acc0;
-    out1 = acc1;
+    const float4 cc = ld16(c, is);
+    const double2* d64 = f64 + cf_off_d1(M);
+    const double2 ea = diag_energy<M, double, double2>(na, d64), eb = diag_energy<M, double, double2>(nb, d64);
+    out0 = cmul(ea, make_double2(cc.x, cc.y)) + make_double2(acc0.x, acc0.y);
+    out1 = cmul(eb, make_double2(cc.z, cc.w)) + make_double2(acc1.x, acc1.y);
 }
 
 // ------------------------------------------------------------- epilogues
@@ -331,14 +313,14 @@ __global__ __launch_bounds__(kHcBlock) void k_hc_pair(RowCtx ctx, PairCtx pc, co
     const bool has0 = slot < pc.n_slots;
     const bool has1 = slot < pc.n_pairs;
     const uint32_t i0 = has0 ? pc.slots[slot] : 0;
-    float2 r0 = make_float2(0, 0), r1 = make_float2(0, 0);
+    double2 r0 = make_double2(0, 0), r1 = make_double2(0, 0);
     if (has1) {
         hc_pair_fused<M>(i0, ctx, padded_rsrc(c, ctx.dim), cf, Wl, ldw, sql, r0, r1);
     } else if (has0) {
         float2 own;
         r0 = hc_row_fused<M, float, kIdxFly, float2>(i0, ctx, c, cf, Wl, ldw, sql, own);
     }
-    epi(i0, has0, has1, to_d2(r0), to_d2(r1), red);
+    epi(i0, has0, has1, r0, r1, red);
 }
 
 template <int M, typename Epi>

@@ -336,7 +336,8 @@ int sil_create(const Space* S, int lm, int prec, int index, Sil** out) {
                                    (size_t) kStreamGrid * (2 * kMaxKrylov + 2));
     bool ok = hipMalloc((void**) &L->d_st, sizeof(SilState)) == hipSuccess;
     ok = ok && hipMalloc((void**) &L->cf64, sizeof(double2) * cfn) == hipSuccess;
-    if (prec == kPrecMixed) ok = ok && hipMalloc((void**) &L->cf32, sizeof(float2) * cfn) == hipSuccess;
+    if (prec == kPrecMixed && !generic)
+        ok = ok && hipMalloc((void**) &L->cf32, sizeof(float2) * cf32_size(M)) == hipSuccess;
     ok = ok && hipMalloc((void**) &L->out, sizeof(double2) * n) == hipSuccess;
     ok = ok && hipMalloc((void**) &L->part, sizeof(double) * L->part_cap) == hipSuccess;
     ok = ok && hipMalloc((void**) &L->tot, sizeof(double) * (2 * kMaxKrylov + 4)) == hipSuccess;

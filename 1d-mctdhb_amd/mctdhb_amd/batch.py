@@ -141,6 +141,22 @@ class Instance:
         e = np.sum(r1 * h) + 0.5 * np.sum(r2 * g.transpose(1, 0, 2, 3))
         return float(e.real) / self.draw.npar
 
+    def health(self, tol: float = 1e-6) -> str:
+        """Empty when the state is usable, else the first problem found:
+        rho1 not finite, its trace not N, or coefficients not of unit norm
+        (every step keeps both, in imaginary time by renormalising and in
+        real time by unitarity).  Call after a synchronisation."""
+        M = self.draw.norb
+        r1 = self.t["r1"].cpu().numpy().reshape(M, M)
+        if not np.all(np.isfinite(r1)):
+            return "rho1 not finite"
+        if abs(np.trace(r1).real - self.draw.npar) > tol * self.draw.npar:
+            return "trace of rho1 is not N"
+        nrm = float(np.linalg.norm(self.t["coef"].cpu().numpy()))
+        if not abs(nrm - 1.0) <= tol:
+            return "coefficient norm is not 1"
+        return ""
+
     def occupations(self) -> np.ndarray:
         M = self.draw.norb
         r1 = self.t["r1"].cpu().numpy().reshape(M, M)

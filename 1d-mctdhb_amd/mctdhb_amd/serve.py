@@ -110,25 +110,44 @@ def _gpu_wait_all():
 
 
 class _Active:
-    def __init__(self, req: Request, admitted: float, make_instance, make_event):
+    """An admitted request.  All of its steps and its completion event go on
+    the one stream it is given at admission, so they run in order without
+    any cross-stream dependency."""
+
+    def __init__(self, req: Request, admitted: float, make_instance, make_event, stream):
         self.req = req
         self.admitted = admitted
         self.inst = make_instance(req)
+        self.stream = stream
         self.done_steps = 0
         self.total = req.imag_steps + req.real_steps
         self.event = make_event()
         self.enqueued_all = False
 
-    def advance(self, stream, k: int):
+    def advance(self, k: int):
         for _ in range(min(k, self.total - self.done_steps)):
             if self.done_steps < self.req.imag_steps:
-                self.inst.step(stream)
+                self.inst.step(self.stream)
             else:
-                self.inst.step_real(stream)
+                self.inst.step_real(self.stream)
             self.done_steps += 1
         if self.done_steps == self.total and not self.enqueued_all:
-            self.event.record(stream)
+            self.event.record(self.stream)
             self.enqueued_all = True
+
+
+def result_status(inst) -> tuple[str, str]:
+    """("ok", "") when the result is usable: finite energy, finite rho1 whose
+    trace is N, and coefficients of unit norm; else ("failed", reason)."""
+    e = inst.energy_per_particle()
+    if not math.isfinite(e):
+        return "failed", "energy not finite"
+    check = getattr(inst, "health", None)
+    if check is not None:
+        reason = check()
+        if reason:
+            return "failed", reason
+    return "ok", ""
 
 
 class Server:
@@ -156,30 +175,35 @@ class Server:
                 raise ValueError(f"request {q.rid} needs {q.footprint} bytes, above the whole budget")
         active: list[_Active] = []
         used, peak_active, peak_used, out = 0, 0, 0, []
+        admitted = 0  # round-robin stream assignment at admission
         t0 = time.perf_counter()
         while pending or active:
             now = time.perf_counter() - t0
             while (pending and pending[0].arrival <= now and used + pending[0].footprint <= self.budget
                    and (self.max_active is None or len(active) < self.max_active)):
                 q = pending.popleft()
-                active.append(_Active(q, time.perf_counter() - t0, self._make_instance, self._make_event))
+                stream = self.streams[admitted % len(self.streams)]
+                admitted += 1
+                active.append(_Active(q, time.perf_counter() - t0, self._make_instance, self._make_event, stream))
                 used += q.footprint
                 peak_used = max(peak_used, used)
             peak_active = max(peak_active, len(active))
             if not active:
                 time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - t0)))
                 continue
-            for i, a in enumerate(active):
+            for a in active:
                 if not a.enqueued_all:
-                    a.advance(self.streams[i % len(self.streams)], self.chunk)
+                    a.advance(self.chunk)
             still = []
             for a in active:
                 if a.enqueued_all and a.event.query():
                     done = time.perf_counter() - t0
+                    status, reason = result_status(a.inst)
                     rec = {"rid": a.req.rid, "kind": a.req.kind, "dim": a.req.draw.dim, "norb": a.req.draw.norb,
-                           "steps": a.total, "arrival": a.req.arrival, "admitted": a.admitted, "done": done,
-                           "time_to_solution": done - a.req.arrival,
-                           "energy_per_particle": a.inst.energy_per_particle()}
+                           "npar": a.req.draw.npar, "steps": a.total, "arrival": a.req.arrival,
+                           "admitted": a.admitted, "done": done, "time_to_solution": done - a.req.arrival,
+                           "energy_per_particle": a.inst.energy_per_particle(), "status": status,
+                           "reason": reason}
                     if keep_state:
                         rec["state"] = {k: v.cpu().numpy().copy() for k, v in a.inst.t.items()}
                     out.append(rec)
@@ -212,6 +236,9 @@ def run_alone(req: Request) -> dict:
 
 
 def latency_summary(records: list[dict]) -> dict:
-    tts = np.array([r["time_to_solution"] for r in records])
+    """Time to solution over the requests that solved (status "ok")."""
+    tts = np.array([r["time_to_solution"] for r in records if r.get("status", "ok") == "ok"])
+    if tts.size == 0:
+        return {"p50_s": None, "p99_s": None, "max_s": None, "count": 0}
     return {"p50_s": float(np.percentile(tts, 50)), "p99_s": float(np.percentile(tts, 99)),
-            "max_s": float(tts.max())}
+            "max_s": float(tts.max()), "count": int(tts.size)}

@@ -59,24 +59,137 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs), q))
 
 
+def physical_cores() -> int:
+    """Physical cores among the CPUs this process may run on: distinct
+    (package, core) pairs of /proc/cpuinfo over the affinity mask."""
+    try:
+        allowed = os.sched_getaffinity(0)
+    except AttributeError:
+        allowed = set(range(os.cpu_count() or 1))
+    cores, cpu, pkg = set(), None, 0
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                k, _, v = ln.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "processor":
+                    cpu, pkg = int(v), 0
+                elif k == "physical id":
+                    pkg = int(v)
+                elif k == "core id" and cpu in allowed:
+                    cores.add((pkg, int(v)))
+    except OSError:
+        pass
+    return len(cores) or len(allowed)
+
+
 def cpu_baseline(npar, norb, lm, dt, steps, seed):
-    """The oracle (reference algorithm restated in C/OpenMP) on this host."""
+    """The oracle (the reference's algorithm restated in C/OpenMP, its
+    table-driven H.C) on this host, timed around each SIL step with a
+    monotonic clock as the reference times its steps (driver.c:251, :284):
+    first on every physical core of this process's CPU set, then on half of
+    them, the reference's own default (apps/mctdhb_main.c:138)."""
     import oracle
     from mctdhb_amd import inputs
 
     if not os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so")):
         oracle.build()
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    phys = physical_cores()
     S = oracle.Space(npar, norb)
-    c = inputs.coefficients(S.dim, seed)
+    c0 = inputs.coefficients(S.dim, seed)
     h, g = inputs.onebody(norb, seed), inputs.twobody(norb, seed, 0.1)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        _, c = S.sil_step(c, h, g, lm, dt, False)
-    el = time.perf_counter() - t0
-    return {"value": steps / el, "unit": "steps/s", "cores": threads, "kind": "port",
+    runs = {}
+    for label, threads in (("all", phys), ("half", max(1, phys // 2))):
+        oracle.set_threads(threads)
+        c = c0
+        per = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            _, c = S.sil_step(c, h, g, lm, dt, False)
+            per.append(time.perf_counter() - t0)
+        runs[label] = {"threads": threads, "steps_per_s": 1.0 / float(np.median(per)),
+                       "step_s": [float(v) for v in per]}
+    return {"value": runs["all"]["steps_per_s"], "unit": "steps/s", "cores": phys, "kind": "port",
+            "physical_cores": phys, "half_cores": runs["half"],
             "sample": f"{steps} real-time SIL steps (m={lm}, dt={dt}) at N={npar}, M={norb} with the "
-                      f"reference's table-driven H.C; {threads} OpenMP threads; table build excluded"}
+                      f"reference's table-driven H.C, median step; {phys} OpenMP threads (all physical cores "
+                      f"of this process's CPUs), then {runs['half']['threads']} (half, the reference's "
+                      f"default); table build excluded"}
+
+
+def hc_loads_per_config(M: int, c64: bool) -> int:
+    """Vector-memory load instructions of one configuration's row in the
+    fused H.C kernel when every occupation is >= 2 (every branch taken, the
+    case of nearly every wave at N=100): the byte-packed occupation row, the
+    row's own coefficient, and one gather per distinct jump target
+    (M(M-1) single, M.C(M,2) same-orbital and C(M,2).C(M-1,2) two-orbital
+    double jumps).  In the complex64 form, targets that differ only in
+    whether an added particle lands in orbital 0 or 1 are adjacent and share
+    one 16-byte load when the removed orbitals are >= 2."""
+    singles = M * (M - 1)
+    same = M * (M - 1) * M // 2
+    diff = math.comb(M, 2) * (M - 2) * (M - 1) // 2
+    if c64:
+        singles -= M - 2
+        same -= (M - 2) * (1 + (M - 3))
+        diff -= math.comb(M - 2, 2) * (1 + max(M - 4, 0))
+    return singles + same + diff + 2
+
+
+# Texture-addresser cycles per 64-lane gather instruction and the unit
+# count: every vector-memory instruction of a wave occupies its CU's
+# texture addresser for ~16 cycles whatever its width (rocprofv3
+# TA_TA_BUSY / SQ_INSTS_VMEM_RD on the H.C kernel, 256 CUs at 2.4 GHz).
+TA_CYCLES_PER_LOAD = 16.0
+TA_UNITS, CLOCK_GHZ = 256, 2.4
+
+
+def measure_traffic(args) -> dict:
+    """HBM bytes per launch of the mixed H.C kernel, measured now: two
+    rocprofv3 counter runs of `bench.py --probe-hc` (FETCH_SIZE, then
+    WRITE_SIZE), each scaled by what the same counter reports for a device
+    copy of known size in the same run."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return {"bytes": None, "error": "rocprofv3 not found"}
+    out = {}
+    env = dict(os.environ, TMPDIR="/tmp")
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="hc_pmc_", dir="/tmp")
+        cmd = [prof, "--pmc", counter, "-f", "csv", "-d", d, "-o", "p", "--", sys.executable,
+               os.path.join(ROOT, "bench.py"), "--probe-hc", "--npar", str(args.npar), "--norb", str(args.norb),
+               "--seed", str(args.seed)]
+        try:
+            subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                           timeout=150, check=False)
+        except subprocess.TimeoutExpired:
+            return {"bytes": None, "error": f"{counter} run timed out"}
+        rows = []
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(f) as fh:
+                for row in csv.DictReader(fh):
+                    if row.get("Counter_Name") == counter:
+                        rows.append((int(row.get("Dispatch_Id", 0) or 0), row.get("Kernel_Name", ""),
+                                     float(row.get("Counter_Value", "nan"))))
+        shutil.rmtree(d, ignore_errors=True)
+        rows.sort()
+        # the probe's last dispatches: 5 H.C launches, then 5 device-to-device
+        # copies of COPY_PROBE_BYTES (the runtime's copyBuffer kernel)
+        hc = [v for _, name, v in rows if "k_hc_fused" in name][-5:]
+        cp = [v for _, name, v in rows if "copyBuffer" in name][-5:]
+        if len(hc) < 5 or len(cp) < 5:
+            return {"bytes": None, "error": f"{counter}: the probe's H.C or copy dispatches are missing"}
+        scale = COPY_PROBE_BYTES / float(np.mean(cp))  # bytes per counter unit
+        out[counter] = {"hc_counter_mean": float(np.mean(hc)), "copy_counter_mean": float(np.mean(cp)),
+                        "scale_bytes_per_unit": scale, "hc_bytes": float(np.mean(hc)) * scale}
+    total = out["FETCH_SIZE"]["hc_bytes"] + out["WRITE_SIZE"]["hc_bytes"]
+    return {"bytes": int(round(total)), "counters": out}
 
 
 COPY_PROBE_BYTES = 512 << 20
@@ -317,26 +430,30 @@ def serve_bench(args):
     wall = time.perf_counter() - t0
     steps = sum(r["steps"] for r in recs)
     lat = serve.latency_summary(recs)
-    log(f"served {len(recs)} requests in {wall:.2f} s: time to solution p50 {lat['p50_s']:.3f} s, "
-        f"p99 {lat['p99_s']:.3f} s, at most {srv.peak_active} at once")
+    log(f"served {len(recs)} requests in {wall:.2f} s ({lat['count']} solved): time to solution p50 "
+        f"{lat['p50_s']} s, p99 {lat['p99_s']} s, at most {srv.peak_active} at once")
     dims = np.array([r["dim"] for r in recs])
-    # interaction scale g (N - 1) of the requests whose energy stayed finite
-    # and of those whose did not (the split step's Runge-Kutta sub-step
-    # holds the interaction term, stable only for g (N - 1) |phi|^2 dt small)
     by_id = {q.rid: q for q in reqs}
-    gn = {True: [], False: []}
+    ok = [r for r in recs if r["status"] == "ok"]
+    failed = []
     for r in recs:
-        d = by_id[r["rid"]].draw
-        gn[bool(np.isfinite(r["energy_per_particle"]))].append(d.g * (d.npar - 1))
+        if r["status"] == "ok":
+            continue
+        q = by_id[r["rid"]]
+        d = q.draw
+        failed.append({"rid": r["rid"], "kind": r["kind"], "reason": r["reason"], "npar": d.npar, "norb": d.norb,
+                       "dim": d.dim, "ngrid": d.ngrid, "g": d.g, "g_factor": q.g_factor,
+                       "g_n": d.g * (d.npar - 1) * (q.g_factor if q.kind == "quench" else 1.0),
+                       "imag_steps": q.imag_steps, "real_steps": q.real_steps})
     return {"workload": f"{len(recs)} requests, Poisson {args.serve_rate}/s for {args.serve_window} s, "
                         f"step counts x{args.serve_time_scale}",
             "requests": len(recs), "quenches": sum(r["kind"] == "quench" for r in recs),
-            "wall_s": wall, "solves_per_s": len(recs) / wall, "instance_steps": steps,
-            "instance_steps_per_s": steps / wall, "time_to_solution": lat, "peak_admitted": srv.peak_active,
+            "wall_s": wall, "solves_per_s": len(ok) / wall, "solved": len(ok), "failed": len(failed),
+            "instance_steps": steps, "instance_steps_per_s": steps / wall,
+            "time_to_solution": lat, "peak_admitted": srv.peak_active,
             "budget_gb": args.serve_budget_gb, "max_active": args.serve_max_active or None,
             "dim_min_median_max": [int(dims.min()), int(np.median(dims)), int(dims.max())],
-            "finite_energies": int(sum(np.isfinite(r["energy_per_particle"]) for r in recs)),
-            "g_times_n_max_finite": max(gn[True], default=None), "g_times_n_min_not_finite": min(gn[False], default=None)}
+            "failed_requests": failed[:20]}
 
 
 def main():
@@ -351,6 +468,10 @@ def main():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--hc-reps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--latency-steps", type=int, default=200,
+                    help="further steps, each timed on its own, for the p50/p99 step latency")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter runs that measure the H.C kernel's HBM bytes")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-full-step", action="store_true",
                     help="skip the whole-step (orbitals + SIL + densities) eager/graph timing")
@@ -455,31 +576,42 @@ def main():
         dm_ms[name] = a.elapsed_time(b) / 5
         log(f"density {name}: {dm_ms[name]:.3f} ms")
 
-    # ---- SIL steps
-    def run_sil(prec, index):
+    # ---- SIL steps: K timed steps (the value), then `latency_steps` more,
+    # each bracketed by its own events, for the per-step p50/p99
+    def run_sil(prec, index, lat_steps):
         sil = amd.DeviceSil(space, lm, prec, index)
         C = C0.clone()
         for _ in range(args.warmup):
             sil.step(C, H, G, dt, False, stream=stream)
         torch.cuda.synchronize()
-        evs = [ev() for _ in range(args.steps + 1)]
-        evs[0].record(stream)
+        a, b = ev(), ev()
+        a.record(stream)
         for k in range(args.steps):
             sil.step(C, H, G, dt, False, stream=stream)
-            evs[k + 1].record(stream)
-        evs[-1].synchronize()
-        per = [evs[k].elapsed_time(evs[k + 1]) for k in range(args.steps)]
-        total_ms = evs[0].elapsed_time(evs[-1])
+        b.record(stream)
+        b.synchronize()
+        total_ms = a.elapsed_time(b)
+        per = []
+        if lat_steps:
+            evs = [ev() for _ in range(lat_steps + 1)]
+            evs[0].record(stream)
+            for k in range(lat_steps):
+                sil.step(C, H, G, dt, False, stream=stream)
+                evs[k + 1].record(stream)
+            evs[-1].synchronize()
+            per = [evs[k].elapsed_time(evs[k + 1]) for k in range(lat_steps)]
         done, _, _ = sil.scalars(stream=stream)
         sil.close()
         log(f"SIL prec={prec} index={index}: {total_ms / args.steps:.3f} ms/step")
-        return C, total_ms, per, done
+        return C, total_ms, per, done, args.warmup + args.steps + lat_steps
 
-    C_mixed, t_mixed, per_mixed, done_mixed = run_sil(amd.MIXED, amd.INDEX_FLY)
-    C_fly, t_fly, per_fly, _ = run_sil(amd.FP64, amd.INDEX_FLY)
-    C_tab, t_tab, per_tab, _ = run_sil(amd.FP64, amd.INDEX_TABLES)
+    C_mixed, t_mixed, per_mixed, done_mixed, n_mixed = run_sil(amd.MIXED, amd.INDEX_FLY, args.latency_steps)
+    C_fly, t_fly, per_fly, _, n_fly = run_sil(amd.FP64, amd.INDEX_FLY, args.latency_steps)
+    C_tab, t_tab, per_tab, _, _ = run_sil(amd.FP64, amd.INDEX_TABLES, 0)
+    if n_fly != n_mixed:
+        raise RuntimeError("the mixed and FP64 runs must end at the same time")
 
-    # ---- error of the mixed path against FP64 at the same time t = (W+K) dt
+    # ---- error of the mixed path against FP64 at the same time t = n dt
     def observables(C):
         hc = torch.empty_like(C)
         space.apply_hamiltonian(C, H, G, hc, amd.FP64, amd.INDEX_FLY, stream=stream)
@@ -493,14 +625,13 @@ def main():
 
     e64, rho64, occ64 = observables(C_fly)
     e32, rho32, occ32 = observables(C_mixed)
-    et, rhot, _ = observables(C_tab)
     errors = {
         "energy_rel": abs(e32 - e64) / abs(e64),
+        "natural_occupations_rel": float(np.max(np.abs(occ32 - occ64) / np.abs(occ64))),
         "natural_occupations_abs": float(np.max(np.abs(occ32 - occ64))),
         "rho1_rel_fro": float(np.linalg.norm(rho32 - rho64) / np.linalg.norm(rho64)),
-        "fp64_tables_vs_fly_energy_rel": abs(et - e64) / abs(e64),
         "coef_norm_drift_mixed": abs(torch.linalg.vector_norm(C_mixed).item() - 1.0),
-        "after_steps": args.warmup + args.steps,
+        "after_steps": n_mixed,
     }
 
     steps_mixed = args.steps / (t_mixed / 1e3)
@@ -509,24 +640,35 @@ def main():
     ab_c64 = algorithmic_bytes(N, M, 8, lm)
     ab_f64 = algorithmic_bytes(N, M, 16, lm)
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "hc_mixed_fly_traffic_N100_M5.md")
-    if (N, M) == (100, 5) and os.path.exists(tpath):
-        for ln in open(tpath):
-            if ln.startswith("bytes_per_launch:"):
-                traffic = int(ln.split(":")[1])
-
+    # ---- roofline of the dominant kernel (the mixed on-the-fly H.C).  Its
+    # algorithmic bytes are what this algorithm must move: coefficients in
+    # and out (8 B each) and the byte-packed occupation row (8 B); it reads
+    # no jump tables.  The table-driven figure is kept as `effective_gbs`.
+    t_hc = hc_ms["mixed_fly"] / 1e3
+    hc_bytes = 24 * nc
+    traffic = None if args.no_traffic else measure_traffic(args)
+    waves = -(-nc // 64)
+    loads = hc_loads_per_config(M, True)
+    floor_s = waves * loads * TA_CYCLES_PER_LOAD / (TA_UNITS * CLOCK_GHZ * 1e9)
     roofline = {
         "bound": "hbm",
-        "kernel": "k_hc_fused<5, float, fly> (mixed H.C, on-the-fly jump indices)",
-        "achieved": ab_c64["hc"] / (hc_ms["mixed_fly"] / 1e3) / 1e9,
+        "kernel": f"k_hc_fused<{M}, float, fly, c64, EpiPlain> (mixed H.C, on-the-fly jump indices)",
+        "achieved": hc_bytes / t_hc / 1e9,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
-        "frac": ab_c64["hc"] / (hc_ms["mixed_fly"] / 1e3) / 1e9 / HBM_PEAK_GBS,
-        "traffic": traffic,
-        "algorithmic_bytes_per_launch": ab_c64["hc"],
-        "note": "effective: algorithmic bytes of a table-driven H.C (jump-table entries read once); "
-                "this kernel computes jump targets from rank deltas and reads no tables",
+        "frac": hc_bytes / t_hc / 1e9 / HBM_PEAK_GBS,
+        "traffic": traffic["bytes"] if traffic else None,
+        "algorithmic_bytes_per_launch": hc_bytes,
+        "algorithmic_bytes": "coefficients in + out (8 B each) + packed occupation row (8 B) per configuration",
+        "launch_us": hc_ms["mixed_fly"] * 1e3,
+        "traffic_measurement": traffic,
+        "gather_issue_roof": {
+            "what": "vector-memory load instructions x texture-addresser cycles each / (CUs x clock)",
+            "loads_per_configuration": loads, "waves": waves, "ta_cycles_per_load": TA_CYCLES_PER_LOAD,
+            "floor_us": floor_s * 1e6, "frac": floor_s / t_hc,
+        },
+        "binding_roof": "gather issue (texture addresser), not HBM",
+        "effective_gbs_table_driven_bytes": ab_c64["hc"] / t_hc / 1e9,
     }
 
     full = full_step_bench(args, stream) if not args.no_full_step else None
@@ -557,14 +699,15 @@ def main():
         "data": "synthetic (seeded complex-normal C, random Hermitian h scale 1, symmetrised g scale 0.1)",
         "config": {"workload": f"real-time SIL steps N={N} M={M} NC={nc} m={lm} dt={dt}",
                    "npar": N, "norb": M, "dim": nc, "lanczos_iter": lm, "dt": dt},
-        "p50_ms": percentile(per_mixed, 50),
-        "p99_ms": percentile(per_mixed, 99),
+        "p50_ms": percentile(per_mixed, 50) if per_mixed else None,
+        "p99_ms": percentile(per_mixed, 99) if per_mixed else None,
+        "latency_samples": len(per_mixed),
         "fp64_fly_steps_per_s": steps_fly,
         "fp64_tables_steps_per_s": steps_tab,
-        "fp64_fly_p50_ms": percentile(per_fly, 50),
-        "fp64_tables_p50_ms": percentile(per_tab, 50),
-        "mixed_vs_fp64_tables": steps_mixed / steps_tab,
+        "fp64_fly_p50_ms": percentile(per_fly, 50) if per_fly else None,
+        "fp64_fly_p99_ms": percentile(per_fly, 99) if per_fly else None,
         "mixed_vs_fp64_fly": steps_mixed / steps_fly,
+        "mixed_fly_vs_fp64_tables": steps_mixed / steps_tab,
         "lanczos_iter_done": done_mixed,
         "errors_mixed_vs_fp64": errors,
         "hc_ms": hc_ms,

@@ -72,6 +72,8 @@ def lib():
         L.orc_set_config.argtypes = [C.c_uint32, C.c_uint16, C.c_uint16, _u16p]
         L.orc_hash_index.argtypes = [C.c_uint16, C.c_uint16, _u32p, _u16p]
         L.orc_hash_index.restype = C.c_uint32
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_get_threads.restype = C.c_int
         L.orc_space_create.argtypes = [C.c_uint16, C.c_uint16]
         L.orc_space_create.restype = C.POINTER(_Space)
         L.orc_space_destroy.argtypes = [C.POINTER(_Space)]
@@ -133,6 +135,15 @@ def lib():
         L.orc_fullstep_orb.argtypes = [oe, C.c_int, C.c_int, C.c_int, _cp, _cp, _cp]
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the oracle's later calls."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().orc_get_threads())
 
 
 def space_dimension(npar: int, norb: int) -> int:

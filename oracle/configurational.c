@@ -14,6 +14,18 @@
 #include <string.h>
 #include <omp.h>
 
+void
+orc_set_threads(int n)
+{
+    if (n > 0) omp_set_num_threads(n);
+}
+
+int
+orc_get_threads(void)
+{
+    return omp_get_max_threads();
+}
+
 /* Chunk counters that locate a double-jump block inside a configuration's
  * mapot / maptt segment (reference hamiltonian.c:92-97 and :236-248). */
 static inline uint32_t

@@ -50,6 +50,10 @@ void     orc_double_diffjump_map(uint16_t npar, uint16_t norb, const uint32_t* s
 void     orc_double_equaljump_map(uint16_t npar, uint16_t norb, const uint32_t* subsdim,
                                   const uint16_t* ht, uint32_t* strides, uint32_t* map);
 orc_space* orc_space_create(uint16_t npar, uint16_t norb);
+/* OpenMP threads of every later oracle call (omp_set_num_threads), and the
+ * current setting; used to time the baseline at a stated core count. */
+void orc_set_threads(int n);
+int orc_get_threads(void);
 void       orc_space_destroy(orc_space* S);
 
 /* ------------------------------------------------------ configurational */

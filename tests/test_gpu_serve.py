@@ -35,3 +35,4 @@ def test_served_results_equal_alone(amd):
             assert np.array_equal(r["state"][k], alone["state"][k]), (r["rid"], k)
         assert r["time_to_solution"] >= 0 and r["done"] >= r["admitted"] >= r["arrival"] - 1e-9
         assert np.isfinite(r["energy_per_particle"])
+        assert r["status"] == "ok" and r["reason"] == ""

@@ -1,7 +1,9 @@
 """Host logic of the request loop (mctdhb_amd.serve.Server) with stand-in
 instances, events and streams: admission only after arrival and within the
 budget and the cap, every step enqueued in order (imaginary then real),
-retirement only once the last step's event reports done."""
+retirement only once the last step's event reports done, every step of a
+request on the one stream it was given at admission, and a per-request
+status that keeps failed solves out of the latency figures."""
 import time
 
 from mctdhb_amd import batch, serve
@@ -24,6 +26,7 @@ class FakeEvent:
 
 class FakeInstance:
     log = []
+    fail_rid = None
 
     def __init__(self, req):
         self.req = req
@@ -37,6 +40,8 @@ class FakeInstance:
         self.steps.append(("real", stream))
 
     def energy_per_particle(self):
+        if self.req.rid == FakeInstance.fail_rid:
+            return float("nan")
         return float(len(self.steps))
 
     def close(self):
@@ -73,14 +78,37 @@ def test_loop_admits_within_budget_and_cap_and_runs_every_step():
     for q in reqs:
         kinds = [k for k, _ in instances[q.rid].steps]
         assert kinds == ["imag"] * q.imag_steps + ["real"] * q.real_steps
+        # one stream per request, whatever retirements do to the active list
+        assert len({id(s) for _, s in instances[q.rid].steps}) == 1
+    # requests were spread over both streams
+    assert len({id(instances[q.rid].steps[0][1]) for q in reqs}) == 2
     for r in recs:
-        assert r["energy_per_particle"] == r["steps"]
+        assert r["status"] == "ok" and r["energy_per_particle"] == r["steps"]
         assert r["admitted"] >= r["arrival"] - 1e-9 and r["done"] >= r["admitted"]
         assert r["time_to_solution"] == r["done"] - r["arrival"]
     # every admission happened after its arrival and each instance was closed once
     t_admit = {rid: t for ev, rid, t in FakeInstance.log if ev == "admit"}
     closes = [rid for ev, rid, _ in FakeInstance.log if ev == "close"]
     assert sorted(closes) == [0, 1, 2, 3, 4] and len(t_admit) == 5
+
+
+def test_failed_solves_are_marked_and_left_out_of_latency():
+    FakeInstance.log.clear()
+    FakeInstance.fail_rid = 3
+    try:
+        reqs = _requests()
+        srv = serve.Server(10 * max(q.footprint for q in reqs), nstreams=3, chunk=2, make_instance=FakeInstance,
+                           make_event=FakeEvent, make_stream=object, wait_all=lambda: None)
+        recs = srv.run(reqs)
+    finally:
+        FakeInstance.fail_rid = None
+    status = {r["rid"]: (r["status"], r["reason"]) for r in recs}
+    assert status[3] == ("failed", "energy not finite")
+    assert all(status[i] == ("ok", "") for i in (0, 1, 2, 4))
+    lat = serve.latency_summary(recs)
+    assert lat["count"] == 4
+    ok = [r["time_to_solution"] for r in recs if r["rid"] != 3]
+    assert lat["max_s"] == max(ok)
 
 
 def test_loop_rejects_a_request_above_the_whole_budget():
