@@ -73,6 +73,11 @@ extern "C" int mctdhb_space_create(uint16_t npar, uint16_t norb, int flags, mctd
     Space* S = nullptr;
     int rc = space_create(npar, norb, flags, &S);
     if (rc) return rc;
+    // the fixed scratch of the no-allocation H.C on this device
+    if ((rc = hc_scratch_setup())) {
+        space_destroy(S);
+        return rc;
+    }
     mctdhb_space* sp = new mctdhb_space();
     sp->S = S;
     *out = sp;
@@ -142,6 +147,16 @@ extern "C" int mctdhb_apply_hamiltonian_dev(const mctdhb_space* sp, const void* 
     }
     return hc_apply(sp->S, coef, (const double2*) hob, (const double2*) hint, hc, prec, index,
                     (hipStream_t) stream);
+}
+
+extern "C" int mctdhb_apply_hamiltonian_noalloc_dev(const mctdhb_space* sp, const void* coef, const void* hob,
+                                                    const void* hint, void* hc, int prec, int index, void* stream) {
+    if (!sp || !coef || !hob || !hint || !hc) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    return hc_apply_noalloc(sp->S, coef, (const double2*) hob, (const double2*) hint, hc, prec, index,
+                            (hipStream_t) stream);
 }
 
 extern "C" size_t mctdhb_hc_coef_bytes(const mctdhb_space* sp, int prec) {

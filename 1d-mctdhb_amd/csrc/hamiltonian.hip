@@ -10,6 +10,8 @@
 #include "hamiltonian_impl.hpp"
 #include "rowctx.hpp"
 
+#include <mutex>
+
 namespace mctdhb {
 
 constexpr int kHcBlockG = 256;
@@ -379,6 +381,95 @@ int hc_apply(const Space* S, const void* coef, const double2* hob, const double2
     hipFreeAsync(cf64, st);
     if (cf32) hipFreeAsync(cf32, st);
     return rc;
+}
+
+// ------------------------------------------- H.C without device allocation
+// The coefficient block of hc_apply_noalloc lives in this array.  Host code
+// never passes its name to a kernel or a copy: each device's copy of it is
+// at the address hipGetSymbolAddress returns with that device current, kept
+// in g_scratch_addr.
+static __device__ __align__(256) unsigned char scratch[kHcScratchBytes];
+constexpr int kScratchMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_addr[kScratchMaxDevices] = {};
+
+static int scratch_address(void** out) {
+    int dev = -1;
+    MCTDHB_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchMaxDevices) {
+        set_error("device index outside the fixed-scratch table");
+        return kDeviceError;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("hipGetSymbolAddress of the H.C scratch failed: ") + hipGetErrorString(e));
+            return kDeviceError;
+        }
+        g_scratch_addr[dev] = p;
+    }
+    *out = g_scratch_addr[dev];
+    return kOk;
+}
+
+int hc_scratch_setup() {
+    void* p = nullptr;
+    return scratch_address(&p);
+}
+
+static int dispatch_plain_tiled(const Space* S, const void* c, const void* cf, void* out, int tiling,
+                                hipStream_t st) {
+    switch (S->norb) {
+        case 2: return HcFused<2>::plain_tiled(S, c, cf, out, tiling, st);
+        case 3: return HcFused<3>::plain_tiled(S, c, cf, out, tiling, st);
+        case 4: return HcFused<4>::plain_tiled(S, c, cf, out, tiling, st);
+        case 5: return HcFused<5>::plain_tiled(S, c, cf, out, tiling, st);
+        case 6: return HcFused<6>::plain_tiled(S, c, cf, out, tiling, st);
+        case 7: return HcFused<7>::plain_tiled(S, c, cf, out, tiling, st);
+        default: set_error("fused H.C kernels cover 2 <= norb <= 7"); return kInvalidArgument;
+    }
+}
+
+int hc_apply_noalloc(const Space* S, const void* coef, const double2* hob, const double2* hint, void* hc, int prec,
+                     int index, hipStream_t st) {
+    int rc = check_mode(S, prec, index);
+    if (rc) return rc;
+    int dev = -1;
+    MCTDHB_HIP_CHECK(hipGetDevice(&dev));
+    if (dev != S->device) {
+        set_error("the space lives on another device: make its device current first");
+        return kInvalidArgument;
+    }
+    if (use_generic(S)) {  // reads hob and hint directly: no block
+        EpiPlainG<double2> epi{(double2*) hc};
+        k_hc_generic<<<hc_grid_blocks(S->dim), kHcBlockG, 0, st>>>(S->norb, row_ctx_g(S), (const double2*) coef,
+                                                                   hob, hint, 0, nullptr, epi);
+        MCTDHB_HIP_CHECK(hipGetLastError());
+        return kOk;
+    }
+    const size_t need = hc_coef_bytes(S, prec);
+    if (need > kHcScratchBytes) {
+        set_error("coefficient block of " + std::to_string(need) + " bytes exceeds the fixed scratch of " +
+                  std::to_string(kHcScratchBytes) + " bytes");
+        return kInvalidArgument;
+    }
+    void* blk = nullptr;
+    if ((rc = scratch_address(&blk))) return rc;
+    // every entry of the block is written here, so nothing needs clearing
+    if ((rc = hc_build_coef(S, hob, hint, blk, prec, st))) return rc;
+    if (prec == kPrecFP64) return dispatch_fused_plain(S, coef, blk, hc, prec, index, st);
+    const void* cf32 = (const double2*) blk + cf_size(S->norb);
+    if (S->hc_tiling == 1 && S->d_pad) {  // staging already allocated: a device copy only
+        const float2* cp = hc_padded_copy(S, (const float2*) coef, st);
+        if (!cp) {
+            set_error("padded staging of the coefficient vector failed");
+            return kDeviceError;
+        }
+        return dispatch_plain_tiled(S, cp, cf32, hc, 1, st);
+    }
+    return dispatch_plain_tiled(S, coef, cf32, hc, 0, st);
 }
 
 size_t hc_coef_bytes(const Space* S, int prec) {

@@ -68,6 +68,17 @@ int hc_coefficients(const Space* S, const double2* hob, const double2* hint, dou
 int hc_apply(const Space* S, const void* coef, const double2* hob, const double2* hint, void* hc, int prec,
              int index, hipStream_t st);
 
+// hc_apply with no device allocation: the coefficient block goes to a fixed
+// file-scope device array of kHcScratchBytes (one per GPU, shared by every
+// call on that GPU, so calls that overlap on one GPU race).  The mixed form
+// runs one configuration per lane unless the paired tiling is already chosen
+// and its padded staging exists.  hc_scratch_setup resolves the array's
+// address on the current device (once per device; idempotent).
+constexpr size_t kHcScratchBytes = 128 * 1024;
+int hc_scratch_setup();
+int hc_apply_noalloc(const Space* S, const void* coef, const double2* hob, const double2* hint, void* hc, int prec,
+                     int index, hipStream_t st);
+
 // Coefficient-block entry points: bytes needed, build, and H.C with a
 // prebuilt block (the generic kernel's block is [hob | hint] in FP64).
 size_t hc_coef_bytes(const Space* S, int prec);

@@ -87,6 +87,7 @@ def lib():
                                     C.POINTER(C.c_uint64), C.POINTER(_i)]
     L.mctdhb_space_tables.argtypes = [_vp] + [_cpp] * 7
     L.mctdhb_apply_hamiltonian_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]
+    L.mctdhb_apply_hamiltonian_noalloc_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, _i, _i, _vp]
     L.mctdhb_hc_coef_bytes.argtypes = [_vp, _i]
     L.mctdhb_hc_coef_bytes.restype = C.c_size_t
     L.mctdhb_hc_coefficients_dev.argtypes = [_vp, _vp, _vp, _vp, _i, _vp]
@@ -204,6 +205,12 @@ class DeviceSpace:
     def apply_hamiltonian(self, coef, hob, hint, out, prec=FP64, index=INDEX_FLY, stream=None):
         check(lib().mctdhb_apply_hamiltonian_dev(self._h, _ptr(coef), _ptr(hob), _ptr(hint), _ptr(out),
                                                  prec, index, _stream(stream)))
+
+    def apply_hamiltonian_noalloc(self, coef, hob, hint, out, prec=FP64, index=INDEX_FLY, stream=None):
+        """apply_hamiltonian with no device allocation (the library's fixed
+        per-GPU scratch; calls that overlap on one GPU race)."""
+        check(lib().mctdhb_apply_hamiltonian_noalloc_dev(self._h, _ptr(coef), _ptr(hob), _ptr(hint), _ptr(out),
+                                                         prec, index, _stream(stream)))
 
     def coef_bytes(self, prec=FP64) -> int:
         return int(lib().mctdhb_hc_coef_bytes(self._h, prec))

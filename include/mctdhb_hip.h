@@ -262,6 +262,20 @@ int mctdhb_space_tables(const mctdhb_space* space, const uint16_t** hash_table,
 int mctdhb_apply_hamiltonian_dev(const mctdhb_space* space, const void* coef, const void* hob,
                                  const void* hint, void* hc, int prec, int index, void* stream);
 
+/* mctdhb_apply_hamiltonian_dev that allocates no device memory.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The (hob, hint)-derived coefficient block goes to a fixed
+ * 128 KiB device array of the library, one per GPU (enough for every
+ * norb <= 7; norb 1 and > 7 read hob and hint directly); the space's device
+ * must be current.  Every call on one GPU shares that array, so calls that
+ * overlap on one GPU (different streams, or threads) race; serialise them.
+ * The mixed form runs one configuration per lane unless the paired tiling
+ * was already chosen on this space by an allocating call.  Same arguments,
+ * results and errors as mctdhb_apply_hamiltonian_dev. */
+int mctdhb_apply_hamiltonian_noalloc_dev(const mctdhb_space* space, const void* coef, const void* hob,
+                                         const void* hint, void* hc, int prec, int index, void* stream);
+
 /* The (hob, hint)-derived coefficient block of the H.C kernels, built once
  * and reused across applications with the same one- and two-body matrices.
  * cf must hold mctdhb_hc_coef_bytes(space, prec) bytes of device memory. */
