@@ -196,7 +196,7 @@ extern "C" int mctdhb_hc_tiling(mctdhb_space* sp, int set, int* tiling) {
     }
     if (set >= 0) sp->S->hc_tiling = set;
     if (set == -1) {
-        sp->S->hc_tiling = -1;
+        hc_forget_tiling(sp->S);
         const int rc = hc_tune_tiling(sp->S, nullptr);
         if (rc) return rc;
     }
@@ -641,6 +641,23 @@ extern "C" int mctdhb_step_graph_launch(mctdhb_step_graph* g, void* stream) {
         return kInvalidArgument;
     }
     MCTDHB_HIP_CHECK(hipGraphLaunch(g->exec, (hipStream_t) stream));
+    return kOk;
+}
+
+extern "C" int mctdhb_step_graph_launch_many(mctdhb_step_graph* const* graphs, int n, void* const* streams,
+                                             int nstreams, int rounds) {
+    if (n < 0 || rounds < 0 || (n > 0 && (!graphs || !streams || nstreams < 1))) {
+        set_error("invalid argument");
+        return kInvalidArgument;
+    }
+    for (int i = 0; i < n; i++)
+        if (!graphs[i]) {
+            set_error("null graph");
+            return kInvalidArgument;
+        }
+    for (int r = 0; r < rounds; r++)
+        for (int i = 0; i < n; i++)
+            MCTDHB_HIP_CHECK(hipGraphLaunch(graphs[i]->exec, (hipStream_t) streams[i % nstreams]));
     return kOk;
 }
 

@@ -25,6 +25,7 @@
 // The mixed form reads complex64 coefficients, multiplies in FP32 and sums
 // each thread's configurations with Kahan compensation; the block and grid
 // stages run in FP64.
+#include <algorithm>
 #include <vector>
 
 #include "density_impl.hpp"
@@ -267,22 +268,57 @@ __global__ __launch_bounds__(256) void k_dm_finish(int M, const double2* __restr
     }
 }
 
-static int dispatch_dm(const Space* S, const void* c, int prec, int index, double2* part, unsigned nb,
+// instantiated once each in dm_m<M>.hip
+extern template struct DmFused<2>;
+extern template struct DmFused<3>;
+extern template struct DmFused<4>;
+extern template struct DmFused<5>;
+extern template struct DmFused<6>;
+extern template struct DmFused<7>;
+
+static int dispatch_dm(const Space* S, const void* c, int prec, int index, double2* part, const DmPlan& plan,
                        hipStream_t st) {
     switch (S->norb) {
-        case 2: return DmFused<2>::run(S, c, prec, index, part, nb, st);
-        case 3: return DmFused<3>::run(S, c, prec, index, part, nb, st);
-        case 4: return DmFused<4>::run(S, c, prec, index, part, nb, st);
-        case 5: return DmFused<5>::run(S, c, prec, index, part, nb, st);
-        case 6: return DmFused<6>::run(S, c, prec, index, part, nb, st);
-        case 7: return DmFused<7>::run(S, c, prec, index, part, nb, st);
+        case 2: return DmFused<2>::run(S, c, prec, index, part, plan, st);
+        case 3: return DmFused<3>::run(S, c, prec, index, part, plan, st);
+        case 4: return DmFused<4>::run(S, c, prec, index, part, plan, st);
+        case 5: return DmFused<5>::run(S, c, prec, index, part, plan, st);
+        case 6: return DmFused<6>::run(S, c, prec, index, part, plan, st);
+        case 7: return DmFused<7>::run(S, c, prec, index, part, plan, st);
         default: set_error("fused density kernels cover 2 <= norb <= 7"); return kInvalidArgument;
     }
 }
 
+static bool wave_fits(const Space* S, int prec, int index) {
+    switch (S->norb) {
+        case 2: return DmFused<2>::wave_fits(S, prec, index);
+        case 3: return DmFused<3>::wave_fits(S, prec, index);
+        case 4: return DmFused<4>::wave_fits(S, prec, index);
+        case 5: return DmFused<5>::wave_fits(S, prec, index);
+        case 6: return DmFused<6>::wave_fits(S, prec, index);
+        case 7: return DmFused<7>::wave_fits(S, prec, index);
+        default: return false;
+    }
+}
+
+DmPlan dm_plan(const Space* S, int prec, int index) {
+    const int M = S->norb;
+    const bool generic = M == 1 || M > 7;
+    if (!generic && S->dim >= kDmWaveMinDim && wave_fits(S, prec, index)) {
+        // at most about 4096 blocks: enough waves to hide the gathers'
+        // latency, and a bounded set of partial sums
+        const int cpt = (int) std::min<uint32_t>(16, div_up(S->dim, (uint32_t) kDmBlock * 4096u));
+        return {true, cpt, (unsigned) div_up(S->dim, (uint32_t) cpt * kDmBlock)};
+    }
+    return {false, kDmCPT, (unsigned) div_up(S->dim, kDmTile)};
+}
+
 size_t dm_workspace_bytes(const Space* S) {
     const size_t E = (size_t) DmLayout{S->norb}.total();
-    const size_t nb = div_up(S->dim, kDmTile);
+    // the largest block count of any form
+    size_t nb = div_up(S->dim, kDmTile);
+    for (int prec = 0; prec < 2; prec++)
+        for (int index = 0; index < 2; index++) nb = std::max<size_t>(nb, dm_plan(S, prec, index).nb);
     return (nb * E + E) * sizeof(double2) + sizeof(ElemDesc) * E + 256;
 }
 
@@ -303,7 +339,8 @@ int density_matrices(const Space* S, const void* coef, double2* rho1, double2* r
         return kNotBuilt;
     }
     const int E = DmLayout{M}.total();
-    const unsigned nb = div_up(S->dim, kDmTile);
+    const DmPlan plan = dm_plan(S, prec, index);
+    const unsigned nb = plan.nb;
     double2* part = (double2*) work;
     double2* tot = part + (size_t) nb * E;
     ElemDesc* elems = (ElemDesc*) (tot + E);
@@ -318,7 +355,7 @@ int density_matrices(const Space* S, const void* coef, double2* rho1, double2* r
         // the host vector must outlive the async copy
         MCTDHB_HIP_CHECK(hipStreamSynchronize(st));
     } else {
-        rc = dispatch_dm(S, coef, prec, index, part, nb, st);
+        rc = dispatch_dm(S, coef, prec, index, part, plan, st);
     }
     if (rc) return rc;
     const DmSync sy = sync ? *sync : DmSync{nullptr, nullptr, 0.0};

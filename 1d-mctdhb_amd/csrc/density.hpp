@@ -15,9 +15,27 @@ int dm_num_elements(int M);
 // rho2 (M^4, index k + sM + qM^2 + lM^3 = <a+_k a+_s a_q a_l>), complex128.
 // coef is complex128 (prec = kDmFP64) or complex64 (kDmMixed); index is
 // kIdxTables or kIdxFly.  work must hold dm_workspace_bytes(S).
-// Per-orbital-count fused kernel (dm_m<M>.hip): writes per-block partials.
+// Launch plan of the fused kernels: the one-pass wave form (tiles of
+// cpt x 256 configurations) for large spaces of M >= 6, else the
+// element-family passes (tiles of 16 x 256, one family per block); nb blocks
+// of partial sums either way.  Measured on MI355X: the wave form takes
+// 1.87 ms at N=30, M=7 against 3.7 for the passes, and the passes 0.66 ms at
+// N=100, M=5 against 1.19 (at M <= 5 the wave sums cost more than they save).
+struct DmPlan {
+    bool wave;
+    int cpt;
+    unsigned nb;
+};
+DmPlan dm_plan(const Space* S, int prec, int index);
+// Spaces from this many configurations and orbitals take the wave form.
+constexpr uint32_t kDmWaveMinDim = 1u << 16;
+constexpr int kDmWaveMinOrb = 6;
+
+// Per-orbital-count fused kernels (dm_m<M>.hip): write per-block partials.
 template <int M> struct DmFused {
-    static int run(const Space* S, const void* c, int prec, int index, double2* part, unsigned nb, hipStream_t st);
+    static bool wave_fits(const Space* S, int prec, int index);
+    static int run(const Space* S, const void* c, int prec, int index, double2* part, const DmPlan& plan,
+                   hipStream_t st);
 };
 
 // With sync->rinv set, also the synchronisation of rho1 (eigenvalues into

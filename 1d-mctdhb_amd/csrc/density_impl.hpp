@@ -272,15 +272,228 @@ __global__ __launch_bounds__(kDmBlock) void k_density(RowCtx ctx, const cplx<R>*
                            use_sq ? sql : nullptr, red, part);
 }
 
+// ------------------------------------------------ one-pass wave form
+// (M >= 6, where the element families are many: 17 at M = 6, 38 at M = 7.)
+// Every configuration is visited once: its 2M + C(M,2) diagonal moments and
+// its jump-target products are formed once each, and each element's values
+// are summed over the wave's 64 configurations right away (DPP row shifts
+// inside each 16-lane row, the four row totals through readlane, all in
+// FP64) into a per-wave accumulator in LDS.  Nothing per configuration is
+// recomputed per element family, and no per-thread accumulator array is
+// needed.  The mixed form forms the products in complex64 and sums in FP64.
+
+// lane i gets v[i - K] within its row of 16 (0 where that leaves the row)
+template <int K> __device__ __forceinline__ double row_shr(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x110 + K, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x110 + K, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+// Sum of v over the 64 lanes of the wave, returned in every lane; every lane
+// of the wave must be active.  The order of the additions is fixed.
+__device__ __forceinline__ double wave_total(double v) {
+    v += row_shr<1>(v);
+    v += row_shr<2>(v);
+    v += row_shr<4>(v);
+    v += row_shr<8>(v);  // lane 15 of each row: the row's sum
+    return (readlane_d(v, 15) + readlane_d(v, 31)) + (readlane_d(v, 47) + readlane_d(v, 63));
+}
+
+// Per-wave accumulators, spread over the lanes: element e lives in lane
+// e % 64 of acc[e / 64], so adding a wave total costs a masked add and no
+// memory round trip.
+template <int E> struct WaveAcc {
+    static constexpr int NR = (E + 63) / 64;
+    double2 acc[NR];
+    int lane;
+    __device__ __forceinline__ void zero(int l) {
+        lane = l;
+#pragma unroll
+        for (int j = 0; j < NR; j++) acc[j] = make_double2(0, 0);
+    }
+    __device__ __forceinline__ void add(int e, double2 v) {
+        const double x = wave_total(v.x), y = wave_total(v.y);
+        if (lane == (e & 63)) acc[e >> 6] = acc[e >> 6] + make_double2(x, y);
+    }
+    __device__ __forceinline__ void add_re(int e, double v) {
+        const double x = wave_total(v);
+        if (lane == (e & 63)) acc[e >> 6].x += x;
+    }
+};
+
+template <int M, typename R, int IDX, typename WA>
+__device__ __forceinline__ void wave_config(const RowCtx& ctx, const cplx<R>* __restrict__ c, uint32_t i,
+                                            const uint32_t* Wl, int ldw, const R* sql, WA& wa) {
+    using Z = cplx<R>;
+    constexpr DmLayout L{M};
+    const bool valid = i < ctx.dim;
+    Cfg<M, R, IDX> x;
+    Z ci = mk<R>(0, 0);
+    if (valid) {
+        x.load(i, ctx, Wl, ldw, sql);
+        ci = c[i];
+    } else {
+#pragma unroll
+        for (int k = 0; k < M; k++) {  // an empty row: every term is zero
+            x.n[k] = 0;
+            x.sm[k] = x.s0[k] = x.s1[k] = x.s2[k] = R(0);
+        }
+    }
+    const int* n = x.n;
+    // diagonal moments (real)
+    {
+        const double w = (double) ci.x * ci.x + (double) ci.y * ci.y;
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+            wa.add_re(L.d1() + k, w * n[k]);
+            wa.add_re(L.d2() + k, w * (double) (n[k] * (n[k] - 1)));
+#pragma unroll
+            for (int s = k + 1; s < M; s++) wa.add_re(L.d3() + L.pidx(k, s), w * (double) (n[k] * n[s]));
+        }
+    }
+    // one particle k -> l (k < l): the pair element and its n_m moments
+#pragma unroll
+    for (int k = 0; k < M - 1; k++) {
+#pragma unroll
+        for (int l = k + 1; l < M; l++) {
+            double2 t = make_double2(0, 0);
+            if (n[k] >= 1) {
+                const Z cj = c[x.single(k, l)];
+                t = to_d2(cscale(cconjmul(ci, cj), x.s0[k] * x.s1[l]));
+            }
+            const int o = L.pair() + L.pidx(k, l) * (M + 1);
+            wa.add(o, t);
+#pragma unroll
+            for (int m = 0; m < M; m++) wa.add(o + 1 + m, cscale(t, (double) n[m]));
+        }
+    }
+    // two particles out of orbital k
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        const bool ok = n[k] >= 2;
+        const R a = x.sm[k] * x.s0[k];
+#pragma unroll
+        for (int q = k + 1; q < M; q++) {
+            double2 t = make_double2(0, 0);
+            if (ok) t = to_d2(cscale(cconjmul(ci, c[x.same(k, q, q)]), a * x.s1[q] * x.s2[q]));
+            wa.add(L.o3() + L.pidx(k, q), t);
+        }
+#pragma unroll
+        for (int q = 0; q < M; q++) {
+            if (q == k) continue;
+#pragma unroll
+            for (int l = q + 1; l < M; l++) {
+                if (l == k) continue;
+                double2 t = make_double2(0, 0);
+                if (ok) t = to_d2(cscale(cconjmul(ci, c[x.same(k, q, l)]), a * x.s1[q] * x.s1[l]));
+                wa.add(L.o6() + k * c2(M - 1) + L.pidx_excl1(k, q, l), t);
+            }
+        }
+    }
+    // one particle out of each of k < s
+    if constexpr (M >= 4) {
+#pragma unroll
+        for (int k = 0; k < M - 1; k++) {
+#pragma unroll
+            for (int s = k + 1; s < M; s++) {
+                const bool ok = n[k] >= 1 && n[s] >= 1;
+                const R a = x.s0[k] * x.s0[s];
+#pragma unroll
+                for (int q = 0; q < M; q++) {
+                    if (q == k || q == s) continue;
+#pragma unroll
+                    for (int l = q + 1; l < M; l++) {
+                        if (l == k || l == s) continue;
+                        double2 t = make_double2(0, 0);
+                        if (ok) t = to_d2(cscale(cconjmul(ci, c[x.diff(k, s, q, l)]), a * x.s1[q] * x.s1[l]));
+                        wa.add(L.t8() + L.pidx(k, s) * c2(M - 2) + L.pidx_excl2(k, s, q, l), t);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// One block of 256 threads takes tile b of `cpt` x 256 consecutive
+// configurations; its four per-wave accumulator sets are added in a fixed
+// order into part[b * E ...].
+template <int M, typename R, int IDX>
+__global__ __launch_bounds__(kDmBlock) void k_density_wave(RowCtx ctx, const cplx<R>* __restrict__ c,
+                                                           const uint32_t* __restrict__ Wg, int ldw, int cpt,
+                                                           double2* __restrict__ part) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int E = DmLayout{M}.total();
+    constexpr int NW = kDmBlock / 64;
+    double2* acc = (double2*) smem;
+    uint32_t* Wl = (uint32_t*) (smem + sizeof(double2) * NW * E);
+    const int wsz = (M - 1) * ldw;
+    R* sql = (R*) ((unsigned char*) Wl + ((wsz * 4 + 15) & ~15));
+    if constexpr (IDX == kIdxFly) {
+        for (int t = threadIdx.x; t < wsz; t += blockDim.x) Wl[t] = Wg[t];
+    }
+    for (int t = threadIdx.x; t < ctx.npar + 3; t += blockDim.x) sql[t] = sqrt((R) t);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    WaveAcc<E> wa;
+    wa.zero(lane);
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t base = b * (uint32_t) cpt * kDmBlock + threadIdx.x;
+    for (int r = 0; r < cpt; r++) wave_config<M, R, IDX>(ctx, c, base + r * kDmBlock, Wl, ldw, sql, wa);
+#pragma unroll
+    for (int j = 0; j < WaveAcc<E>::NR; j++)
+        if (j * 64 + lane < E) acc[wid * E + j * 64 + lane] = wa.acc[j];
+    __syncthreads();
+    for (int e = threadIdx.x; e < E; e += blockDim.x) {
+        double2 v = acc[e];
+#pragma unroll
+        for (int w = 1; w < NW; w++) v = v + acc[w * E + e];
+        part[(size_t) b * E + e] = v;
+    }
+}
+
 template <int M> static constexpr size_t dm_red_bytes() {
     return (size_t) 2 * (2 * M + c2(M) + 8 * 9 + 64) * (kDmBlock / 64) * sizeof(double);
 }
 
+template <int M> static size_t dm_wave_smem(const Space* S, size_t rsz) {
+    return sizeof(double2) * (kDmBlock / 64) * DmLayout{M}.total() +
+           (((size_t) (M - 1) * S->ldw * 4 + 15) & ~(size_t) 15) + rsz * (S->npar + 3);
+}
+
 template <int M>
-int DmFused<M>::run(const Space* S, const void* c, int prec, int index, double2* part, unsigned nb, hipStream_t st) {
-    const bool fly = prec == kDmMixed || index == kIdxFly;
-    const bool use_sq = S->npar + 3 <= 2048;
+bool DmFused<M>::wave_fits(const Space* S, int prec, int index) {
     const size_t rsz = prec == kDmMixed ? sizeof(float) : sizeof(double);
+    return M >= kDmWaveMinOrb && dm_wave_smem<M>(S, rsz) <= 64 * 1024;
+}
+
+template <int M>
+int DmFused<M>::run(const Space* S, const void* c, int prec, int index, double2* part, const DmPlan& plan,
+                    hipStream_t st) {
+    const bool fly = prec == kDmMixed || index == kIdxFly;
+    const size_t rsz = prec == kDmMixed ? sizeof(float) : sizeof(double);
+    RowCtx ctx = make_row_ctx(S->dim, S->npar, S->d_ht, S->d_map, S->d_mapot, S->d_maptt, S->d_strideot,
+                              S->d_stridett, S->d_occ8);
+    if constexpr (M >= kDmWaveMinOrb) {
+    if (plan.wave) {
+        const size_t smem = dm_wave_smem<M>(S, rsz);
+        if (prec == kDmMixed)
+            k_density_wave<M, float, kIdxFly><<<plan.nb, kDmBlock, smem, st>>>(ctx, (const float2*) c, S->d_W, S->ldw,
+                                                                              plan.cpt, part);
+        else if (fly)
+            k_density_wave<M, double, kIdxFly><<<plan.nb, kDmBlock, smem, st>>>(ctx, (const double2*) c, S->d_W,
+                                                                               S->ldw, plan.cpt, part);
+        else
+            k_density_wave<M, double, kIdxTables><<<plan.nb, kDmBlock, smem, st>>>(ctx, (const double2*) c, S->d_W,
+                                                                                  S->ldw, plan.cpt, part);
+        MCTDHB_HIP_CHECK(hipGetLastError());
+        return kOk;
+    }
+    }
+    const unsigned nb = plan.nb;
+    const bool use_sq = S->npar + 3 <= 2048;
     size_t smem = dm_red_bytes<M>();
     smem += ((size_t) (M - 1) * S->ldw * 4 + 15) & ~(size_t) 15;  // W region, reserved in both modes
     if (use_sq) smem += rsz * (S->npar + 3);
@@ -288,10 +501,10 @@ int DmFused<M>::run(const Space* S, const void* c, int prec, int index, double2*
         set_error("density kernel LDS budget exceeded for this (N, M); use table indexing");
         return kInvalidArgument;
     }
-    RowCtx ctx = make_row_ctx(S->dim, S->npar, S->d_ht, S->d_map, S->d_mapot, S->d_maptt, S->d_strideot,
-                              S->d_stridett, S->d_occ8);
-    // Spaces too small to fill the chip with tiles run one pass per block.
-    const dim3 grid(nb, nb < 256 ? Passes<M>::count() : 1);
+    // One element family per block (blockIdx.y): every block's chain of
+    // dependent loads is one family long, and 14x more blocks (M = 5) are
+    // in flight than with every family in turn in one block.
+    const dim3 grid(nb, Passes<M>::count());
     if (prec == kDmMixed)
         k_density<M, float, kIdxFly><<<grid, kDmBlock, smem, st>>>(ctx, (const float2*) c, S->d_W, S->ldw, use_sq,
                                                                    part);

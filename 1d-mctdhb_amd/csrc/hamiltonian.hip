@@ -10,7 +10,9 @@
 #include "hamiltonian_impl.hpp"
 #include "rowctx.hpp"
 
+#include <map>
 #include <mutex>
+#include <utility>
 
 namespace mctdhb {
 
@@ -29,11 +31,32 @@ const float2* hc_padded_copy(const Space* S, const float2* c, hipStream_t st) {
     return p;
 }
 
+// The autotuner's choice per (N, M), shared by every space of the process:
+// timings vary from run to run, and two spaces of one shape must run the same
+// kernels so that a solve gives the same bits alone and among others.
+static std::mutex g_tiling_mu;
+static std::map<std::pair<int, int>, int> g_tiling_by_shape;
+
+void hc_forget_tiling(const Space* S) {
+    std::lock_guard<std::mutex> lock(g_tiling_mu);
+    g_tiling_by_shape.erase(std::make_pair((int) S->npar, (int) S->norb));
+    S->hc_tiling = -1;
+}
+
 int hc_tune_tiling(const Space* S, hipStream_t st) {
     if (S->hc_tiling >= 0) return kOk;
     if (!pair_possible(S)) {
         S->hc_tiling = 0;
         return kOk;
+    }
+    const std::pair<int, int> shape(S->npar, S->norb);
+    {
+        std::lock_guard<std::mutex> lock(g_tiling_mu);
+        auto it = g_tiling_by_shape.find(shape);
+        if (it != g_tiling_by_shape.end()) {
+            S->hc_tiling = it->second;
+            return kOk;
+        }
     }
     const int M = S->norb;
     float2 *cf = nullptr, *out = nullptr;
@@ -77,6 +100,8 @@ int hc_tune_tiling(const Space* S, hipStream_t st) {
     MCTDHB_HIP_CHECK(hipStreamSynchronize(st));
     if (rc) return rc;
     S->hc_tiling = best[1] < best[0] ? 1 : 0;
+    std::lock_guard<std::mutex> lock(g_tiling_mu);
+    g_tiling_by_shape[shape] = S->hc_tiling;
     return kOk;
 }
 

@@ -108,10 +108,13 @@ unsigned hc_grid_blocks(uint32_t dim);
 // configurations of one run per lane (the pair work list of the space).
 inline bool pair_possible(const Space* S) { return S->d_slots && S->dim >= 2 && S->norb >= 2 && S->norb <= 7; }
 inline bool use_pair(const Space* S, int prec) { return prec == kPrecMixed && pair_possible(S) && S->hc_tiling == 1; }
-// Times both mixed tilings once on this space (zero data: the kernels'
-// cost does not depend on the values) and records the faster in
-// S->hc_tiling.  Synchronises the stream; no-op once done.
+// Times both mixed tilings once per (N, M) (zero data: the kernels' cost
+// does not depend on the values) and records the faster in S->hc_tiling.
+// Synchronises the stream; no-op once done.
 int hc_tune_tiling(const Space* S, hipStream_t st);
+// The choice is kept per (N, M) for the process (so every space of a shape
+// runs the same kernels); this drops it, and the space's, for a new timing.
+void hc_forget_tiling(const Space* S);
 // Blocks of the H.C launch of the Lanczos form (one partial-sum row each).
 unsigned hc_lanczos_blocks(const Space* S, int prec);
 // Whether the last Lanczos iteration runs the half-row Rayleigh form (the

@@ -126,6 +126,7 @@ def lib():
     L.mctdhb_propagate_step_dev.argtypes = [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
     L.mctdhb_step_graph_create.argtypes = [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _cpp]
     L.mctdhb_step_graph_launch.argtypes = [_vp, _vp]
+    L.mctdhb_step_graph_launch_many.argtypes = [_cpp, _i, _cpp, _i, _i]
     L.mctdhb_step_graph_destroy.argtypes = [_vp]
     # reference-named drop-in layer
     L.space_dimension.argtypes = [C.c_uint16, C.c_uint16]
@@ -393,6 +394,17 @@ class StepGraph:
             self._h = None
 
     __del__ = close
+
+
+def launch_graphs(graphs, streams, rounds: int = 1):
+    """`rounds` replays of every StepGraph in `graphs`, graph i on
+    streams[i % len(streams)], in one library call."""
+    n = len(graphs)
+    if n == 0 or rounds <= 0:
+        return
+    ga = (_vp * n)(*[g._h for g in graphs])
+    sa = (_vp * len(streams))(*[_stream(s) for s in streams])
+    check(lib().mctdhb_step_graph_launch_many(ga, n, sa, len(streams), int(rounds)))
 
 
 class Multiconf:

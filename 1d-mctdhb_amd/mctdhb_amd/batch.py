@@ -184,10 +184,11 @@ class Batch:
         self.streams = [torch.cuda.Stream() for _ in range(max(1, nstreams))]
 
     def run(self, steps: int):
-        S = self.streams
-        for _ in range(steps):
-            for i, inst in enumerate(self.instances):
-                inst.step(S[i % len(S)])
+        """steps rounds; in each, instance i's step graph on stream i mod
+        len(streams) (one library call for all the launches)."""
+        import mctdhb_amd as amd
+
+        amd.launch_graphs([inst.graph for inst in self.instances], self.streams, steps)
 
     def synchronize(self):
         for s in self.streams:

@@ -27,6 +27,11 @@ import time
 
 import numpy as np
 
+# More hardware queues than HIP's default 4 let the batched and served
+# sections' independent solves (one stream each) run side by side; HIP reads
+# this when it initialises, so it is set before torch is imported.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "1d-mctdhb_amd"))
 sys.path.insert(0, ROOT)
@@ -216,6 +221,27 @@ def probe_hc(args, space, C0, H, G, stream):
     torch.cuda.synchronize()
     print(json.dumps({"probe": "hc_mixed_fly", "hc_launches": 5, "copy_launches": 5,
                       "copy_bytes_read": COPY_PROBE_BYTES, "copy_bytes_written": COPY_PROBE_BYTES}), flush=True)
+
+
+def probe_sil(args, space, C0, H, G, stream):
+    """Two mixed SIL steps (their H.C launches carry the Lanczos epilogue),
+    then five device copies of COPY_PROBE_BYTES, for counter runs of the
+    kernels that dominate the timed step."""
+    import torch
+    import mctdhb_amd as amd
+
+    sil = amd.DeviceSil(space, args.lanczos, amd.MIXED, amd.INDEX_FLY)
+    C = C0.clone()
+    for _ in range(2):
+        sil.step(C, H, G, args.dt, False, stream=stream)
+    a = torch.empty(COPY_PROBE_BYTES // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    sil.close()
+    print(json.dumps({"probe": "sil_mixed", "steps": 2, "copy_launches": 5, "copy_bytes": COPY_PROBE_BYTES}),
+          flush=True)
 
 
 def full_step_bench(args, stream):
@@ -423,7 +449,7 @@ def serve_bench(args):
 
     reqs = serve.draw_requests(args.serve_rate, args.serve_window, seed=args.seed,
                                time_scale=args.serve_time_scale)
-    srv = serve.Server(int(args.serve_budget_gb * 1e9), nstreams=4, chunk=8,
+    srv = serve.Server(int(args.serve_budget_gb * 1e9), nstreams=8, chunk=8,
                        max_active=args.serve_max_active or None)
     t0 = time.perf_counter()
     recs = srv.run(reqs)
@@ -479,7 +505,7 @@ def main():
                     help="skip the mixed-vs-FP64 comparison on (10,4), (200,4), (30,7), (100,5) with m=8")
     ap.add_argument("--batch-instances", type=int, default=128)
     ap.add_argument("--batch-steps", type=int, default=50)
-    ap.add_argument("--batch-streams", type=int, default=4)
+    ap.add_argument("--batch-streams", type=int, default=8)
     ap.add_argument("--no-batched", action="store_true", help="skip the many-instance timing")
     ap.add_argument("--batched-only", action="store_true", help="only the many-instance timing (its JSON object)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -491,6 +517,8 @@ def main():
     ap.add_argument("--serve-only", action="store_true", help="only the request loop (its JSON object)")
     ap.add_argument("--full-step-only", action="store_true",
                     help="only the whole-step timing (prints its JSON object)")
+    ap.add_argument("--probe-sil", action="store_true",
+                    help="only two mixed SIL steps and a plain copy kernel of known size, for hardware-counter runs")
     ap.add_argument("--probe-hc", action="store_true",
                     help="only the mixed H.C kernel (5 launches) and a plain copy kernel of known size, "
                          "for hardware-counter runs")
@@ -537,6 +565,9 @@ def main():
 
     if args.probe_hc:
         probe_hc(args, space, C0, H, G, stream)
+        return
+    if args.probe_sil:
+        probe_sil(args, space, C0, H, G, stream)
         return
 
     # ---- H.C kernel alone (coefficient block prebuilt), three forms

@@ -287,8 +287,10 @@ int    mctdhb_apply_hamiltonian_cf_dev(const mctdhb_space* space, const void* co
 
 /* Tiling of the mixed H.C kernel on this space: 0 one configuration per
  * lane, 1 two configurations of one run per lane (one 16-byte gather serves
- * both).  By default both are timed at the first mixed call and the faster
- * kept.  set: -2 query only, -1 time both again, 0 or 1 force; *tiling
+ * both).  By default both are timed at the first mixed call on a space of
+ * a new (N, M) and the faster kept for every space of that shape in the
+ * process (so a solve runs the same kernels alone and among others).
+ * set: -2 query only, -1 time both again, 0 or 1 force; *tiling
  * (may be NULL) receives the current choice (-1: not yet timed).  Callers
  * that change it must not have a mixed SIL workspace open on the space. */
 int mctdhb_hc_tiling(mctdhb_space* space, int set, int* tiling);
@@ -405,6 +407,12 @@ int mctdhb_step_graph_create(mctdhb_space* space, mctdhb_sil* sil, mctdhb_orbita
                              void* coef, void* hob, void* hint, void* rho1, void* rho2, void* rinv,
                              mctdhb_step_graph** out);
 int mctdhb_step_graph_launch(mctdhb_step_graph* g, void* stream);
+/* `rounds` replays of n step graphs of independent solves: in each round
+ * graph i is launched on streams[i % nstreams], so every solve's steps stay
+ * in order on its own stream.  One call in place of rounds * n launches from
+ * the caller (many solves at once, reference driver.c:159-202 per solve). */
+int mctdhb_step_graph_launch_many(mctdhb_step_graph* const* graphs, int n, void* const* streams, int nstreams,
+                                  int rounds);
 int mctdhb_step_graph_destroy(mctdhb_step_graph* g);
 
 #ifdef __cplusplus
