@@ -82,10 +82,11 @@ class Instance:
     form (complex64 H.C, FP64 Lanczos scalars)."""
 
     def __init__(self, d: Draw, dt: float = 0.005, lanczos: int = 5, small_fp64_dim: int = 8192,
-                 quench_g: float | None = None):
+                 quench_g: float | None = None, fp64: bool = False):
         """quench_g: also capture a real-time step with interaction
         quench_g on the same state (`step_real`), for an interaction quench
-        after the imaginary-time relaxation."""
+        after the imaginary-time relaxation.  fp64: the FP64 form whatever
+        the size (the twin a mixed result is checked against)."""
         import torch
         import mctdhb_amd as amd
 
@@ -95,7 +96,7 @@ class Instance:
         L = d.half_width
         x = np.linspace(-L, L, grid)
         pot = d.a * (x ** 2 - d.b ** 2) ** 2
-        self.prec = amd.FP64 if d.dim <= small_fp64_dim else amd.MIXED
+        self.prec = amd.FP64 if (fp64 or d.dim <= small_fp64_dim) else amd.MIXED
         self.space = amd.DeviceSpace(N, M, tables=False)
         self.sil = amd.DeviceSil(self.space, lanczos, self.prec, amd.INDEX_FLY)
         self.orbital = amd.DeviceOrbital(M, grid, x[0], x[-1], dt, d.g, pot, imagtime=True)
@@ -114,6 +115,7 @@ class Instance:
         self.graph = amd.StepGraph(self.space, self.sil, self.orbital, t["orb"], t["coef"], t["hob"], t["hint"],
                                    t["r1"], t["r2"], t["rinv"])
         self.orbital_real = self.graph_real = None
+        self.quench_g = quench_g
         if quench_g is not None:
             self.orbital_real = amd.DeviceOrbital(M, grid, x[0], x[-1], dt, quench_g, pot, imagtime=False)
             self.graph_real = amd.StepGraph(self.space, self.sil, self.orbital_real, t["orb"], t["coef"], t["hob"],
@@ -150,12 +152,33 @@ class Instance:
         r1 = self.t["r1"].cpu().numpy().reshape(M, M)
         if not np.all(np.isfinite(r1)):
             return "rho1 not finite"
+        if abs(self.energy_per_particle()) > self.energy_bound():
+            return "energy above the grid's bound (diverged)"
         if abs(np.trace(r1).real - self.draw.npar) > tol * self.draw.npar:
             return "trace of rho1 is not N"
         nrm = float(np.linalg.norm(self.t["coef"].cpu().numpy()))
         if not abs(nrm - 1.0) <= tol:
             return "coefficient norm is not 1"
         return ""
+
+    def energy_bound(self) -> float:
+        """An upper bound of |E/N| for any normalised state on this grid:
+        max V, plus the largest grid kinetic energy (pi/dx)^2/2, plus
+        |g|(N - 1) times the largest |phi|^2 of a normalised grid function
+        (1/dx).  A result above it has diverged."""
+        d = self.draw
+        L = d.half_width
+        dx = 2 * L / d.ngrid
+        vmax = d.a * (L * L - d.b * d.b) ** 2
+        g = max(abs(d.g), abs(self.quench_g)) if self.quench_g is not None else abs(d.g)
+        return vmax + 0.5 * (np.pi / dx) ** 2 + g * (d.npar - 1) / dx
+
+    def observables(self) -> tuple[float, np.ndarray, np.ndarray]:
+        """(E/N, rho1, natural occupations in ascending order); call after a
+        synchronisation."""
+        M = self.draw.norb
+        r1 = self.t["r1"].cpu().numpy().reshape(M, M)
+        return self.energy_per_particle(), r1, np.sort(np.linalg.eigvalsh(r1)) / self.draw.npar
 
     def occupations(self) -> np.ndarray:
         M = self.draw.norb
@@ -171,6 +194,21 @@ class Instance:
         self.orbital.close()
         self.sil.close()
         self.space.close()
+
+
+def errors_vs_fp64(got: tuple, ref: tuple) -> dict:
+    """Errors of a result's observables against its FP64 twin's (both from
+    Instance.observables): energy per particle (relative), natural
+    occupations (largest relative deviation, and largest absolute one of
+    the fractions n_k / N), rho1 (relative, Frobenius)."""
+    (e, r1, occ), (e0, r10, occ0) = got, ref
+    if not (np.isfinite(e) and np.isfinite(e0)):
+        return {"energy_rel": None, "natural_occupations_rel": None, "natural_occupations_abs": None,
+                "rho1_rel_fro": None}
+    return {"energy_rel": abs(e - e0) / max(abs(e0), 1e-300),
+            "natural_occupations_rel": float(np.max(np.abs(occ - occ0) / np.maximum(np.abs(occ0), 1e-300))),
+            "natural_occupations_abs": float(np.max(np.abs(occ - occ0))),
+            "rho1_rel_fro": float(np.linalg.norm(r1 - r10) / max(np.linalg.norm(r10), 1e-300))}
 
 
 class Batch:

@@ -157,10 +157,16 @@ class Server:
 
     def __init__(self, budget_bytes: int, nstreams: int = 4, chunk: int = 8, max_active: int | None = None,
                  make_instance=_gpu_instance, make_event=_gpu_event, make_stream=_gpu_stream,
-                 wait_all=_gpu_wait_all):
+                 wait_all=_gpu_wait_all, reclaim_bytes: int | None = None):
         """make_instance / make_event / make_stream / wait_all: the GPU
-        objects by default; a test of the loop's host logic passes stand-ins."""
+        objects by default; a test of the loop's host logic passes stand-ins.
+        reclaim_bytes: a retired request's memory returns to the budget at
+        once, but its device buffers are released in bulk once this many
+        bytes of retired requests are held (and at the end): releasing
+        device memory waits for the whole device, so doing it per retirement
+        would drain the GPU each time.  None releases at every retirement."""
         self.budget = budget_bytes
+        self.reclaim_bytes = reclaim_bytes
         self.streams = [make_stream() for _ in range(max(1, nstreams))]
         self.chunk = chunk
         self.max_active = max_active
@@ -175,6 +181,16 @@ class Server:
                 raise ValueError(f"request {q.rid} needs {q.footprint} bytes, above the whole budget")
         active: list[_Active] = []
         used, peak_active, peak_used, out = 0, 0, 0, []
+        retired, retired_bytes, self.releases = [], 0, 0
+
+        def release_retired():
+            nonlocal retired, retired_bytes
+            if retired:
+                for inst in retired:
+                    inst.close()
+                self.releases += 1
+            retired, retired_bytes = [], 0
+
         admitted = 0  # round-robin stream assignment at admission
         t0 = time.perf_counter()
         while pending or active:
@@ -199,31 +215,43 @@ class Server:
                 if a.enqueued_all and a.event.query():
                     done = time.perf_counter() - t0
                     status, reason = result_status(a.inst)
+                    obs = a.inst.observables() if hasattr(a.inst, "observables") else None
                     rec = {"rid": a.req.rid, "kind": a.req.kind, "dim": a.req.draw.dim, "norb": a.req.draw.norb,
                            "npar": a.req.draw.npar, "steps": a.total, "arrival": a.req.arrival,
                            "admitted": a.admitted, "done": done, "time_to_solution": done - a.req.arrival,
                            "energy_per_particle": a.inst.energy_per_particle(), "status": status,
                            "reason": reason}
+                    rec["mixed"] = bool(getattr(a.inst, "prec", 0))
+                    if obs is not None:
+                        rec["observables"] = obs
                     if keep_state:
                         rec["state"] = {k: v.cpu().numpy().copy() for k, v in a.inst.t.items()}
                     out.append(rec)
-                    a.inst.close()
                     used -= a.req.footprint
+                    if self.reclaim_bytes is None:
+                        a.inst.close()
+                    else:
+                        retired.append(a.inst)
+                        retired_bytes += a.req.footprint
+                        if retired_bytes >= self.reclaim_bytes:
+                            release_retired()
                 else:
                     still.append(a)
             if len(still) == len(active) and all(a.enqueued_all for a in active):
                 self._wait_all()  # only completions are left to wait for
             active = still
+        release_retired()
         self.peak_active, self.peak_used = peak_active, peak_used
         return out
 
 
-def run_alone(req: Request) -> dict:
-    """The same request on its own (default stream), for comparisons."""
+def run_alone(req: Request, fp64: bool = False) -> dict:
+    """The same request on its own (default stream), for comparisons; fp64:
+    its FP64 twin, whatever the size."""
     import torch
 
     quench_g = req.draw.g * req.g_factor if req.kind == "quench" else None
-    inst = batch.Instance(req.draw, quench_g=quench_g)
+    inst = batch.Instance(req.draw, quench_g=quench_g, fp64=fp64)
     for _ in range(req.imag_steps):
         inst.step()
     for _ in range(req.real_steps):
@@ -231,8 +259,23 @@ def run_alone(req: Request) -> dict:
     torch.cuda.synchronize()
     st = {k: v.cpu().numpy().copy() for k, v in inst.t.items()}
     e = inst.energy_per_particle()
+    obs = inst.observables()
     inst.close()
-    return {"state": st, "energy_per_particle": e}
+    return {"state": st, "energy_per_particle": e, "observables": obs}
+
+
+def attach_fp64_errors(records: list[dict], requests: list[Request]) -> None:
+    """For every record of a mixed-precision solve, run the request's FP64
+    twin alone and store the relative errors of its energy, natural
+    occupations and rho1 (`errors_vs_fp64`); FP64 solves get zeros."""
+    by_id = {q.rid: q for q in requests}
+    for r in records:
+        if not r.get("mixed"):
+            r["errors_vs_fp64"] = {"energy_rel": 0.0, "natural_occupations_rel": 0.0,
+                                   "natural_occupations_abs": 0.0, "rho1_rel_fro": 0.0}
+            continue
+        twin = run_alone(by_id[r["rid"]], fp64=True)
+        r["errors_vs_fp64"] = batch.errors_vs_fp64(r["observables"], twin["observables"])
 
 
 def latency_summary(records: list[dict]) -> dict:

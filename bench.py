@@ -409,6 +409,28 @@ def batched_bench(args):
     b.synchronize()
     wall = time.perf_counter() - t0
     log(f"batched: {len(insts)} instances x {K} steps in {wall:.2f} s (setup {setup_s:.1f} s)")
+    # every mixed result against its FP64 twin after the same 1 + K steps
+    obs = [inst.observables() for inst in insts]
+    mixed_idx = [i for i, inst in enumerate(insts) if inst.prec != 0]
+    twins = [batch.Instance(draws[i], fp64=True) for i in mixed_idx]
+    tb = batch.Batch(twins, nstreams=args.batch_streams)
+    tb.run(1 + K)
+    tb.synchronize()
+    twin_obs = [t.observables() for t in twins]
+    errs = [batch.errors_vs_fp64(obs[i], o) for i, o in zip(mixed_idx, twin_obs)]
+    worst_occ = None
+    if errs:
+        j = max(range(len(errs)), key=lambda j: errs[j]["natural_occupations_rel"] or 0.0)
+        d = draws[mixed_idx[j]]
+        worst_occ = {"npar": d.npar, "norb": d.norb, "dim": d.dim, "g": d.g, **errs[j],
+                     "natural_occupations_fp64": [float(v) for v in twin_obs[j][2]],
+                     "natural_occupations_mixed": [float(v) for v in obs[mixed_idx[j]][2]]}
+    for t in twins:
+        t.close()
+
+    def worst(key):
+        vals = [e[key] for e in errs if e[key] is not None]
+        return max(vals) if vals else None
     # the same instances one after another on one stream, and together
     K2 = min(K, 10)
     st = b.streams[0]
@@ -434,9 +456,31 @@ def batched_bench(args):
            "dim_min_median_max": [int(dims.min()), int(np.median(dims)), int(dims.max())],
            "mixed_instances": int(sum(1 for i in insts if i.prec != 0)),
            "instances_with_finite_energy": len(finite),
-           "energy_per_particle_first8": e[:8]}
+           "energy_per_particle_first8": e[:8],
+           "errors_vs_fp64": {"what": f"every mixed instance against its FP64 twin after {1 + K} steps; FP64 "
+                                      f"instances are exact by construction",
+                              "mixed_checked": len(errs), "energy_rel_max": worst("energy_rel"),
+                              "natural_occupations_rel_max": worst("natural_occupations_rel"),
+                              "natural_occupations_abs_max": worst("natural_occupations_abs"),
+                              "rho1_rel_fro_max": worst("rho1_rel_fro"), "worst_occupations": worst_occ}}
     for inst in insts:
         inst.close()
+    return out
+
+
+def worst_requests(recs, by_id, n=5):
+    """The n mixed solved requests with the largest energy error against
+    their FP64 twins, with what characterises them."""
+    rows = [r for r in recs if r["mixed"] and r["status"] == "ok" and r["errors_vs_fp64"]["energy_rel"] is not None]
+    rows.sort(key=lambda r: -r["errors_vs_fp64"]["energy_rel"])
+    out = []
+    for r in rows[:n]:
+        q = by_id[r["rid"]]
+        d = q.draw
+        out.append({"rid": r["rid"], "kind": r["kind"], "npar": d.npar, "norb": d.norb, "dim": d.dim,
+                    "ngrid": d.ngrid, "g": d.g, "g_factor": q.g_factor, "imag_steps": q.imag_steps,
+                    "real_steps": q.real_steps, "energy_per_particle": r["energy_per_particle"],
+                    **r["errors_vs_fp64"]})
     return out
 
 
@@ -450,12 +494,21 @@ def serve_bench(args):
     reqs = serve.draw_requests(args.serve_rate, args.serve_window, seed=args.seed,
                                time_scale=args.serve_time_scale)
     srv = serve.Server(int(args.serve_budget_gb * 1e9), nstreams=8, chunk=8,
-                       max_active=args.serve_max_active or None)
+                       max_active=args.serve_max_active or None,
+                       reclaim_bytes=int(args.serve_reclaim_gb * 1e9) if args.serve_reclaim_gb > 0 else None)
     t0 = time.perf_counter()
     recs = srv.run(reqs)
     wall = time.perf_counter() - t0
     steps = sum(r["steps"] for r in recs)
     lat = serve.latency_summary(recs)
+    t1 = time.perf_counter()
+    serve.attach_fp64_errors(recs, reqs)  # every mixed result against its FP64 twin, run alone
+    log(f"served: FP64 twins of the mixed results in {time.perf_counter() - t1:.1f} s")
+    checked = [r["errors_vs_fp64"] for r in recs if r["mixed"] and r["status"] == "ok"]
+
+    def worst(key):
+        vals = [e[key] for e in checked if e[key] is not None]
+        return max(vals) if vals else None
     log(f"served {len(recs)} requests in {wall:.2f} s ({lat['count']} solved): time to solution p50 "
         f"{lat['p50_s']} s, p99 {lat['p99_s']} s, at most {srv.peak_active} at once")
     dims = np.array([r["dim"] for r in recs])
@@ -478,8 +531,16 @@ def serve_bench(args):
             "instance_steps": steps, "instance_steps_per_s": steps / wall,
             "time_to_solution": lat, "peak_admitted": srv.peak_active,
             "budget_gb": args.serve_budget_gb, "max_active": args.serve_max_active or None,
+            "reclaim_gb": args.serve_reclaim_gb, "bulk_releases": srv.releases,
             "dim_min_median_max": [int(dims.min()), int(np.median(dims)), int(dims.max())],
-            "failed_requests": failed[:20]}
+            "failed_requests": failed[:20],
+            "errors_vs_fp64": {"what": "every solved mixed request against its FP64 twin run alone; FP64 requests "
+                                       "are exact by construction",
+                               "mixed_checked": len(checked), "energy_rel_max": worst("energy_rel"),
+                               "natural_occupations_rel_max": worst("natural_occupations_rel"),
+                               "natural_occupations_abs_max": worst("natural_occupations_abs"),
+                               "rho1_rel_fro_max": worst("rho1_rel_fro"),
+                               "worst5": worst_requests(recs, by_id)}}
 
 
 def main():
@@ -513,6 +574,8 @@ def main():
     ap.add_argument("--serve-time-scale", type=float, default=0.02)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--serve-max-active", type=int, default=0, help="admitted at once (0: budget only)")
+    ap.add_argument("--serve-reclaim-gb", type=float, default=64.0,
+                    help="retired requests' device memory is released in bulk once this much is held (0: at once)")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-loop timing")
     ap.add_argument("--serve-only", action="store_true", help="only the request loop (its JSON object)")
     ap.add_argument("--full-step-only", action="store_true",

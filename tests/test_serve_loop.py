@@ -121,3 +121,36 @@ def test_loop_rejects_a_request_above_the_whole_budget():
         assert "above the whole budget" in str(e)
     else:
         raise AssertionError("expected a ValueError")
+
+
+def test_retired_requests_are_released_in_bulk():
+    FakeInstance.log.clear()
+    reqs = _requests()
+    big = 10 * max(q.footprint for q in reqs)
+    srv = serve.Server(big, nstreams=2, chunk=4, make_instance=FakeInstance, make_event=FakeEvent,
+                       make_stream=object, wait_all=lambda: None, reclaim_bytes=big)
+    recs = srv.run(reqs)
+    assert sorted(r["rid"] for r in recs) == [0, 1, 2, 3, 4]
+    closes = [rid for ev, rid, _ in FakeInstance.log if ev == "close"]
+    assert sorted(closes) == [0, 1, 2, 3, 4]
+    assert srv.releases == 1  # nothing reached the threshold: one release at the end
+    # every close after the last completion
+    t_last_done = max(t for ev, rid, t in FakeInstance.log if ev == "admit")
+    assert all(t >= t_last_done for ev, rid, t in FakeInstance.log if ev == "close")
+
+
+def test_energy_bound_covers_the_grid_and_flags_divergence():
+    """The divergence bound of Instance.health: above V(L) = 200 and the
+    grid's largest kinetic energy, growing with |g| (N - 1), and using the
+    larger of the relaxation and quench interactions."""
+    import math
+
+    d = batch.Draw(20, 3, 256, 1.0, 2.0, 0.5, 0)
+    inst = batch.Instance.__new__(batch.Instance)
+    inst.draw, inst.quench_g = d, None
+    dx = 2 * d.half_width / d.ngrid
+    b0 = inst.energy_bound()
+    assert b0 >= 200.0 * (1 - 1e-12) + 0.5 * (math.pi / dx) ** 2
+    inst.quench_g = -3.0  # an attractive quench, larger in magnitude
+    assert inst.energy_bound() > b0
+    assert 1.4e30 > inst.energy_bound()  # a diverged quench's E/N is far above it
