@@ -186,12 +186,17 @@ extern "C" int mctdhb_hc_tiling(mctdhb_space* sp, int set, int* tiling) {
         set_error("null space");
         return kInvalidArgument;
     }
-    if (set < -2 || set > 1) {
-        set_error("tiling must be -2 (query), -1 (time both again), 0 (one configuration per lane) or 1 (pairs)");
+    if (set < -2 || set > 2) {
+        set_error("tiling must be -2 (query), -1 (time again), 0 (one configuration per lane), 1 (pairs) or 2 "
+                  "(coefficient windows)");
         return kInvalidArgument;
     }
-    if (set == 1 && !pair_possible(sp->S)) {
+    if (set == kTilingPair && !pair_possible(sp->S)) {
         set_error("this space has no pair work list (needs 2 <= norb <= 7 and dim >= 2)");
+        return kInvalidArgument;
+    }
+    if (set == kTilingWindow && !hc_window_possible(sp->S)) {
+        set_error("the windowed H.C needs 2 <= norb <= 5 and windows (256 + 4 N + 6 entries) within its LDS budget");
         return kInvalidArgument;
     }
     if (set >= 0) sp->S->hc_tiling = set;

@@ -8,6 +8,7 @@
 // deltas of fock.hpp (W table staged in LDS).  See hamiltonian.hpp for the
 // rule grouping.
 #include "hamiltonian_impl.hpp"
+#include "hamiltonian_window.hpp"
 #include "rowctx.hpp"
 
 #include <map>
@@ -43,10 +44,13 @@ void hc_forget_tiling(const Space* S) {
     S->hc_tiling = -1;
 }
 
+bool hc_window_possible(const Space* S) { return window_possible(S); }
+
 int hc_tune_tiling(const Space* S, hipStream_t st) {
     if (S->hc_tiling >= 0) return kOk;
-    if (!pair_possible(S)) {
-        S->hc_tiling = 0;
+    const bool cand[3] = {true, pair_possible(S), hc_window_possible(S)};
+    if (!cand[kTilingPair] && !cand[kTilingWindow]) {
+        S->hc_tiling = kTilingLane;
         return kOk;
     }
     const std::pair<int, int> shape(S->npar, S->norb);
@@ -72,10 +76,11 @@ int hc_tune_tiling(const Space* S, hipStream_t st) {
     hipEvent_t e0, e1;
     MCTDHB_HIP_CHECK(hipEventCreate(&e0));
     MCTDHB_HIP_CHECK(hipEventCreate(&e1));
-    float best[2] = {1e30f, 1e30f};
+    float best[3] = {1e30f, 1e30f, 1e30f};
     int rc = kOk;
     for (int rep = 0; rep < 3 && !rc; rep++) {
-        for (int t = 0; t < 2 && !rc; t++) {
+        for (int t = 0; t < 3 && !rc; t++) {
+            if (!cand[t]) continue;
             hipEventRecord(e0, st);
             switch (M) {
                 case 2: rc = HcFused<2>::plain_tiled(S, in, cf, out, t, st); break;
@@ -99,7 +104,10 @@ int hc_tune_tiling(const Space* S, hipStream_t st) {
     hipFreeAsync(out, st);
     MCTDHB_HIP_CHECK(hipStreamSynchronize(st));
     if (rc) return rc;
-    S->hc_tiling = best[1] < best[0] ? 1 : 0;
+    int pick = kTilingLane;
+    for (int t = 1; t < 3; t++)
+        if (cand[t] && best[t] < best[pick]) pick = t;
+    S->hc_tiling = pick;
     std::lock_guard<std::mutex> lock(g_tiling_mu);
     g_tiling_by_shape[shape] = S->hc_tiling;
     return kOk;

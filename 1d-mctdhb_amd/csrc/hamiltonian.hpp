@@ -104,12 +104,24 @@ int hc_lanczos(const Space* S, const double2* cf64, const float2* cf32, const La
                int index, hipStream_t st);
 
 unsigned hc_grid_blocks(uint32_t dim);
-// The mixed form has two tilings: one configuration per lane, or two
-// configurations of one run per lane (the pair work list of the space).
+// The mixed form has three tilings: one configuration per lane (0), two
+// configurations of one run per lane (1, the pair work list of the space),
+// or one configuration per lane with each block's coefficient windows staged
+// in LDS, one far group of jump types after another (2, 2 <= norb <= 5 and
+// N up to 185; the plain product only: the Lanczos forms keep the one-lane
+// kernel under it).
+constexpr int kTilingLane = 0, kTilingPair = 1, kTilingWindow = 2;
 inline bool pair_possible(const Space* S) { return S->d_slots && S->dim >= 2 && S->norb >= 2 && S->norb <= 7; }
-inline bool use_pair(const Space* S, int prec) { return prec == kPrecMixed && pair_possible(S) && S->hc_tiling == 1; }
-// Times both mixed tilings once per (N, M) (zero data: the kernels' cost
-// does not depend on the values) and records the faster in S->hc_tiling.
+inline bool use_pair(const Space* S, int prec) {
+    return prec == kPrecMixed && pair_possible(S) && S->hc_tiling == kTilingPair;
+}
+// Whether the windows of this (N, M) fit the windowed kernel's LDS budget.
+bool hc_window_possible(const Space* S);
+inline bool use_window(const Space* S, int prec) {
+    return prec == kPrecMixed && S->hc_tiling == kTilingWindow && hc_window_possible(S);
+}
+// Times the possible mixed tilings once per (N, M) (zero data: the kernels'
+// cost does not depend on the values) and records the fastest in S->hc_tiling.
 // Synchronises the stream; no-op once done.
 int hc_tune_tiling(const Space* S, hipStream_t st);
 // The choice is kept per (N, M) for the process (so every space of a shape
@@ -168,7 +180,7 @@ int hc_sil_small(const Space* S, const double2* cf64, const SmallSilArgs& a, int
 // mixed form always uses on-the-fly indices.
 template <int M> struct HcFused {
     static int plain(const Space* S, const void* c, const void* cf, void* out, int prec, int index, hipStream_t st);
-    // mixed form with an explicit tiling (0 per lane, 1 pairs; c padded for 1)
+    // mixed form with an explicit tiling (0 per lane, 1 pairs, 2 windows; c padded for 1)
     static int plain_tiled(const Space* S, const void* c, const void* cf, void* out, int tiling, hipStream_t st);
     static int lanczos(const Space* S, const void* cf, const LanczosHcArgs& a, int prec, int index, hipStream_t st);
     static int sil_small(const Space* S, const void* cf, const SmallSilArgs& a, int index, hipStream_t st);

@@ -15,6 +15,7 @@
 #pragma once
 
 #include "hamiltonian_impl.hpp"
+#include "hamiltonian_window.hpp"
 #include "sil_small.hpp"
 
 namespace mctdhb {
@@ -359,19 +360,29 @@ int HcFused<M>::plain(const Space* S, const void* c, const void* cf, void* out, 
         }
         return launch_pair<M>(S, cp, (const float2*) cf, 0, nullptr, EpiPlainPair<float2>{(float2*) out}, st);
     }
+    if (use_window(S, prec))
+        return launch_window<M>(S, (const float2*) c, (const float2*) cf, 0, nullptr, EpiPlain<float2>{(float2*) out},
+                                st);
     return launch_fused<M, float, float2>(S, (const float2*) c, (const float2*) cf, kIndexOnTheFly, 0, nullptr,
                                           EpiPlain<float2>{(float2*) out}, st);
 }
 
 template <int M>
 int HcFused<M>::plain_tiled(const Space* S, const void* c, const void* cf, void* out, int tiling, hipStream_t st) {
-    if (tiling == 1)
+    if (tiling == kTilingPair)
         return launch_pair<M>(S, (const float2*) c, (const float2*) cf, 0, nullptr, EpiPlainPair<float2>{(float2*) out},
                               st);
+    if (tiling == kTilingWindow)
+        return launch_window<M>(S, (const float2*) c, (const float2*) cf, 0, nullptr, EpiPlain<float2>{(float2*) out},
+                                st);
     return launch_fused<M, float, float2>(S, (const float2*) c, (const float2*) cf, kIndexOnTheFly, 0, nullptr,
                                           EpiPlain<float2>{(float2*) out}, st);
 }
 
+// The Lanczos forms run the one-lane kernel under the window tiling too: the
+// windowed kernel is bound by vector ALU work, so the epilogue's dot products
+// and sums cost it their full time (measured at N = 100, M = 5: 317-416 us
+// per iteration against 295-396 us, and 188 against 149 us for the half row).
 template <int M>
 int HcFused<M>::lanczos(const Space* S, const void* cf, const LanczosHcArgs& a, int prec, int index,
                         hipStream_t st) {

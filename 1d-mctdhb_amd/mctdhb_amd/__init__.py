@@ -229,7 +229,8 @@ class DeviceSpace:
 
     def hc_tiling(self, set: int = -2) -> int:
         """Tiling of the mixed H.C kernel (0 per lane, 1 configuration
-        pairs); set -1 to time both again, 0/1 to force."""
+        pairs, 2 coefficient windows in LDS); set -1 to time the possible
+        ones again, 0/1/2 to force."""
         t = _i()
         check(lib().mctdhb_hc_tiling(self._h, set, C.byref(t)))
         return t.value

@@ -287,10 +287,13 @@ int    mctdhb_apply_hamiltonian_cf_dev(const mctdhb_space* space, const void* co
 
 /* Tiling of the mixed H.C kernel on this space: 0 one configuration per
  * lane, 1 two configurations of one run per lane (one 16-byte gather serves
- * both).  By default both are timed at the first mixed call on a space of
- * a new (N, M) and the faster kept for every space of that shape in the
- * process (so a solve runs the same kernels alone and among others).
- * set: -2 query only, -1 time both again, 0 or 1 force; *tiling
+ * both), 2 one configuration per lane with the coefficients each block
+ * reaches staged in LDS, one window per far group of jump types
+ * (2 <= norb <= 5, N up to about 330).  By default the possible ones are
+ * timed at the first mixed call on a space of a new (N, M) and the fastest
+ * kept for every space of that shape in the process (so a solve runs the
+ * same kernels alone and among others).
+ * set: -2 query only, -1 time again, 0, 1 or 2 force; *tiling
  * (may be NULL) receives the current choice (-1: not yet timed).  Callers
  * that change it must not have a mixed SIL workspace open on the space. */
 int mctdhb_hc_tiling(mctdhb_space* space, int set, int* tiling);

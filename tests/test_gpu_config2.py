@@ -171,3 +171,27 @@ def test_mixed_full_size(amd, dspace, prob, ref_hc, ospace):
     dspace.density(dc32, r1, r2, prec=amd.MIXED)
     assert gu.rel_err(gu.host(r1).reshape(NORB, NORB), ospace.onebody_dm(c)) < 2e-6
     assert gu.rel_err(gu.host(r2), ospace.twobody_dm(c)) < 2e-6
+
+
+@pytest.mark.parametrize("tiling", [0, 1, 2])
+def test_mixed_tilings_full_size(amd, dspace, prob, ref_hc, tiling):
+    """Each mixed H.C tiling (one configuration per lane, pairs, coefficient
+    windows in LDS) at the benchmark size against the oracle; the windowed
+    one at this size copies 19 far-group windows per block and sends the
+    lanes whose targets leave them (blocks crossing a change of P_2 or P_3,
+    and the rows next to the last configuration) through the gathering
+    path, so both of its paths are exercised."""
+    print(f"config2 mixed hc tiling {tiling}", flush=True)
+    c, h, g = prob
+    before = dspace.hc_tiling()
+    try:
+        dspace.hc_tiling(tiling)
+        out = gu.empty(DIM, "complex64")
+        dspace.apply_hamiltonian(gu.dev(c, np.complex64), gu.dev(h), gu.dev(g), out, prec=amd.MIXED,
+                                 index=amd.INDEX_FLY)
+        got = gu.host(out)
+        assert gu.rel_err(got, ref_hc) < 2e-5
+        # entry-wise too: a wrong window entry would show in a few rows only
+        assert np.max(np.abs(got - ref_hc)) <= 2e-5 * np.max(np.abs(ref_hc))
+    finally:
+        dspace.hc_tiling(before if before >= 0 else -1)

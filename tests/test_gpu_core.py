@@ -77,8 +77,10 @@ def test_hc_fp64_matches_oracle(amd, orc, npar, norb):
 # at the ends of odd-length runs (the single slots of the paired kernel)
 @pytest.mark.parametrize("npar,norb", [(9, 2), (10, 4), (10, 5), (5, 7), (40, 4), (1, 2), (2, 2), (1, 3),
                                        (2, 3), (3, 4), (2, 7), (1, 5), (7, 6), (60, 3)])
-@pytest.mark.parametrize("tiling", [0, 1])
+@pytest.mark.parametrize("tiling", [0, 1, 2])
 def test_hc_mixed_matches_oracle(amd, orc, npar, norb, tiling):
+    if tiling == 2 and norb > 5:
+        pytest.skip("the windowed tiling covers 2 <= M <= 5")
     print(f"hc mixed N={npar} M={norb} tiling={tiling}", flush=True)
     S = orc.Space(npar, norb)
     c, h, g = _problem(npar, norb)
@@ -254,8 +256,10 @@ def test_eig_residual_matches_definition(amd, orc, npar, norb):
 
 @pytest.mark.parametrize("npar,norb,lm,imag", [(10, 5, 8, False), (9, 2, 5, True), (2, 3, 3, False),
                                                (6, 7, 6, True), (30, 4, 18, False)])
-@pytest.mark.parametrize("tiling", [0, 1])
+@pytest.mark.parametrize("tiling", [0, 1, 2])
 def test_sil_step_mixed_matches_oracle(amd, orc, npar, norb, lm, imag, tiling):
+    if tiling == 2 and norb > 5:
+        pytest.skip("the windowed tiling covers 2 <= M <= 5")
     print(f"sil mixed N={npar} M={norb} lm={lm} imag={imag} tiling={tiling}", flush=True)
     S = orc.Space(npar, norb)
     c, h, g = _problem(npar, norb, 43)
