@@ -248,9 +248,14 @@ __device__ __forceinline__ double2 hc_row_fused(uint32_t i, const RowCtx& ctx, c
 
 // Epilogues receive the row value and the row's own coefficient c_i (as
 // gathered by the row), and are called by every thread of the block.
+// An epilogue may load what it needs before the row is formed
+// (prefetch(i, valid), handed back as the last argument).
+struct NoPre {};
 template <typename Zout> struct EpiPlain {
     Zout* out;
-    __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 v, double2 ci, double* lds) const {
+    __device__ __forceinline__ NoPre prefetch(uint32_t, bool) const { return {}; }
+    __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 v, double2 ci, double* lds,
+                                               NoPre = {}) const {
         if (valid) out[i] = from_d2<Zout>(v);
     }
 };
@@ -283,16 +288,48 @@ struct SlotSums {
 
 template <typename Zw> struct EpiLanczos {
     static constexpr bool kReuse = std::is_same<Zw, float2>::value;
+    // complex64: w_{it-1} .. w_{it-kPre} at this lane's row are loaded before
+    // the row is formed, so their HBM latency overlaps the gathers (loaded in
+    // the epilogue, it stalled every wave at its end); older vectors are read
+    // there.  kPre complex64 values are 2 kPre VGPRs, within the kernel's
+    // occupancy step.  complex128: everything is read in the epilogue (an
+    // early read slowed the FP64 step by 8 %).
+    static constexpr int kPre = kReuse ? 4 : 1;
+    struct Pre {
+        Zw w[kPre];
+    };
     LanczosHcArgs a;
+    __device__ __forceinline__ Pre prefetch(uint32_t i, bool valid) const {
+        Pre p;
+        const int it = a.it;
+        // the last iteration needs w_{it-1} only (the recurrence)
+        const int lo = a.last ? it - 1 : 0;
+#pragma unroll
+        for (int j = 0; j < kPre; j++) {
+            const int k = it - 1 - j;
+            p.w[j] = Zw{};
+            if (kReuse && valid && k >= 0 && k >= lo) p.w[j] = ((const Zw*) a.w[k])[i];
+        }
+        return p;
+    }
     __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 row, double2 ci, double* lds) const {
+        (*this)(i, valid, row, ci, lds, prefetch(i, valid));
+    }
+    __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 row, double2 ci, double* lds,
+                                               const Pre& pre) const {
         const SilState* st = a.st;
         const int it = a.it;
-        // w_it[i] is the row's own coefficient; w_{it-1}[i] is read once
+        // w_k[i] for it - kPre <= k < it: from the prefetch (complex64)
+        auto earlier = [&](int j) {
+            return kReuse ? to_d2(pre.w[j]) : to_d2(((const Zw*) a.w[it - 1 - j])[i]);
+        };
+        // w_it[i] is the row's own coefficient (complex64) or read here
+        // (complex128); w_{it-1}[i] is earlier(0)
         double2 o = make_double2(0, 0), wprev = make_double2(0, 0);
         if (valid) {
             o = cscale(row, st->s[it]);
             if (it > 0) {
-                wprev = to_d2(((const Zw*) a.w[it - 1])[i]);
+                wprev = earlier(0);
                 o = o - cscale(wprev, st->beta[it - 1] * st->s[it - 1]);
             }
             if (!a.last) a.out[i] = o;
@@ -300,19 +337,21 @@ template <typename Zw> struct EpiLanczos {
         // slot-major partials: slot t of block b at part[t * gridDim.x + b]
         const SlotSums ss(lds);
         const int k0 = a.last ? it : 0;
-        for (int k = k0; k <= it; k++) {
+        auto put_dot = [&](int k, double2 wk) {
             double2 p = make_double2(0, 0);
-            if (valid) {
-                // (complex64 only: reusing c_i in the complex128 kernel
-                // lengthens its live range and costs more than the load)
-                const double2 wk = (kReuse && k == it) ? ci
-                                                       : (k == it - 1 ? wprev : to_d2(((const Zw*) a.w[k])[i]));
-                p = cscale(cconjmul(wk, o), st->s[k]);
-            }
+            if (valid) p = cscale(cconjmul(wk, o), st->s[k]);
             ss.put(2 * k, p.x);
             ss.put(2 * k + 1, p.y);
-        }
+        };
+        // k = it
+        put_dot(it, valid ? (kReuse ? ci : to_d2(((const Zw*) a.w[it])[i])) : make_double2(0, 0));
         if (!a.last) {
+            // it - kPre <= k < it from the registers, older ones from memory
+#pragma unroll
+            for (int j = 0; j < kPre; j++)
+                if (it - 1 - j >= 0) put_dot(it - 1 - j, !valid ? make_double2(0, 0) : j == 0 ? wprev : earlier(j));
+            for (int k = 0; k < it - kPre; k++)
+                put_dot(k, valid ? to_d2(((const Zw*) a.w[k])[i]) : make_double2(0, 0));
             double n2 = 0;  // |w_0|^2
             if (valid && it == 0) {
                 const double2 w0 = kReuse ? ci : to_d2(((const Zw*) a.w[0])[i]);
@@ -329,7 +368,9 @@ template <typename Zw> struct EpiLanczos {
 // into slot 2 it; alpha is formed from it in the scalar update.
 struct EpiRayleigh {
     LanczosHcArgs a;
-    __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 row, double2 ci, double* lds) const {
+    __device__ __forceinline__ NoPre prefetch(uint32_t, bool) const { return {}; }
+    __device__ __forceinline__ void operator()(uint32_t i, bool valid, double2 row, double2 ci, double* lds,
+                                               NoPre = {}) const {
         double v[1] = {valid ? row.x : 0.0};
         block_sum<1>(v, lds);
         if (threadIdx.x == 0) {
@@ -361,13 +402,14 @@ __global__ __launch_bounds__(kHcBlock) void k_hc_fused(RowCtx ctx, const Zin* __
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t i = b * kHcBlock + threadIdx.x;
     const bool valid = i < ctx.dim;
+    const auto pre = epi.prefetch(i, valid);
     double2 v = make_double2(0, 0), ci = make_double2(0, 0);
     if (valid) {
         cplx<R> own;
         v = hc_row_fused<M, R, IDX, Zin, HALF>(i, ctx, c, cf, Wl, ldw, sql, own);
         if constexpr (std::is_same<Zin, float2>::value) ci = to_d2(own);
     }
-    epi(i, valid, v, ci, red);
+    epi(i, valid, v, ci, red, pre);
 }
 
 static inline RowCtx row_ctx(const Space* S) {

@@ -809,7 +809,8 @@ def main():
         "hc_effective_gbs": {"fp64_tables": ab_f64["hc"] / hc_ms["fp64_tables"] / 1e6,
                              "fp64_fly": ab_f64["hc"] / hc_ms["fp64_fly"] / 1e6,
                              "mixed_fly": ab_c64["hc"] / hc_ms["mixed_fly"] / 1e6},
-        "hc_mixed_tiling": {0: "one configuration per lane", 1: "configuration pairs"}.get(space.hc_tiling(), "?"),
+        "hc_mixed_tiling": {0: "one configuration per lane", 1: "configuration pairs",
+                            2: "coefficient windows in LDS"}.get(space.hc_tiling(), "?"),
         "density_ms": dm_ms,
         "table_build_s": table_build_s,
         "roofline": roofline,
