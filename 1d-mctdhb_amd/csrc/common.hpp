@@ -94,6 +94,22 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// lane i gets v[i - K] within its row of 16 (0 where that leaves the row);
+// every lane of the wave must be active
+template <int K> __device__ __forceinline__ double row_shr(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x110 + K, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x110 + K, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+// Lane 15 of each row of 16 gets the row's sum (fixed order); every lane of
+// the wave must be active.
+__device__ __forceinline__ double row_sum(double v) {
+    v += row_shr<1>(v);
+    v += row_shr<2>(v);
+    v += row_shr<4>(v);
+    return v + row_shr<8>(v);
+}
+
 // Block-wide sum of NV doubles per thread into lane-0-of-block results.
 // `scratch` must hold NV * (blockDim.x / 64) doubles.  Returns the totals in
 // thread 0 (other threads get garbage).  Deterministic for a fixed launch.

@@ -282,12 +282,6 @@ __global__ __launch_bounds__(kDmBlock) void k_density(RowCtx ctx, const cplx<R>*
 // recomputed per element family, and no per-thread accumulator array is
 // needed.  The mixed form forms the products in complex64 and sums in FP64.
 
-// lane i gets v[i - K] within its row of 16 (0 where that leaves the row)
-template <int K> __device__ __forceinline__ double row_shr(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x110 + K, 0xf, 0xf, true);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x110 + K, 0xf, 0xf, true);
-    return __hiloint2double(hi, lo);
-}
 __device__ __forceinline__ double readlane_d(double v, int l) {
     return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
                             __builtin_amdgcn_readlane(__double2loint(v), l));
@@ -295,10 +289,7 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
 // Sum of v over the 64 lanes of the wave, returned in every lane; every lane
 // of the wave must be active.  The order of the additions is fixed.
 __device__ __forceinline__ double wave_total(double v) {
-    v += row_shr<1>(v);
-    v += row_shr<2>(v);
-    v += row_shr<4>(v);
-    v += row_shr<8>(v);  // lane 15 of each row: the row's sum
+    v = row_sum(v);  // lane 15 of each row: the row's sum
     return (readlane_d(v, 15) + readlane_d(v, 31)) + (readlane_d(v, 47) + readlane_d(v, 63));
 }
 

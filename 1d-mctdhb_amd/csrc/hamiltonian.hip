@@ -329,7 +329,10 @@ __global__ __launch_bounds__(kHcBlockG) void k_hc_generic(int M, RowCtx ctx, con
                                                          const double2* __restrict__ hob,
                                                          const double2* __restrict__ hint, int stop_check,
                                                          const SilState* st, Epi epi) {
-    __shared__ double red[kHcBlockG / 64 * 2];
+    // the Lanczos epilogue's slot partials (SlotSums: up to 2 kMaxKrylov + 4
+    // slots of blockDim / 16 row sums)
+    static_assert(kHcBlockG == kHcBlock, "the epilogue partials are sized for kHcBlock threads");
+    __shared__ double red[kRedBytes / sizeof(double)];
     if (stop_check && st->stop) return;
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t i = b * kHcBlockG + threadIdx.x;

@@ -260,27 +260,30 @@ template <typename Zout> struct EpiPlain {
     }
 };
 
-// LDS bytes in front of the W table of the fused kernels: per-wave partial
-// sums of up to 2 kMaxKrylov + 4 slots.
-constexpr int kRedBytes = 8 * (kHcBlock / 64) * (2 * kMaxKrylov + 4);
+// LDS bytes in front of the W table of the fused kernels: per-row partial
+// sums (4 rows of 16 lanes per wave) of up to 2 kMaxKrylov + 4 slots.
+constexpr int kRedBytes = 8 * (kHcBlock / 16) * (2 * kMaxKrylov + 4);
 
-// Per-wave partial sums into LDS, one barrier for any number of slots:
-// put() reduces a value over the wave, finish() adds the waves in a fixed
-// order and stores slot t of this block at part[t * ps].
+// Per-row partial sums into LDS, one barrier for any number of slots: put()
+// sums a value over each 16-lane row with DPP shifts (vector ALU only; the
+// cross-lane shuffles of a whole-wave sum are LDS permutes whose latency
+// chain ended every wave), finish() adds the block's rows in a fixed order
+// and stores slot t of this block at part[t * ps].  Every thread of the
+// block calls both.
 struct SlotSums {
     double* red;
-    int nw, wid, lane;
+    int nr, row, lane;
     __device__ __forceinline__ SlotSums(double* lds)
-        : red(lds), nw(blockDim.x >> 6), wid(threadIdx.x >> 6), lane(threadIdx.x & 63) {}
+        : red(lds), nr(blockDim.x >> 4), row(threadIdx.x >> 4), lane(threadIdx.x & 15) {}
     __device__ __forceinline__ void put(int slot, double v) const {
-        v = wave_sum(v);
-        if (lane == 0) red[slot * nw + wid] = v;
+        v = row_sum(v);
+        if (lane == 15) red[slot * nr + row] = v;
     }
     __device__ __forceinline__ void finish(int slot0, int nslot, double* part, size_t ps) const {
         __syncthreads();
         for (int t = threadIdx.x; t < nslot; t += blockDim.x) {
             double s = 0;
-            for (int w = 0; w < nw; w++) s += red[(slot0 + t) * nw + w];
+            for (int r = 0; r < nr; r++) s += red[(slot0 + t) * nr + r];
             part[(size_t) (slot0 + t) * ps] = s;
         }
     }
