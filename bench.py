@@ -208,6 +208,7 @@ def probe_hc(args, space, C0, H, G, stream):
     import torch
     import mctdhb_amd as amd
 
+    space.hc_tiling(args.probe_tiling)  # default: the one-lane kernel (the SIL step's Lanczos forms run it)
     cf = torch.empty(space.coef_bytes(amd.MIXED), dtype=torch.uint8, device="cuda")
     space.hc_coefficients(H, G, cf, amd.MIXED, stream=stream)
     src = C0.to(torch.complex64)
@@ -582,6 +583,8 @@ def main():
                     help="only the whole-step timing (prints its JSON object)")
     ap.add_argument("--probe-sil", action="store_true",
                     help="only two mixed SIL steps and a plain copy kernel of known size, for hardware-counter runs")
+    ap.add_argument("--probe-tiling", type=int, default=0,
+                    help="mixed H.C tiling of --probe-hc (0 one configuration per lane, 1 pairs, 2 windows)")
     ap.add_argument("--probe-hc", action="store_true",
                     help="only the mixed H.C kernel (5 launches) and a plain copy kernel of known size, "
                          "for hardware-counter runs")
@@ -633,11 +636,18 @@ def main():
         probe_sil(args, space, C0, H, G, stream)
         return
 
-    # ---- H.C kernel alone (coefficient block prebuilt), three forms
+    # ---- H.C kernel alone (coefficient block prebuilt): FP64 with tables and
+    # on the fly, mixed with the tiling the space chose, and mixed with one
+    # configuration per lane (the kernel of the SIL step's Lanczos forms,
+    # whose roofline is reported)
     hc_ms = {}
     forms = {"fp64_tables": (amd.FP64, amd.INDEX_TABLES), "fp64_fly": (amd.FP64, amd.INDEX_FLY),
-             "mixed_fly": (amd.MIXED, amd.INDEX_FLY)}
+             "mixed_fly": (amd.MIXED, amd.INDEX_FLY), "mixed_fly_lane": (amd.MIXED, amd.INDEX_FLY)}
+    tuned = None
     for name, (prec, index) in forms.items():
+        if name == "mixed_fly_lane":
+            tuned = space.hc_tiling()
+            space.hc_tiling(0)
         cf = torch.empty(space.coef_bytes(prec), dtype=torch.uint8, device=dev)
         space.hc_coefficients(H, G, cf, prec, stream=stream)
         src = C0 if prec == amd.FP64 else C0.to(torch.complex64)
@@ -653,6 +663,7 @@ def main():
         hc_ms[name] = a.elapsed_time(b) / args.hc_reps
         log(f"H.C {name}: {hc_ms[name]:.3f} ms")
         del cf, out
+    space.hc_tiling(tuned)
 
     # ---- density matrices (rho1 + rho2)
     dm_ms = {}
@@ -738,7 +749,7 @@ def main():
     # algorithmic bytes are what this algorithm must move: coefficients in
     # and out (8 B each) and the byte-packed occupation row (8 B); it reads
     # no jump tables.  The table-driven figure is kept as `effective_gbs`.
-    t_hc = hc_ms["mixed_fly"] / 1e3
+    t_hc = hc_ms["mixed_fly_lane"] / 1e3
     hc_bytes = 24 * nc
     traffic = None if args.no_traffic else measure_traffic(args)
     waves = -(-nc // 64)
@@ -754,7 +765,7 @@ def main():
         "traffic": traffic["bytes"] if traffic else None,
         "algorithmic_bytes_per_launch": hc_bytes,
         "algorithmic_bytes": "coefficients in + out (8 B each) + packed occupation row (8 B) per configuration",
-        "launch_us": hc_ms["mixed_fly"] * 1e3,
+        "launch_us": hc_ms["mixed_fly_lane"] * 1e3,
         "traffic_measurement": traffic,
         "gather_issue_roof": {
             "what": "vector-memory load instructions x texture-addresser cycles each / (CUs x clock)",
