@@ -129,7 +129,7 @@ template <int M> struct Passes {
 template <int M, typename R, int IDX, int P>
 __device__ __forceinline__ void run_pass(const RowCtx& ctx, const cplx<R>* __restrict__ c, uint32_t base,
                                          const uint32_t* Wl, int ldw, const R* sql, double* red,
-                                         double2* __restrict__ part) {
+                                         double2* __restrict__ part, uint32_t row) {
     using Z = cplx<R>;
     constexpr PassDesc d = Passes<M>::get(P);
     constexpr int NA = Passes<M>::nacc(P) > 0 ? Passes<M>::nacc(P) : 1;
@@ -220,7 +220,7 @@ __device__ __forceinline__ void run_pass(const RowCtx& ctx, const cplx<R>* __res
     block_sum<2 * NA>(v, red);
     if (threadIdx.x == 0) {
         const int E = L.total();
-        double2* out = part + (size_t) blockIdx.x * E;
+        double2* out = part + (size_t) row * E;
 #pragma unroll
         for (int t = 0; t < NA; t++) {
             int e;
@@ -243,15 +243,34 @@ __device__ __forceinline__ void run_pass(const RowCtx& ctx, const cplx<R>* __res
 template <int M, typename R, int IDX, int... P>
 __device__ __forceinline__ void run_one(std::integer_sequence<int, P...>, int pass, const RowCtx& ctx,
                                         const cplx<R>* __restrict__ c, uint32_t base, const uint32_t* Wl, int ldw,
-                                        const R* sql, double* red, double2* part) {
-    ((P == pass ? run_pass<M, R, IDX, P>(ctx, c, base, Wl, ldw, sql, red, part) : (void) 0), ...);
+                                        const R* sql, double* red, double2* part, uint32_t row) {
+    ((P == pass ? run_pass<M, R, IDX, P>(ctx, c, base, Wl, ldw, sql, red, part, row) : (void) 0), ...);
 }
+
+// Block w of the (padded) grid of 8 R blocks runs (tile, family) pair
+// v = (w mod 8) R + w / 8, tile-major: XCD w mod 8 gets the contiguous range
+// [x R, (x + 1) R), so a tile's families are dispatched one after another on
+// one XCD and read its coefficients and occupation rows through that XCD's
+// L2 (dispatched family after family over the whole space, every family
+// streamed them from memory again: 3.2 GB per FP64 launch at N=100, M=5,
+// 0.35 GB now).  Returns false for the padding blocks.
+__device__ __forceinline__ bool dm_tile_family(uint32_t ntiles, uint32_t nfam, uint32_t& tile, uint32_t& fam) {
+    const uint32_t R = gridDim.x / 8;
+    const uint32_t v = (blockIdx.x % 8) * R + blockIdx.x / 8;
+    if (v >= ntiles * nfam) return false;
+    tile = v / nfam;
+    fam = v % nfam;
+    return true;
+}
+inline unsigned dm_family_grid(unsigned ntiles, unsigned nfam) { return 8 * div_up((uint64_t) ntiles * nfam, 8); }
 
 template <int M, typename R, int IDX>
 __global__ __launch_bounds__(kDmBlock) void k_density(RowCtx ctx, const cplx<R>* __restrict__ c,
                                                       const uint32_t* __restrict__ Wg, int ldw, int use_sq,
-                                                      double2* __restrict__ part) {
+                                                      double2* __restrict__ part, uint32_t ntiles) {
     extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t tile, fam;
+    if (!dm_tile_family(ntiles, Passes<M>::count(), tile, fam)) return;  // before any barrier
     constexpr int kRed = 2 * (2 * M + c2(M) + 8 * 9 + 64);  // >= 2 * max nacc
     double* red = (double*) smem;
     uint32_t* Wl = (uint32_t*) (smem + kRed * (kDmBlock / 64) * sizeof(double));
@@ -263,13 +282,8 @@ __global__ __launch_bounds__(kDmBlock) void k_density(RowCtx ctx, const cplx<R>*
     if (use_sq)
         for (int t = threadIdx.x; t < ctx.npar + 3; t += blockDim.x) sql[t] = sqrt((R) t);
     __syncthreads();
-    const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-    // gridDim.y == 1: every pass in turn over the tile; otherwise pass blockIdx.y only
-    const int p0 = gridDim.y > 1 ? (int) blockIdx.y : 0;
-    const int p1 = gridDim.y > 1 ? p0 + 1 : Passes<M>::count();
-    for (int p = p0; p < p1; p++)
-        run_one<M, R, IDX>(std::make_integer_sequence<int, Passes<M>::count()>{}, p, ctx, c, b * kDmTile, Wl, ldw,
-                           use_sq ? sql : nullptr, red, part);
+    run_one<M, R, IDX>(std::make_integer_sequence<int, Passes<M>::count()>{}, (int) fam, ctx, c, tile * kDmTile, Wl,
+                       ldw, use_sq ? sql : nullptr, red, part, tile);
 }
 
 // ------------------------------------------------ one-pass wave form
@@ -492,19 +506,20 @@ int DmFused<M>::run(const Space* S, const void* c, int prec, int index, double2*
         set_error("density kernel LDS budget exceeded for this (N, M); use table indexing");
         return kInvalidArgument;
     }
-    // One element family per block (blockIdx.y): every block's chain of
-    // dependent loads is one family long, and 14x more blocks (M = 5) are
-    // in flight than with every family in turn in one block.
-    const dim3 grid(nb, Passes<M>::count());
+    // One element family per block: every block's chain of dependent loads
+    // is one family long, and 14x more blocks (M = 5) are in flight than
+    // with every family in turn in one block; a tile's families run
+    // together on one XCD (dm_tile_family).
+    const unsigned grid = dm_family_grid(nb, Passes<M>::count());
     if (prec == kDmMixed)
         k_density<M, float, kIdxFly><<<grid, kDmBlock, smem, st>>>(ctx, (const float2*) c, S->d_W, S->ldw, use_sq,
-                                                                   part);
+                                                                   part, nb);
     else if (fly)
         k_density<M, double, kIdxFly><<<grid, kDmBlock, smem, st>>>(ctx, (const double2*) c, S->d_W, S->ldw,
-                                                                    use_sq, part);
+                                                                    use_sq, part, nb);
     else
         k_density<M, double, kIdxTables><<<grid, kDmBlock, smem, st>>>(ctx, (const double2*) c, S->d_W, S->ldw,
-                                                                       use_sq, part);
+                                                                       use_sq, part, nb);
     MCTDHB_HIP_CHECK(hipGetLastError());
     return kOk;
 }

@@ -224,6 +224,23 @@ def probe_hc(args, space, C0, H, G, stream):
                       "copy_bytes_read": COPY_PROBE_BYTES, "copy_bytes_written": COPY_PROBE_BYTES}), flush=True)
 
 
+def probe_density(args, space, C0, H, G, stream):
+    """rho1 + rho2 three times in FP64, then three times in the mixed form
+    (on-the-fly indices), for counter runs of the density kernels."""
+    import torch
+    import mctdhb_amd as amd
+
+    M = args.norb
+    r1 = torch.empty(M * M, dtype=torch.complex128, device="cuda")
+    r2 = torch.empty(M ** 4, dtype=torch.complex128, device="cuda")
+    c32 = C0.to(torch.complex64)
+    for src, prec in ((C0, amd.FP64), (c32, amd.MIXED)):
+        for _ in range(3):
+            space.density(src, r1, r2, prec, amd.INDEX_FLY, stream=stream)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "density", "launches_per_form": 3}), flush=True)
+
+
 def probe_sil(args, space, C0, H, G, stream):
     """Two mixed SIL steps (their H.C launches carry the Lanczos epilogue),
     then five device copies of COPY_PROBE_BYTES, for counter runs of the
@@ -581,6 +598,8 @@ def main():
     ap.add_argument("--serve-only", action="store_true", help="only the request loop (its JSON object)")
     ap.add_argument("--full-step-only", action="store_true",
                     help="only the whole-step timing (prints its JSON object)")
+    ap.add_argument("--probe-density", action="store_true",
+                    help="rho1 + rho2 three times per form (FP64, mixed) for counter runs")
     ap.add_argument("--probe-sil", action="store_true",
                     help="only two mixed SIL steps and a plain copy kernel of known size, for hardware-counter runs")
     ap.add_argument("--probe-tiling", type=int, default=0,
@@ -634,6 +653,9 @@ def main():
         return
     if args.probe_sil:
         probe_sil(args, space, C0, H, G, stream)
+        return
+    if args.probe_density:
+        probe_density(args, space, C0, H, G, stream)
         return
 
     # ---- H.C kernel alone (coefficient block prebuilt): FP64 with tables and
