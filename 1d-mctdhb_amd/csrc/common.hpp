@@ -88,12 +88,6 @@ template <> __device__ __forceinline__ double2 from_d2<double2>(double2 a) { ret
 template <> __device__ __forceinline__ float2 from_d2<float2>(double2 a) { return make_float2((float) a.x, (float) a.y); }
 
 // ----------------------------------------------------------- wave reduce
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
 // lane i gets v[i - K] within its row of 16 (0 where that leaves the row);
 // every lane of the wave must be active
 template <int K> __device__ __forceinline__ double row_shr(double v) {
@@ -109,6 +103,20 @@ __device__ __forceinline__ double row_sum(double v) {
     v += row_shr<4>(v);
     return v + row_shr<8>(v);
 }
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+// Sum of v over the 64 lanes of the wave, the same value in every lane, in a
+// fixed order: DPP row sums (vector ALU), then the four row totals through
+// readlane (cross-lane shuffles are LDS permutes whose dependent chain cost
+// each sum some 1000 cycles).  Every lane of the wave must be active.
+__device__ __forceinline__ double wave_sum(double v) {
+    v = row_sum(v);
+    return (readlane_d(v, 15) + readlane_d(v, 31)) + (readlane_d(v, 47) + readlane_d(v, 63));
+}
+
 
 // Block-wide sum of NV doubles per thread into lane-0-of-block results.
 // `scratch` must hold NV * (blockDim.x / 64) doubles.  Returns the totals in

@@ -296,16 +296,9 @@ __global__ __launch_bounds__(kDmBlock) void k_density(RowCtx ctx, const cplx<R>*
 // recomputed per element family, and no per-thread accumulator array is
 // needed.  The mixed form forms the products in complex64 and sums in FP64.
 
-__device__ __forceinline__ double readlane_d(double v, int l) {
-    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
-                            __builtin_amdgcn_readlane(__double2loint(v), l));
-}
 // Sum of v over the 64 lanes of the wave, returned in every lane; every lane
 // of the wave must be active.  The order of the additions is fixed.
-__device__ __forceinline__ double wave_total(double v) {
-    v = row_sum(v);  // lane 15 of each row: the row's sum
-    return (readlane_d(v, 15) + readlane_d(v, 31)) + (readlane_d(v, 47) + readlane_d(v, 63));
-}
+__device__ __forceinline__ double wave_total(double v) { return wave_sum(v); }
 
 // Per-wave accumulators, spread over the lanes: element e lives in lane
 // e % 64 of acc[e / 64], so adding a wave total costs a masked add and no
