@@ -259,6 +259,112 @@ __global__ void k_cn_sequential(double2* __restrict__ orb, int M, int grid, int 
     f[m] = f[0];
 }
 
+// One block per orbital: the CN right-hand side, then the tridiagonal solve
+// by parallel cyclic reduction (PCR) in place of the two sequential sweeps:
+// log2(m) steps, each eliminating the couplings to the rows s = 1, 2, 4, ...
+// away, every row at once.  Periodic bounds (a cyclic system of size
+// m = grid - 1) go through Sherman-Morrison like the reference
+// (tridiagonal_solver.c:161-212), with the textbook split gamma = -b_0
+// (Numerical Recipes' cyclic solver) instead of the reference's
+// gamma = b_0, which leaves a zero first pivot that only a sequential sweep
+// can take; both give the solution of the same system.  The matrix is
+// strictly diagonally dominant (|b_i| > |a_i| + |c_i| for the CN operator),
+// where PCR is stable.  Coefficients (a, b, c, two right-hand sides) live in
+// the workspace scratch, double-buffered: 10 m entries per orbital.
+constexpr int kCnPcrBlock = 1024;
+constexpr int kCnPcrMaxRows = 2 * kCnPcrBlock;  // rows per thread <= 2
+__global__ __launch_bounds__(kCnPcrBlock) void k_cn_pcr(double2* __restrict__ orb, int grid, int bounds,
+                                                         const double2* __restrict__ cn, double2 rhs_up,
+                                                         double2 rhs_lo, double2 rhs_mid, double2* __restrict__ scratch) {
+    const int k = blockIdx.x;
+    double2* f = orb + (size_t) k * grid;
+    const double2 *up = cn, *lo = cn + grid, *mid0 = cn + 2 * grid;
+    const int n = grid;
+    const bool cyc = bounds != kZeroBounds;
+    const int m = cyc ? n - 1 : n;
+    double2* S = scratch + (size_t) k * 10 * grid;
+    double2* X[5] = {S, S + m, S + 2 * m, S + 3 * m, S + 4 * m};              // a, b, c, d, e
+    double2* Y[5] = {S + 5 * m, S + 6 * m, S + 7 * m, S + 8 * m, S + 9 * m};  // next step
+    const double2 zero = make_double2(0, 0);
+    // Sherman-Morrison split of the corners: beta = up[m-1] couples row 0 to
+    // x_{m-1}, alpha = lo[m-1] row m-1 to x_0
+    const double2 gam = make_double2(-mid0[0].x, -mid0[0].y);
+    const double2 alpha = cyc ? lo[m - 1] : zero, beta = cyc ? up[m - 1] : zero;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+        double2 r;  // right-hand side (split_linear_orbitals.c:288-333)
+        if (i == 0) {
+            r = cmul(rhs_mid, f[0]) + cmul(rhs_up, f[1]);
+            if (cyc) r = r + cmul(rhs_lo, f[n - 2]);
+        } else if (cyc && i == n - 2) {
+            r = cmul(rhs_mid, f[n - 2]) + cmul(rhs_up, f[0]) + cmul(rhs_lo, f[n - 3]);
+        } else if (!cyc && i == n - 1) {
+            r = cmul(rhs_mid, f[n - 1]) + cmul(rhs_lo, f[n - 2]);
+        } else {
+            r = cmul(rhs_mid, f[i]) + cmul(rhs_up, f[i + 1]) + cmul(rhs_lo, f[i - 1]);
+        }
+        double2 b = mid0[i];
+        if (cyc && i == 0) b = b - gam;
+        if (cyc && i == m - 1) b = b - cdiv(cmul(alpha, beta), gam);
+        X[0][i] = i >= 1 ? lo[i - 1] : zero;
+        X[1][i] = b;
+        X[2][i] = i <= m - 2 ? up[i] : zero;
+        X[3][i] = r;
+        X[4][i] = (cyc && i == 0) ? gam : ((cyc && i == m - 1) ? alpha : zero);
+    }
+    __syncthreads();
+    const int nrhs = cyc ? 2 : 1;
+    for (int s = 1; s < m; s *= 2) {
+        for (int i = threadIdx.x; i < m; i += blockDim.x) {
+            const int im = i - s, ip = i + s;
+            const double2 ai = X[0][i], ci = X[2][i];
+            double2 bn = X[1][i], an = zero, cn2 = zero, dn = X[3][i], en = nrhs > 1 ? X[4][i] : zero;
+            if (im >= 0) {
+                const double2 al = cdiv(make_double2(-ai.x, -ai.y), X[1][im]);
+                an = cmul(al, X[0][im]);
+                bn = bn + cmul(al, X[2][im]);
+                dn = dn + cmul(al, X[3][im]);
+                if (nrhs > 1) en = en + cmul(al, X[4][im]);
+            }
+            if (ip < m) {
+                const double2 ga = cdiv(make_double2(-ci.x, -ci.y), X[1][ip]);
+                cn2 = cmul(ga, X[2][ip]);
+                bn = bn + cmul(ga, X[0][ip]);
+                dn = dn + cmul(ga, X[3][ip]);
+                if (nrhs > 1) en = en + cmul(ga, X[4][ip]);
+            }
+            Y[0][i] = an;
+            Y[1][i] = bn;
+            Y[2][i] = cn2;
+            Y[3][i] = dn;
+            Y[4][i] = en;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 5; t++) {
+            double2* sw = X[t];
+            X[t] = Y[t];
+            Y[t] = sw;
+        }
+    }
+    // every row decoupled: y = d / b, z = e / b
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+        const double2 b = X[1][i];
+        Y[3][i] = cdiv(X[3][i], b);
+        if (cyc) Y[4][i] = cdiv(X[4][i], b);
+    }
+    __syncthreads();
+    double2 fac = zero;
+    if (cyc) {
+        // x = y - (v.y / (1 + v.z)) z with v = (1, 0, ..., beta / gamma)
+        const double2 vm = cdiv(beta, gam);
+        const double2 vy = Y[3][0] + cmul(vm, Y[3][m - 1]);
+        const double2 vz = Y[4][0] + cmul(vm, Y[4][m - 1]);
+        fac = cdiv(vy, make_double2(1.0 + vz.x, vz.y));
+    }
+    for (int i = threadIdx.x; i < m; i += blockDim.x) f[i] = cyc ? Y[3][i] - cmul(fac, Y[4][i]) : Y[3][i];
+    if (cyc && threadIdx.x == 0) f[m] = Y[3][0] - cmul(fac, Y[4][0]);
+}
+
 int linear_cn(OrbWork* w, double2* orb, hipStream_t st) {
     const OrbEq& q = w->eq;
     // dt = prop_dt / 2, rhs coefficients of cn_rhs
@@ -268,8 +374,11 @@ int linear_cn(OrbWork* w, double2* orb, hipStream_t st) {
     const double2 a1dt = cscale(cmul(q.d1coef, dt), 1.0 / dx / 4);
     const double2 up = cscale(a2dt, 0.5) + a1dt, lo = cscale(a2dt, 0.5) - a1dt;
     const double2 mid = make_double2(-a2dt.x, 1.0 - a2dt.y);
-    k_cn_sequential<<<div_up(q.norb, 64), 64, 0, st>>>(orb, q.norb, q.grid_size, q.bounds, w->d_cn, up, lo, mid,
-                                                       w->d_cnwork);
+    if (w->cn_pcr)
+        k_cn_pcr<<<q.norb, kCnPcrBlock, 0, st>>>(orb, q.grid_size, q.bounds, w->d_cn, up, lo, mid, w->d_cnwork);
+    else
+        k_cn_sequential<<<div_up(q.norb, 64), 64, 0, st>>>(orb, q.norb, q.grid_size, q.bounds, w->d_cn, up, lo, mid,
+                                                           w->d_cnwork);
     MCTDHB_HIP_CHECK(hipGetLastError());
     return kOk;
 }
@@ -1513,7 +1622,7 @@ int orbwork_create(const OrbEq& eq, int imagtime, const double* pot_host, OrbWor
     ok = ok && hipMalloc((void**) &w->d_cn, sizeof(double2) * 3 * grid) == hipSuccess;
     ok = ok && hipMalloc((void**) &w->d_stage, sizeof(double2) * 7 * (size_t) M * grid) == hipSuccess;
     ok = ok && hipMalloc((void**) &w->d_red, sizeof(double) * 1024) == hipSuccess;
-    ok = ok && hipMalloc((void**) &w->d_cnwork, sizeof(double2) * 8 * (size_t) M * grid) == hipSuccess;
+    ok = ok && hipMalloc((void**) &w->d_cnwork, sizeof(double2) * 10 * (size_t) M * grid) == hipSuccess;
     ok = ok && hipMalloc((void**) &w->d_nl, sizeof(double2) * ((size_t) M * M * M * (M + 1) / 2 + M * M)) ==
                    hipSuccess;
     ok = ok && hipMalloc((void**) &w->d_fsH, sizeof(double2) * (size_t) M * grid) == hipSuccess;
@@ -1527,6 +1636,13 @@ int orbwork_create(const OrbEq& eq, int imagtime, const double* pot_host, OrbWor
     MCTDHB_HIP_CHECK(hipMemcpy(w->d_fexp, fexp.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
     MCTDHB_HIP_CHECK(hipMemcpy(w->d_tw, tw.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
     MCTDHB_HIP_CHECK(hipMemcpy(w->d_cn, cn.data(), sizeof(double2) * 3 * grid, hipMemcpyHostToDevice));
+    // the parallel CN solve needs a nonzero first diagonal (its cyclic split
+    // divides by it; the reference's reduced system for mid[0] == 0 stays on
+    // the sequential sweep) and at most two rows per thread
+    {
+        const double2 m0 = cn[2 * (size_t) grid];
+        w->cn_pcr = (m0.x != 0 || m0.y != 0) && grid >= 4 && grid <= kCnPcrMaxRows + 1 ? 1 : 0;
+    }
     // spectral symbol of the linear part: i freq d1 - freq^2 d2 (split_linear_orbitals.c:275-278)
     std::vector<double2> sym(n);
     for (int i = 0; i < n; i++) {

@@ -31,7 +31,8 @@ struct OrbWork {
     double2* d_cn = nullptr;     // 3 * grid_size: upper | lower | mid
     double2* d_stage = nullptr;  // 7 * norb * grid_size: up to six stages and y
     double* d_red = nullptr;     // reduction scratch
-    double2* d_cnwork = nullptr; // 8 * norb * grid_size: Crank-Nicolson sweep scratch
+    double2* d_cnwork = nullptr; // 10 * norb * grid_size: Crank-Nicolson solve scratch
+    int cn_pcr = 0;              // the parallel (PCR) CN solve applies
     double2* d_nl = nullptr;     // grid-independent factors of the nonlinear term
     double2* d_fsH = nullptr;    // norb * grid_size: H y of the full-equation step
     double2* d_fsmat = nullptr;  // 3 norb^2: projections, Gram matrix, projector coefficients

@@ -46,11 +46,14 @@ def test_linear_fft_step(orc, amd, grid, imagtime):
     assert gu.rel_err(gu.host(d), ref) < 1e-12
 
 
+# grid 2049: the largest the parallel (PCR) solve takes, 2 or 3 rows per
+# thread; beyond it the sequential sweeps run
+@pytest.mark.parametrize("grid", [129, 257, 2049, 2305])
 @pytest.mark.parametrize("bounds", [1001, 1000])
 @pytest.mark.parametrize("imagtime", [False, True])
-def test_linear_cn_step(orc, amd, bounds, imagtime):
-    print(f"cn bounds={bounds} imag={imagtime}", flush=True)
-    x, eq, w, orb = _setup(orc, amd, 3, 129, imagtime, bounds=bounds, d1coef=0.3 + 0.1j)
+def test_linear_cn_step(orc, amd, bounds, imagtime, grid):
+    print(f"cn bounds={bounds} imag={imagtime} grid={grid}", flush=True)
+    x, eq, w, orb = _setup(orc, amd, 3, grid, imagtime, bounds=bounds, d1coef=0.3 + 0.1j)
     ref = orb.copy()
     orc.lib().orc_advance_linear_cn(eq, ref)
     d = gu.dev(orb)
