@@ -67,6 +67,11 @@ def draw_ground_states(count: int, seed: int = 11, max_dim: int = 200_000) -> li
     return out
 
 
+# Above this many configurations an instance's initial coefficients are drawn
+# on the device (inputs.coefficients_device), below it on the host.
+DEVICE_DRAW_DIM = 1 << 16
+
+
 def footprint_bytes(dim: int, norb: int, grid: int, lanczos: int = 5) -> int:
     """Device memory of one instance, upper estimate: coefficients and the
     Krylov vectors (complex128), the H.C output, occupation rows, the
@@ -103,8 +108,11 @@ class Instance:
         orb = inputs.hermite_gauss(M, x, seed=d.seed, noise=1e-3)
         orb[:, -1] = orb[:, 0]
         dev = "cuda"
-        self.t = {"orb": torch.from_numpy(np.ascontiguousarray(orb.ravel())).to(dev),
-                  "coef": torch.from_numpy(inputs.coefficients(d.dim, d.seed)).to(dev)}
+        # large synthetic states are drawn on the device (a host draw and copy
+        # dominated the admission of a request with millions of configurations)
+        coef = (inputs.coefficients_device(d.dim, d.seed, dev) if d.dim > DEVICE_DRAW_DIM else
+                torch.from_numpy(inputs.coefficients(d.dim, d.seed)).to(dev))
+        self.t = {"orb": torch.from_numpy(np.ascontiguousarray(orb.ravel())).to(dev), "coef": coef}
         for k, n in (("hob", M * M), ("hint", M ** 4), ("r1", M * M), ("r2", M ** 4), ("rinv", M * M)):
             self.t[k] = torch.zeros(n, dtype=torch.complex128, device=dev)
         t = self.t
