@@ -289,7 +289,8 @@ int    mctdhb_apply_hamiltonian_cf_dev(const mctdhb_space* space, const void* co
  * lane, 1 two configurations of one run per lane (one 16-byte gather serves
  * both), 2 one configuration per lane with the coefficients each block
  * reaches staged in LDS, one window per far group of jump types
- * (2 <= norb <= 5, N up to about 330).  By default the possible ones are
+ * (2 <= norb <= 5, N <= 185; the plain product only, the SIL step's
+ * Lanczos forms keep tiling 0 under it).  By default the possible ones are
  * timed at the first mixed call on a space of a new (N, M) and the fastest
  * kept for every space of that shape in the process (so a solve runs the
  * same kernels alone and among others).
