@@ -5,6 +5,7 @@
 // src/integrator/coefficients_integration.c, src/assistant/types_definition.c)
 // on host arrays, moving the work to the GPU.  Layer 2 (mctdhb_*) is the
 // device-resident, status-returning API the drop-in layer is built on.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -20,6 +21,7 @@
 #include "monitor.hpp"
 #include "mctdhb_hip.h"
 #include "orbital.hpp"
+#include "pack.hpp"
 #include "sil_api.hpp"
 
 namespace mctdhb {
@@ -672,6 +674,274 @@ extern "C" int mctdhb_step_graph_destroy(mctdhb_step_graph* g) {
     hipGraphExecDestroy(g->exec);
     hipGraphDestroy(g->graph);
     delete g;
+    return kOk;
+}
+
+// ------------------------------------------------------- packed steps
+// Many instances' whole steps (driver.c:159-202 per instance) as one launch
+// per stage (pack.hpp).  Instances are grouped by the stages' template
+// parameters; each group's step is captured once as a graph of its stage
+// launches over its argument tables.
+
+struct PackGroup {
+    int M = 0, ns = 0, imagtime = 0, index = 0, fold = 0;
+    bool stage = false;
+    int count = 0;
+    void* d_tab = nullptr;  // every stage table, one allocation
+    size_t fft_smem = 0, gs_smem = 0, sil_smem = 0, dm_smem = 0;
+    unsigned fft_n = 0, nl_blocks = 0, dm_grid = 0;
+    mctdhb_step_graph graph;
+};
+
+struct mctdhb_pack {
+    std::vector<PackGroup> groups;
+};
+
+static int pack_rk_stages(int rk_order) { return rk_order == 2 ? 2 : (rk_order == 5 ? 6 : 4); }
+
+// Why the instance's step cannot run packed, or nullptr.
+static const char* pack_refusal(const mctdhb_space* sp, const mctdhb_sil* s, const mctdhb_orbital* o,
+                                int use_fft) {
+    const Space* S = sp->S;
+    const Sil* L = s->L;
+    const OrbWork* w = o->w;
+    if (s->sp != sp) return "the SIL workspace belongs to another space";
+    if (w->fullstep) return "packed steps take the split-step orbital integrator";
+    if (!use_fft || w->log2n < 0) return "packed steps take the FFT linear step of a power-of-two length";
+    if (S->norb < 2 || S->norb > 7 || w->eq.norb != S->norb) return "packed steps cover 2 <= norb <= 7";
+    if (L->prec != kPrecFP64 || L->generic || !L->small || !sil_small_fits(S, L->lm))
+        return "packed steps take the one-workgroup FP64 SIL step";
+    const DmPlan plan = dm_plan(S, kDmFP64, kIndexOnTheFly);
+    if (plan.wave || plan.nb > kDmFinishBlocks || !dm_family_smem(S, sizeof(double)))
+        return "packed steps take the one-block density finish";
+    return nullptr;
+}
+
+extern "C" int mctdhb_pack_eligible(const mctdhb_space* sp, const mctdhb_sil* s, const mctdhb_orbital* o,
+                                    int use_fft, int* eligible) {
+    if (!sp || !s || !o || !eligible) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    const char* why = pack_refusal(sp, s, o, use_fft);
+    *eligible = why ? 0 : 1;
+    if (why) set_error(why);
+    return kOk;
+}
+
+extern "C" int mctdhb_pack_destroy(mctdhb_pack* p) {
+    if (!p) return kOk;
+    hipDeviceSynchronize();
+    for (PackGroup& g : p->groups) {
+        if (g.graph.exec) hipGraphExecDestroy(g.graph.exec);
+        if (g.graph.graph) hipGraphDestroy(g.graph.graph);
+        hipFree(g.d_tab);
+    }
+    delete p;
+    return kOk;
+}
+
+// The stage launches of one group's step, in the order of
+// mctdhb_propagate_step_dev with the FFT linear step: linear half, h and g,
+// nonlinear RK sub-step (its tables first where the kernel does not form
+// them), linear half, Gram-Schmidt (imaginary time), h and g, coefficient
+// block, SIL step, density matrices with the synchronisation.
+static int pack_enqueue(const PackGroup& g, PkFft* fft, PkOm* om, PkNlCoef* nlc, PkNl* nl, PkGs* gs, PkCf* cf,
+                        PkSil* sil, PkDm* dm, PkDmFin* fin, hipStream_t st) {
+    int rc = pk_linear_fft(fft, g.count, g.M, g.fft_smem, st);
+    if (!rc) rc = pk_orbital_matrices(om, g.count, g.M, st);
+    if (!rc && !g.fold) rc = pk_nl_coef(nlc, g.count, g.M, st);
+    if (!rc) rc = pk_nl_rk(nl, g.count, g.M, g.ns, g.nl_blocks, st);
+    if (!rc) rc = pk_linear_fft(fft, g.count, g.M, g.fft_smem, st);
+    if (!rc && g.imagtime) rc = pk_gram_schmidt(gs, g.count, g.M, g.gs_smem, st);
+    if (!rc) rc = pk_orbital_matrices(om, g.count, g.M, st);
+    if (!rc) rc = pk_hc_coefficients(cf, g.count, g.M, st);
+    if (!rc) rc = pk_sil_small(sil, g.count, g.M, g.index, g.stage, g.sil_smem, st);
+    if (!rc) rc = pk_density(dm, g.count, g.M, g.dm_grid, g.dm_smem, st);
+    if (!rc) rc = pk_dm_finish(fin, g.count, g.M, sizeof(double2) * 4 * g.M * g.M, st);
+    return rc;
+}
+
+extern "C" int mctdhb_pack_create(int n, mctdhb_space* const* spaces, mctdhb_sil* const* sils,
+                                  mctdhb_orbital* const* orbitals, int use_fft, void* const* orb, void* const* coef,
+                                  void* const* hob, void* const* hint, void* const* rho1, void* const* rho2,
+                                  void* const* rinv, mctdhb_pack** out) {
+    if (!out || n < 1 || !spaces || !sils || !orbitals || !orb || !coef || !hob || !hint || !rho1 || !rho2 ||
+        !rinv) {
+        set_error("invalid argument");
+        return kInvalidArgument;
+    }
+    *out = nullptr;
+    for (int i = 0; i < n; i++) {
+        if (!spaces[i] || !sils[i] || !orbitals[i] || !orb[i] || !coef[i] || !hob[i] || !hint[i] || !rho1[i] ||
+            !rho2[i] || !rinv[i]) {
+            set_error("null argument");
+            return kInvalidArgument;
+        }
+        if (const char* why = pack_refusal(spaces[i], sils[i], orbitals[i], use_fft)) {
+            set_error(std::string("instance ") + std::to_string(i) + ": " + why);
+            return kInvalidArgument;
+        }
+    }
+    // groups by the stages' template parameters, members in input order
+    std::vector<std::vector<int>> members;
+    std::vector<PackGroup> groups;
+    for (int i = 0; i < n; i++) {
+        const Space* S = spaces[i]->S;
+        const Sil* L = sils[i]->L;
+        const OrbWork* w = orbitals[i]->w;
+        PackGroup key;
+        key.M = S->norb;
+        key.ns = pack_rk_stages(w->rk_order);
+        key.imagtime = w->imagtime ? 1 : 0;
+        key.index = L->index;
+        key.stage = sil_small_stage(S);
+        key.fold = pk_nl_folds(key.M, key.ns) ? 1 : 0;
+        size_t gi = 0;
+        for (; gi < groups.size(); gi++) {
+            const PackGroup& g = groups[gi];
+            if (g.M == key.M && g.ns == key.ns && g.imagtime == key.imagtime && g.index == key.index &&
+                g.stage == key.stage)
+                break;
+        }
+        if (gi == groups.size()) {
+            groups.push_back(key);
+            members.emplace_back();
+        }
+        members[gi].push_back(i);
+    }
+    mctdhb_pack* p = new mctdhb_pack();
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+        PackGroup g = groups[gi];
+        const std::vector<int>& mem = members[gi];
+        g.count = (int) mem.size();
+        std::vector<PkFft> fft(g.count);
+        std::vector<PkOm> om(g.count);
+        std::vector<PkNlCoef> nlc(g.count);
+        std::vector<PkNl> nl(g.count);
+        std::vector<PkGs> gs(g.count);
+        std::vector<PkCf> cf(g.count);
+        std::vector<PkSil> sil(g.count);
+        std::vector<PkDm> dm(g.count);
+        std::vector<PkDmFin> fin(g.count);
+        for (int j = 0; j < g.count; j++) {
+            const int i = mem[j];
+            mctdhb_space* sp = spaces[i];
+            const Space* S = sp->S;
+            const Sil* L = sils[i]->L;
+            const OrbWork* w = orbitals[i]->w;
+            const OrbEq& q = w->eq;
+            if (!sp->dm_work && hipMalloc(&sp->dm_work, dm_workspace_bytes(S)) != hipSuccess) {
+                set_error("hipMalloc of the density workspace failed");
+                mctdhb_pack_destroy(p);
+                return kOutOfMemory;
+            }
+            double2* o = (double2*) orb[i];
+            double2* h1 = (double2*) hob[i];
+            double2* h2 = (double2*) hint[i];
+            double2* c = (double2*) coef[i];
+            fft[j] = PkFft{o, w->d_tw, w->d_fexp, q.grid_size, w->nfft, w->log2n};
+            g.fft_smem = std::max(g.fft_smem, sizeof(double2) * w->nfft);
+            om[j] = PkOm{o, w->d_pot, h1, h2, q.dx, q.d2coef, q.g, q.d1coef, q.grid_size};
+            nlc[j] = PkNlCoef{h1, h2, (const double2*) rho2[i], (const double2*) rinv[i], w->d_nl};
+            nl[j] = PkNl{o, w->d_pot, w->d_nl, h1, h2, (const double2*) rho2[i], (const double2*) rinv[i],
+                         q.tstep, q.g, q.time_fac, q.grid_size};
+            g.nl_blocks = std::max(g.nl_blocks, div_up((unsigned) q.grid_size, 64u));
+            const size_t gsm = pk_gs_smem(q.norb, q.grid_size);
+            gs[j] = PkGs{o, q.dx, q.grid_size, q.bounds == kPeriodicBounds ? 1 : 0, gsm ? 1 : 0};
+            g.gs_smem = std::max(g.gs_smem, gsm);
+            cf[j] = PkCf{h1, h2, L->cf64, S->npar};
+            SmallSilArgs a{};
+            a.st = L->d_st;
+            a.w[0] = c;
+            for (int k = 1; k < L->lm; k++) a.w[k] = L->w[k];
+            a.out = L->out;
+            a.lm = L->lm;
+            a.dt_re = q.prop_dt.x;
+            a.dt_im = q.prop_dt.y;
+            a.imagtime = w->imagtime;
+            const RowCtx ctx = make_row_ctx(S->dim, S->npar, S->d_ht, S->d_map, S->d_mapot, S->d_maptt,
+                                            S->d_strideot, S->d_stridett, S->d_occ8);
+            sil[j] = PkSil{ctx, L->cf64, S->d_W, S->ldw, a};
+            g.sil_smem = std::max(g.sil_smem, sil_small_smem(S) + (g.stage ? sil_small_stage_bytes(S) : 0));
+            const DmPlan plan = dm_plan(S, kDmFP64, kIndexOnTheFly);
+            const unsigned grid = pk_density_grid(S->norb, plan.nb);
+            double2* part = (double2*) sp->dm_work;
+            dm[j] = PkDm{ctx, c, S->d_W, part, S->ldw, S->npar + 3 <= 2048 ? 1 : 0, plan.nb, grid};
+            g.dm_grid = std::max(g.dm_grid, grid);
+            g.dm_smem = std::max(g.dm_smem, dm_family_smem(S, sizeof(double)));
+            const int E = dm_num_elements(S->norb);
+            fin[j] = PkDmFin{part, part + (size_t) plan.nb * E, (double2*) rho1[i], (double2*) rho2[i],
+                             DmSync{(double2*) rinv[i], nullptr, g_reg_factor}, (int) plan.nb};
+        }
+        // one allocation, each table at a 256-byte boundary
+        size_t off = 0;
+        auto place = [&](size_t bytes) {
+            const size_t at = off;
+            off += (bytes + 255) & ~(size_t) 255;
+            return at;
+        };
+        const size_t o_fft = place(sizeof(PkFft) * g.count), o_om = place(sizeof(PkOm) * g.count);
+        const size_t o_nlc = place(sizeof(PkNlCoef) * g.count), o_nl = place(sizeof(PkNl) * g.count);
+        const size_t o_gs = place(sizeof(PkGs) * g.count), o_cf = place(sizeof(PkCf) * g.count);
+        const size_t o_sil = place(sizeof(PkSil) * g.count), o_dm = place(sizeof(PkDm) * g.count);
+        const size_t o_fin = place(sizeof(PkDmFin) * g.count);
+        std::vector<unsigned char> host(off);
+        std::memcpy(host.data() + o_fft, fft.data(), sizeof(PkFft) * g.count);
+        std::memcpy(host.data() + o_om, om.data(), sizeof(PkOm) * g.count);
+        std::memcpy(host.data() + o_nlc, nlc.data(), sizeof(PkNlCoef) * g.count);
+        std::memcpy(host.data() + o_nl, nl.data(), sizeof(PkNl) * g.count);
+        std::memcpy(host.data() + o_gs, gs.data(), sizeof(PkGs) * g.count);
+        std::memcpy(host.data() + o_cf, cf.data(), sizeof(PkCf) * g.count);
+        std::memcpy(host.data() + o_sil, sil.data(), sizeof(PkSil) * g.count);
+        std::memcpy(host.data() + o_dm, dm.data(), sizeof(PkDm) * g.count);
+        std::memcpy(host.data() + o_fin, fin.data(), sizeof(PkDmFin) * g.count);
+        if (hipMalloc(&g.d_tab, off) != hipSuccess) {
+            set_error("hipMalloc of the pack tables failed");
+            mctdhb_pack_destroy(p);
+            return kOutOfMemory;
+        }
+        p->groups.push_back(g);  // owned by the pack from here (freed by destroy on failure)
+        PackGroup& pg = p->groups.back();
+        if (hipMemcpy(pg.d_tab, host.data(), off, hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("copy of the pack tables failed");
+            mctdhb_pack_destroy(p);
+            return kDeviceError;
+        }
+        unsigned char* base = (unsigned char*) pg.d_tab;
+        const int rc = capture_graph(
+            [&](hipStream_t cs) {
+                return pack_enqueue(pg, (PkFft*) (base + o_fft), (PkOm*) (base + o_om), (PkNlCoef*) (base + o_nlc),
+                                    (PkNl*) (base + o_nl), (PkGs*) (base + o_gs), (PkCf*) (base + o_cf),
+                                    (PkSil*) (base + o_sil), (PkDm*) (base + o_dm), (PkDmFin*) (base + o_fin), cs);
+            },
+            &pg.graph);
+        if (rc) {
+            mctdhb_pack_destroy(p);
+            return rc;
+        }
+    }
+    *out = p;
+    return kOk;
+}
+
+extern "C" int mctdhb_pack_groups(const mctdhb_pack* p, int* groups) {
+    if (!p || !groups) {
+        set_error("null argument");
+        return kInvalidArgument;
+    }
+    *groups = (int) p->groups.size();
+    return kOk;
+}
+
+extern "C" int mctdhb_pack_step(mctdhb_pack* p, void* const* streams, int nstreams, int rounds) {
+    if (!p || rounds < 0 || !streams || nstreams < 1) {
+        set_error("invalid argument");
+        return kInvalidArgument;
+    }
+    for (int r = 0; r < rounds; r++)
+        for (size_t gi = 0; gi < p->groups.size(); gi++)
+            MCTDHB_HIP_CHECK(hipGraphLaunch(p->groups[gi].graph.exec, (hipStream_t) streams[gi % nstreams]));
     return kOk;
 }
 

@@ -30,6 +30,7 @@
 
 #include "density_impl.hpp"
 #include "mxm_wave.hpp"
+#include "pack.hpp"
 
 namespace mctdhb {
 
@@ -249,10 +250,9 @@ __global__ __launch_bounds__(256) void k_dm_assemble(int M, const double2* __res
 // With fill set (few configuration blocks): the second-stage sums (each
 // element by one thread, over the blocks in order) and the fill; then, with
 // sy.rinv set, the density synchronisation of rho1, all in one block.
-__global__ __launch_bounds__(256) void k_dm_finish(int M, const double2* __restrict__ part, int nblocks, int E,
-                                                   double2* __restrict__ tot, double2* __restrict__ rho1,
-                                                   double2* __restrict__ rho2, DmSync sy, int fill) {
-    extern __shared__ double2 work[];  // 4 M^2 for the synchronisation
+__device__ __forceinline__ void dm_finish_block(double2* work, int M, const double2* __restrict__ part, int nblocks,
+                                                int E, double2* __restrict__ tot, double2* __restrict__ rho1,
+                                                double2* __restrict__ rho2, const DmSync& sy, int fill) {
     if (fill) {
         for (int e = threadIdx.x; e < E; e += blockDim.x) {
             double2 v = make_double2(0, 0);
@@ -266,6 +266,60 @@ __global__ __launch_bounds__(256) void k_dm_finish(int M, const double2* __restr
         __syncthreads();
         density_sync_body(M, rho1, sy.rinv, sy.eig, sy.reg, work);
     }
+}
+
+__global__ __launch_bounds__(256) void k_dm_finish(int M, const double2* __restrict__ part, int nblocks, int E,
+                                                   double2* __restrict__ tot, double2* __restrict__ rho1,
+                                                   double2* __restrict__ rho2, DmSync sy, int fill) {
+    extern __shared__ double2 work[];  // 4 M^2 for the synchronisation
+    dm_finish_block(work, M, part, nblocks, E, tot, rho1, rho2, sy, fill);
+}
+
+// Packed form: one block per instance, the fill and (with sy.rinv) the
+// synchronisation.
+__global__ __launch_bounds__(256) void k_dm_finish_pk(const PkDmFin* __restrict__ tab, int M, int E) {
+    extern __shared__ double2 work[];
+    const PkDmFin& a = tab[blockIdx.x];
+    dm_finish_block(work, M, a.part, a.nblocks, E, a.tot, a.rho1, a.rho2, a.sy, 1);
+}
+
+int pk_dm_finish(const PkDmFin* tab, int count, int M, size_t smem, hipStream_t st) {
+    k_dm_finish_pk<<<count, 256, smem, st>>>(tab, M, DmLayout{M}.total());
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
+int pk_density(const PkDm* tab, int count, int M, unsigned max_grid, size_t smem, hipStream_t st) {
+    switch (M) {
+        case 2: return DmFused<2>::run_pk(tab, count, max_grid, smem, st);
+        case 3: return DmFused<3>::run_pk(tab, count, max_grid, smem, st);
+        case 4: return DmFused<4>::run_pk(tab, count, max_grid, smem, st);
+        case 5: return DmFused<5>::run_pk(tab, count, max_grid, smem, st);
+        case 6: return DmFused<6>::run_pk(tab, count, max_grid, smem, st);
+        case 7: return DmFused<7>::run_pk(tab, count, max_grid, smem, st);
+        default: set_error("packed density kernels cover 2 <= norb <= 7"); return kInvalidArgument;
+    }
+}
+
+unsigned pk_density_grid(int M, unsigned ntiles) {
+    switch (M) {
+        case 2: return dm_family_grid(ntiles, Passes<2>::count());
+        case 3: return dm_family_grid(ntiles, Passes<3>::count());
+        case 4: return dm_family_grid(ntiles, Passes<4>::count());
+        case 5: return dm_family_grid(ntiles, Passes<5>::count());
+        case 6: return dm_family_grid(ntiles, Passes<6>::count());
+        case 7: return dm_family_grid(ntiles, Passes<7>::count());
+        default: return 0;
+    }
+}
+
+size_t dm_family_smem(const Space* S, size_t rsz) {
+    const int M = S->norb;
+    // block sums (as k_density lays them out), the W table, the sqrt table
+    size_t smem = (size_t) 2 * (2 * M + c2(M) + 8 * 9 + 64) * (kDmBlock / 64) * sizeof(double);
+    smem += ((size_t) (M - 1) * S->ldw * 4 + 15) & ~(size_t) 15;
+    if (S->npar + 3 <= 2048) smem += rsz * (S->npar + 3);
+    return smem > 64 * 1024 ? 0 : smem;
 }
 
 // instantiated once each in dm_m<M>.hip

@@ -32,11 +32,18 @@ constexpr uint32_t kDmWaveMinDim = 1u << 16;
 constexpr int kDmWaveMinOrb = 6;
 
 // Per-orbital-count fused kernels (dm_m<M>.hip): write per-block partials.
+struct PkDm;
 template <int M> struct DmFused {
     static bool wave_fits(const Space* S, int prec, int index);
     static int run(const Space* S, const void* c, int prec, int index, double2* part, const DmPlan& plan,
                    hipStream_t st);
+    // FP64, on-the-fly indices, element-family form, count instances
+    // (pack.hpp); max_grid: the largest per-instance family grid
+    static int run_pk(const PkDm* tab, int count, unsigned max_grid, size_t smem, hipStream_t st);
 };
+// LDS of the element-family kernel (FP64 or mixed by rsz), or 0 where it
+// exceeds the 64 KiB budget.
+size_t dm_family_smem(const Space* S, size_t rsz);
 
 // With sync->rinv set, also the synchronisation of rho1 (eigenvalues into
 // sync->eig when set, regularisation by sync->reg, rinv = rho1^-1), as the

@@ -6,6 +6,7 @@
 #include <utility>
 
 #include "density.hpp"
+#include "pack.hpp"
 #include "rowctx.hpp"
 
 namespace mctdhb {
@@ -254,9 +255,10 @@ __device__ __forceinline__ void run_one(std::integer_sequence<int, P...>, int pa
 // L2 (dispatched family after family over the whole space, every family
 // streamed them from memory again: 3.2 GB per FP64 launch at N=100, M=5,
 // 0.35 GB now).  Returns false for the padding blocks.
-__device__ __forceinline__ bool dm_tile_family(uint32_t ntiles, uint32_t nfam, uint32_t& tile, uint32_t& fam) {
-    const uint32_t R = gridDim.x / 8;
-    const uint32_t v = (blockIdx.x % 8) * R + blockIdx.x / 8;
+__device__ __forceinline__ bool dm_tile_family(uint32_t bid, uint32_t grid, uint32_t ntiles, uint32_t nfam,
+                                               uint32_t& tile, uint32_t& fam) {
+    const uint32_t R = grid / 8;
+    const uint32_t v = (bid % 8) * R + bid / 8;
     if (v >= ntiles * nfam) return false;
     tile = v / nfam;
     fam = v % nfam;
@@ -265,12 +267,11 @@ __device__ __forceinline__ bool dm_tile_family(uint32_t ntiles, uint32_t nfam, u
 inline unsigned dm_family_grid(unsigned ntiles, unsigned nfam) { return 8 * div_up((uint64_t) ntiles * nfam, 8); }
 
 template <int M, typename R, int IDX>
-__global__ __launch_bounds__(kDmBlock) void k_density(RowCtx ctx, const cplx<R>* __restrict__ c,
-                                                      const uint32_t* __restrict__ Wg, int ldw, int use_sq,
-                                                      double2* __restrict__ part, uint32_t ntiles) {
-    extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ void density_block(unsigned char* smem, uint32_t bid, uint32_t grid, const RowCtx& ctx,
+                                              const cplx<R>* __restrict__ c, const uint32_t* __restrict__ Wg,
+                                              int ldw, int use_sq, double2* __restrict__ part, uint32_t ntiles) {
     uint32_t tile, fam;
-    if (!dm_tile_family(ntiles, Passes<M>::count(), tile, fam)) return;  // before any barrier
+    if (!dm_tile_family(bid, grid, ntiles, Passes<M>::count(), tile, fam)) return;  // before any barrier
     constexpr int kRed = 2 * (2 * M + c2(M) + 8 * 9 + 64);  // >= 2 * max nacc
     double* red = (double*) smem;
     uint32_t* Wl = (uint32_t*) (smem + kRed * (kDmBlock / 64) * sizeof(double));
@@ -284,6 +285,25 @@ __global__ __launch_bounds__(kDmBlock) void k_density(RowCtx ctx, const cplx<R>*
     __syncthreads();
     run_one<M, R, IDX>(std::make_integer_sequence<int, Passes<M>::count()>{}, (int) fam, ctx, c, tile * kDmTile, Wl,
                        ldw, use_sq ? sql : nullptr, red, part, tile);
+}
+
+template <int M, typename R, int IDX>
+__global__ __launch_bounds__(kDmBlock) void k_density(RowCtx ctx, const cplx<R>* __restrict__ c,
+                                                      const uint32_t* __restrict__ Wg, int ldw, int use_sq,
+                                                      double2* __restrict__ part, uint32_t ntiles) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    density_block<M, R, IDX>(smem, blockIdx.x, gridDim.x, ctx, c, Wg, ldw, use_sq, part, ntiles);
+}
+
+// Packed form: instance blockIdx.y over its own family grid; blocks past it
+// return.
+template <int M>
+__global__ __launch_bounds__(kDmBlock) void k_density_pk(const PkDm* __restrict__ tab) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const PkDm& p = tab[blockIdx.y];
+    if (blockIdx.x >= p.grid) return;
+    density_block<M, double, kIdxFly>(smem, blockIdx.x, p.grid, p.ctx, (const double2*) p.c, p.W, p.ldw, p.use_sq,
+                                      p.part, p.ntiles);
 }
 
 // ------------------------------------------------ one-pass wave form
@@ -452,10 +472,6 @@ __global__ __launch_bounds__(kDmBlock) void k_density_wave(RowCtx ctx, const cpl
     }
 }
 
-template <int M> static constexpr size_t dm_red_bytes() {
-    return (size_t) 2 * (2 * M + c2(M) + 8 * 9 + 64) * (kDmBlock / 64) * sizeof(double);
-}
-
 template <int M> static size_t dm_wave_smem(const Space* S, size_t rsz) {
     return sizeof(double2) * (kDmBlock / 64) * DmLayout{M}.total() +
            (((size_t) (M - 1) * S->ldw * 4 + 15) & ~(size_t) 15) + rsz * (S->npar + 3);
@@ -465,6 +481,13 @@ template <int M>
 bool DmFused<M>::wave_fits(const Space* S, int prec, int index) {
     const size_t rsz = prec == kDmMixed ? sizeof(float) : sizeof(double);
     return M >= kDmWaveMinOrb && dm_wave_smem<M>(S, rsz) <= 64 * 1024;
+}
+
+template <int M>
+int DmFused<M>::run_pk(const PkDm* tab, int count, unsigned max_grid, size_t smem, hipStream_t st) {
+    k_density_pk<M><<<dim3(max_grid, count), kDmBlock, smem, st>>>(tab);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
 }
 
 template <int M>
@@ -492,10 +515,8 @@ int DmFused<M>::run(const Space* S, const void* c, int prec, int index, double2*
     }
     const unsigned nb = plan.nb;
     const bool use_sq = S->npar + 3 <= 2048;
-    size_t smem = dm_red_bytes<M>();
-    smem += ((size_t) (M - 1) * S->ldw * 4 + 15) & ~(size_t) 15;  // W region, reserved in both modes
-    if (use_sq) smem += rsz * (S->npar + 3);
-    if (smem > 64 * 1024) {
+    const size_t smem = dm_family_smem(S, rsz);
+    if (!smem) {
         set_error("density kernel LDS budget exceeded for this (N, M); use table indexing");
         return kInvalidArgument;
     }

@@ -9,6 +9,7 @@
 // rule grouping.
 #include "hamiltonian_impl.hpp"
 #include "hamiltonian_window.hpp"
+#include "pack.hpp"
 #include "rowctx.hpp"
 
 #include <map>
@@ -124,8 +125,9 @@ unsigned hc_lanczos_blocks(const Space* S, int prec) {
 // With sum_m n_m = N the last occupation is folded into the single-jump
 // coefficients: A'_kl = A_kl + N B_kl,M-1 and B'_klm = B_klm - B_kl,M-1, so
 // F_kl(n) = A'_kl + sum_{m < M-1} n_m B'_klm.
-__global__ void k_hc_coefficients(int M, int N, const double2* __restrict__ hob, const double2* __restrict__ hint,
-                                  double2* __restrict__ cf64, float2* __restrict__ cf32) {
+__device__ __forceinline__ void hc_coefficients_block(int M, int N, const double2* __restrict__ hob,
+                                                      const double2* __restrict__ hint, double2* __restrict__ cf64,
+                                                      float2* __restrict__ cf32) {
     const int t = threadIdx.x;
     const int M2 = M * M, M3 = M2 * M;
     auto G = [&](int a, int b, int c, int d) { return hint[a + b * M + c * M2 + d * M3]; };
@@ -176,6 +178,23 @@ __global__ void k_hc_coefficients(int M, int N, const double2* __restrict__ hob,
         }
         put(idx, v);
     }
+}
+
+__global__ void k_hc_coefficients(int M, int N, const double2* __restrict__ hob, const double2* __restrict__ hint,
+                                  double2* __restrict__ cf64, float2* __restrict__ cf32) {
+    hc_coefficients_block(M, N, hob, hint, cf64, cf32);
+}
+
+// Packed form (FP64 block only): one block per instance.
+__global__ void k_hc_coefficients_pk(const PkCf* __restrict__ tab, int M) {
+    const PkCf a = tab[blockIdx.x];
+    hc_coefficients_block(M, a.npar, a.hob, a.hint, a.cf64, nullptr);
+}
+
+int pk_hc_coefficients(const PkCf* tab, int count, int M, hipStream_t st) {
+    k_hc_coefficients_pk<<<count, 256, 0, st>>>(tab, M);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
 }
 
 int hc_coefficients(const Space* S, const double2* hob, const double2* hint, double2* cf64, float2* cf32,
@@ -545,6 +564,18 @@ int hc_apply_cf(const Space* S, const void* coef, const void* cf, void* hc, int 
     if (prec == kPrecMixed && (rc = hc_tune_tiling(S, st))) return rc;
     const void* blk = prec == kPrecFP64 ? cf : (const void*) ((const double2*) cf + cf_size(M));
     return dispatch_fused_plain(S, coef, blk, hc, prec, index, st);
+}
+
+int pk_sil_small(const PkSil* tab, int count, int M, int index, bool stage, size_t smem, hipStream_t st) {
+    switch (M) {
+        case 2: return HcFused<2>::sil_small_pk(tab, count, index, stage, smem, st);
+        case 3: return HcFused<3>::sil_small_pk(tab, count, index, stage, smem, st);
+        case 4: return HcFused<4>::sil_small_pk(tab, count, index, stage, smem, st);
+        case 5: return HcFused<5>::sil_small_pk(tab, count, index, stage, smem, st);
+        case 6: return HcFused<6>::sil_small_pk(tab, count, index, stage, smem, st);
+        case 7: return HcFused<7>::sil_small_pk(tab, count, index, stage, smem, st);
+        default: set_error("the one-workgroup SIL step covers 2 <= norb <= 7"); return kInvalidArgument;
+    }
 }
 
 int hc_sil_small(const Space* S, const double2* cf64, const SmallSilArgs& a, int index, hipStream_t st) {

@@ -176,6 +176,8 @@ struct SmallSilArgs {
 };
 int hc_sil_small(const Space* S, const double2* cf64, const SmallSilArgs& a, int index, hipStream_t st);
 
+struct PkSil;
+
 // Per-orbital-count entry points of the fused kernel (hc_m<M>.hip).  The
 // mixed form always uses on-the-fly indices.
 template <int M> struct HcFused {
@@ -184,6 +186,8 @@ template <int M> struct HcFused {
     static int plain_tiled(const Space* S, const void* c, const void* cf, void* out, int tiling, hipStream_t st);
     static int lanczos(const Space* S, const void* cf, const LanczosHcArgs& a, int prec, int index, hipStream_t st);
     static int sil_small(const Space* S, const void* cf, const SmallSilArgs& a, int index, hipStream_t st);
+    // the same step for count instances, one workgroup each (pack.hpp)
+    static int sil_small_pk(const PkSil* tab, int count, int index, bool stage, size_t smem, hipStream_t st);
 };
 
 }  // namespace mctdhb

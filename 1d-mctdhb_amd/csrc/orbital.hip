@@ -15,6 +15,7 @@
 
 #include "mxm_wave.hpp"
 #include "orbital.hpp"
+#include "pack.hpp"
 
 namespace mctdhb {
 
@@ -82,11 +83,10 @@ __device__ void fft_inplace(double2* X, int n, int log2n, const double2* __restr
 // One block per orbital: forward DFT, multiply by exp(-i dt/2 T(k)), inverse
 // DFT (both scaled 1/sqrt(n) as the reference's DFTI descriptor), last point
 // = first (split_linear_orbitals.c:375-402).
-__global__ __launch_bounds__(256) void k_fft_linear(double2* __restrict__ orb, int grid, int n, int log2n,
-                                                    const double2* __restrict__ tw,
-                                                    const double2* __restrict__ fexp) {
-    extern __shared__ double2 X[];
-    double2* o = orb + (size_t) blockIdx.x * grid;
+__device__ __forceinline__ void fft_linear_block(double2* X, double2* __restrict__ orb, int grid, int n, int log2n,
+                                                 const double2* __restrict__ tw, const double2* __restrict__ fexp,
+                                                 int k) {
+    double2* o = orb + (size_t) k * grid;
     for (int i = threadIdx.x; i < n; i += blockDim.x) X[bitrev(i, log2n)] = o[i];
     __syncthreads();
     fft_inplace(X, n, log2n, tw, false);
@@ -106,6 +106,26 @@ __global__ __launch_bounds__(256) void k_fft_linear(double2* __restrict__ orb, i
     for (int i = threadIdx.x; i < n; i += blockDim.x) o[i] = cscale(X[i], s);
     __syncthreads();
     if (threadIdx.x == 0) o[n] = cscale(X[0], s);
+}
+
+__global__ __launch_bounds__(256) void k_fft_linear(double2* __restrict__ orb, int grid, int n, int log2n,
+                                                    const double2* __restrict__ tw,
+                                                    const double2* __restrict__ fexp) {
+    extern __shared__ double2 X[];
+    fft_linear_block(X, orb, grid, n, log2n, tw, fexp, blockIdx.x);
+}
+
+// Packed form: instance blockIdx.y, orbital blockIdx.x.
+__global__ __launch_bounds__(256) void k_fft_linear_pk(const PkFft* __restrict__ tab) {
+    extern __shared__ double2 X[];
+    const PkFft a = tab[blockIdx.y];
+    fft_linear_block(X, a.orb, a.grid, a.n, a.log2n, a.tw, a.fexp, blockIdx.x);
+}
+
+int pk_linear_fft(const PkFft* tab, int count, int M, size_t smem, hipStream_t st) {
+    k_fft_linear_pk<<<dim3(M, count), 256, smem, st>>>(tab);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
 }
 
 // Lengths that are not powers of two: direct transforms, twiddles
@@ -432,17 +452,38 @@ __device__ __forceinline__ void hint_block(int e, const double2* __restrict__ or
 
 // Both in one launch: blocks [0, M^2) are the hob elements (i = b / M,
 // j = b % M, only i <= j work), the next M^4 blocks the hint elements.
-__global__ __launch_bounds__(256) void k_orbital_matrices(const double2* __restrict__ orb, int M, int n, double dx,
-                                                          double a2, double2 a1, const double* __restrict__ V,
-                                                          double g, double2* __restrict__ hob,
-                                                          double2* __restrict__ hint) {
-    __shared__ double red[8];
-    const int b = blockIdx.x, mm = M * M;
+__device__ __forceinline__ void orbital_matrices_block(int b, const double2* __restrict__ orb, int M, int n,
+                                                       double dx, double a2, double2 a1,
+                                                       const double* __restrict__ V, double g,
+                                                       double2* __restrict__ hob, double2* __restrict__ hint,
+                                                       double* red) {
+    const int mm = M * M;
     if (b < mm) {
         if (b % M >= b / M) hob_block(b / M, b % M, orb, M, n, dx, a2, a1, V, hob, red);
     } else {
         hint_block(b - mm, orb, M, n, dx, g, hint, red);
     }
+}
+
+__global__ __launch_bounds__(256) void k_orbital_matrices(const double2* __restrict__ orb, int M, int n, double dx,
+                                                          double a2, double2 a1, const double* __restrict__ V,
+                                                          double g, double2* __restrict__ hob,
+                                                          double2* __restrict__ hint) {
+    __shared__ double red[8];
+    orbital_matrices_block(blockIdx.x, orb, M, n, dx, a2, a1, V, g, hob, hint, red);
+}
+
+// Packed form: instance blockIdx.y, matrix element blockIdx.x.
+__global__ __launch_bounds__(256) void k_orbital_matrices_pk(const PkOm* __restrict__ tab, int M) {
+    __shared__ double red[8];
+    const PkOm a = tab[blockIdx.y];
+    orbital_matrices_block(blockIdx.x, a.orb, M, a.n, a.dx, a.a2, a.a1, a.V, a.g, a.hob, a.hint, red);
+}
+
+int pk_orbital_matrices(const PkOm* tab, int count, int M, hipStream_t st) {
+    k_orbital_matrices_pk<<<dim3(M * M + M * M * M * M, count), 256, 0, st>>>(tab, M);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
 }
 
 int orbital_matrices(OrbWork* w, const double2* orb, double2* hob, double2* hint, hipStream_t st) {
@@ -599,8 +640,21 @@ __global__ __launch_bounds__(256) void k_nl_coef(int M, const double2* __restric
     nl_coef_block(M, hob, hint, rho, rinv, out, lpart);
 }
 
+// Packed form: one block per instance.
+__global__ __launch_bounds__(256) void k_nl_coef_pk(const PkNlCoef* __restrict__ tab, int M) {
+    extern __shared__ double2 lpart[];
+    const PkNlCoef a = tab[blockIdx.x];
+    nl_coef_block(M, a.hob, a.hint, a.rho2, a.rinv, a.out, lpart);
+}
+
 // LDS of k_nl_coef (the split L sums for M <= 8).
 static size_t nl_coef_smem(int M) { return M <= 8 ? sizeof(double2) * M * M * M * (M + 1) / 2 : 0; }
+
+int pk_nl_coef(const PkNlCoef* tab, int count, int M, hipStream_t st) {
+    k_nl_coef_pk<<<count, 256, nl_coef_smem(M), st>>>(tab, M);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
 
 // out = y0 + dt (a_0 k_0 + a_1 k_1 + ...), the stage sums of an explicit
 // tableau in ascending stage order.
@@ -770,28 +824,33 @@ template <int M, int NS> constexpr bool nl_fold() {
     return sizeof(double2) * (M * 64 * (NS + 1) + M * P * M + M * M + M * M * P) <= 60 * 1024;
 }
 
+template <int M, int NS> struct NlRkLds {
+    static constexpr int P = M * (M + 1) / 2;
+    static constexpr bool FOLD = nl_fold<M, NS>();
+    double2 y0[M][64], ks[NS][M][64];
+    double2 tables[FOLD ? M * P * M + M * M : 1], lpart[FOLD ? M * M * P : 1];
+};
+
+// Block xb of grid points (points 64 xb .. 64 xb + 63).
 template <int M, int NS>
-__global__ __launch_bounds__(64 * M) void k_nl_rk_point(double2* __restrict__ orb, int n, RkTab tab, double dt,
-                                                        double g, double2 tf, const double* __restrict__ V,
-                                                        const double2* __restrict__ coef_g,
-                                                        const double2* __restrict__ hob,
-                                                        const double2* __restrict__ hint,
-                                                        const double2* __restrict__ rho2,
-                                                        const double2* __restrict__ rinv) {
+__device__ __forceinline__ void nl_rk_point_block(NlRkLds<M, NS>& L, int xb, double2* __restrict__ orb, int n,
+                                                  const RkTab& tab, double dt, double g, double2 tf,
+                                                  const double* __restrict__ V, const double2* __restrict__ coef_g,
+                                                  const double2* __restrict__ hob, const double2* __restrict__ hint,
+                                                  const double2* __restrict__ rho2,
+                                                  const double2* __restrict__ rinv) {
     constexpr int P = M * (M + 1) / 2;
     constexpr bool FOLD = nl_fold<M, NS>();
-    __shared__ double2 y0[M][64], ks[NS][M][64];
-    __shared__ double2 tables[FOLD ? M * P * M + M * M : 1], lpart[FOLD ? M * M * P : 1];
     const double2* coef = coef_g;
     if constexpr (FOLD) {
-        nl_coef_block(M, hob, hint, rho2, rinv, tables, lpart);
-        coef = tables;  // the barrier below orders the table writes before use
+        nl_coef_block(M, hob, hint, rho2, rinv, L.tables, L.lpart);
+        coef = L.tables;  // the barrier below orders the table writes before use
     }
     // k is uniform across the wave: the table reads below become scalar loads
     const int xl = threadIdx.x & 63, k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int x = blockIdx.x * 64 + xl;
+    const int x = xb * 64 + xl;
     const bool in = x < n;
-    y0[k][xl] = in ? orb[(size_t) k * n + x] : make_double2(0, 0);
+    L.y0[k][xl] = in ? orb[(size_t) k * n + x] : make_double2(0, 0);
     const double vx = in ? V[x] : 0.0;
     const double2 mi_tf = make_double2(tf.y, -tf.x);
     const double2* A = coef + (size_t) k * P * M;
@@ -803,12 +862,12 @@ __global__ __launch_bounds__(64 * M) void k_nl_rk_point(double2* __restrict__ or
 #pragma unroll
         for (int a = 0; a < M; a++) {
             if (s == 0) {
-                f[a] = y0[a][xl];
+                f[a] = L.y0[a][xl];
             } else {
-                double2 acc = cscale(ks[0][a][xl], tab.a[s * NS]);
+                double2 acc = cscale(L.ks[0][a][xl], tab.a[s * NS]);
 #pragma unroll
-                for (int j = 1; j < s; j++) acc = acc + cscale(ks[j][a][xl], tab.a[s * NS + j]);
-                f[a] = y0[a][xl] + cscale(acc, dt);
+                for (int j = 1; j < s; j++) acc = acc + cscale(L.ks[j][a][xl], tab.a[s * NS + j]);
+                f[a] = L.y0[a][xl] + cscale(acc, dt);
             }
         }
         double2 acc[M];
@@ -832,14 +891,36 @@ __global__ __launch_bounds__(64 * M) void k_nl_rk_point(double2* __restrict__ or
         double2 total = cscale(cubic, g) + cscale(fk, vx);
 #pragma unroll
         for (int j = 0; j < M; j++) total = total - cmul(Lk[j], f[j]);
-        ks[s][k][xl] = cmul(mi_tf, total);
+        L.ks[s][k][xl] = cmul(mi_tf, total);
         __syncthreads();
     }
     if (!in) return;
-    double2 acc = cscale(ks[0][k][xl], tab.b[0]);
+    double2 acc = cscale(L.ks[0][k][xl], tab.b[0]);
 #pragma unroll
-    for (int j = 1; j < NS; j++) acc = acc + cscale(ks[j][k][xl], tab.b[j]);
-    orb[(size_t) k * n + x] = y0[k][xl] + cscale(acc, dt);
+    for (int j = 1; j < NS; j++) acc = acc + cscale(L.ks[j][k][xl], tab.b[j]);
+    orb[(size_t) k * n + x] = L.y0[k][xl] + cscale(acc, dt);
+}
+
+template <int M, int NS>
+__global__ __launch_bounds__(64 * M) void k_nl_rk_point(double2* __restrict__ orb, int n, RkTab tab, double dt,
+                                                        double g, double2 tf, const double* __restrict__ V,
+                                                        const double2* __restrict__ coef_g,
+                                                        const double2* __restrict__ hob,
+                                                        const double2* __restrict__ hint,
+                                                        const double2* __restrict__ rho2,
+                                                        const double2* __restrict__ rinv) {
+    __shared__ NlRkLds<M, NS> L;
+    nl_rk_point_block<M, NS>(L, blockIdx.x, orb, n, tab, dt, g, tf, V, coef_g, hob, hint, rho2, rinv);
+}
+
+// Packed form: instance blockIdx.y; blocks past the instance's grid return.
+template <int M, int NS>
+__global__ __launch_bounds__(64 * M) void k_nl_rk_point_pk(const PkNl* __restrict__ tab_i, RkTab tab) {
+    __shared__ NlRkLds<M, NS> L;
+    const PkNl a = tab_i[blockIdx.y];
+    if ((int) blockIdx.x * 64 >= a.n) return;  // whole block, before any barrier
+    nl_rk_point_block<M, NS>(L, blockIdx.x, a.orb, a.n, tab, a.dt, a.g, a.tf, a.V, a.coef, a.hob, a.hint, a.rho2,
+                             a.rinv);
 }
 
 constexpr double kRk4A_[4 * 4] = {0, 0, 0, 0, 0.5, 0, 0, 0, 0, 0.5, 0, 0, 0, 0, 1, 0};
@@ -893,6 +974,55 @@ static int nl_rk_pointwise(OrbWork* w, double2* orb, const double2* hob, const d
         RKP_M(6)
         RKP_M(7)
         RKP_M(8)
+    }
+#undef RKP_M
+#undef RKP
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
+bool pk_nl_folds(int M, int ns) {
+    switch (M * 8 + ns) {
+#define FOLD_CASE(m)                                  \
+    case m * 8 + 2: return nl_fold<m, 2>();           \
+    case m * 8 + 4: return nl_fold<m, 4>();           \
+    case m * 8 + 6: return nl_fold<m, 6>();
+        FOLD_CASE(1) FOLD_CASE(2) FOLD_CASE(3) FOLD_CASE(4) FOLD_CASE(5) FOLD_CASE(6) FOLD_CASE(7) FOLD_CASE(8)
+#undef FOLD_CASE
+        default: return false;
+    }
+}
+
+// The packed sub-step: the tableau of nl_rk_pointwise for ns stages.
+int pk_nl_rk(const PkNl* tab_i, int count, int M, int ns, unsigned max_blocks, hipStream_t st) {
+    RkTab tab{};
+    const double *A = kRk4A_, *b = kRk4B_;
+    if (ns == 2) A = kRk2A, b = kRk2B;
+    else if (ns == 6) A = kRk5A, b = kRk5B;
+    else if (ns != 4) {
+        set_error("packed RK sub-step: 2, 4 or 6 stages");
+        return kInvalidArgument;
+    }
+    for (int i = 0; i < ns * ns; i++) tab.a[i] = A[i];
+    for (int i = 0; i < ns; i++) tab.b[i] = b[i];
+    const dim3 grid(max_blocks, count);
+#define RKP(m, s) k_nl_rk_point_pk<m, s><<<grid, 64 * m, 0, st>>>(tab_i, tab)
+#define RKP_M(m)                     \
+    case m:                          \
+        if (ns == 2) RKP(m, 2);      \
+        else if (ns == 4) RKP(m, 4); \
+        else RKP(m, 6);              \
+        break;
+    switch (M) {
+        RKP_M(1)
+        RKP_M(2)
+        RKP_M(3)
+        RKP_M(4)
+        RKP_M(5)
+        RKP_M(6)
+        RKP_M(7)
+        RKP_M(8)
+        default: set_error("packed RK sub-step: 1 <= norb <= 8"); return kInvalidArgument;
     }
 #undef RKP_M
 #undef RKP
@@ -1014,9 +1144,8 @@ __device__ __forceinline__ void gram_schmidt_block(double2* __restrict__ F, int 
 // every orbital is then set to the first (the periodic copy the whole step
 // applies after the orbitals).
 constexpr size_t kGsStageBytes = 64 * 1024;
-__global__ __launch_bounds__(256) void k_gram_schmidt(double2* __restrict__ Fg, int M, int n, double dx, int tail,
-                                                      int staged) {
-    extern __shared__ double2 Fs[];
+__device__ __forceinline__ void gram_schmidt_whole(double2* Fs, double2* __restrict__ Fg, int M, int n, double dx,
+                                                   int tail, int staged) {
     const int tot = M * n;
     double2* F = Fg;
     if (staged) {
@@ -1034,6 +1163,30 @@ __global__ __launch_bounds__(256) void k_gram_schmidt(double2* __restrict__ Fg, 
         __syncthreads();
         for (int t = threadIdx.x; t < tot; t += blockDim.x) Fg[t] = Fs[t];
     }
+}
+
+__global__ __launch_bounds__(256) void k_gram_schmidt(double2* __restrict__ Fg, int M, int n, double dx, int tail,
+                                                      int staged) {
+    extern __shared__ double2 Fs[];
+    gram_schmidt_whole(Fs, Fg, M, n, dx, tail, staged);
+}
+
+// Packed form: one block per instance.
+__global__ __launch_bounds__(256) void k_gram_schmidt_pk(const PkGs* __restrict__ tab, int M) {
+    extern __shared__ double2 Fs[];
+    const PkGs a = tab[blockIdx.x];
+    gram_schmidt_whole(Fs, a.F, M, a.n, a.dx, a.tail, a.staged);
+}
+
+int pk_gram_schmidt(const PkGs* tab, int count, int M, size_t smem, hipStream_t st) {
+    k_gram_schmidt_pk<<<count, 256, smem, st>>>(tab, M);
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
+size_t pk_gs_smem(int M, int n) {
+    const size_t bytes = sizeof(double2) * M * n;
+    return bytes <= kGsStageBytes ? bytes : 0;
 }
 
 int orthonormalize(OrbWork* w, double2* orb, hipStream_t st, bool periodic_tail) {

@@ -13,6 +13,7 @@
 #pragma once
 
 #include "hamiltonian_impl.hpp"
+#include "pack.hpp"
 #include "sil_device.hpp"
 
 namespace mctdhb {
@@ -44,10 +45,8 @@ __device__ __forceinline__ void block_sum_slots(const double (&acc)[kSmallSlots]
 // the row gathers read LDS; w_{it+1} is written there by the pass that
 // forms it, while the pass over w_it has finished.
 template <int M, int IDX, bool STAGE>
-__global__ __launch_bounds__(kSmallSilBlock) void k_sil_small(RowCtx ctx, const double2* __restrict__ cf,
-                                                              const uint32_t* __restrict__ Wg, int ldw,
-                                                              SmallSilArgs a) {
-    extern __shared__ __align__(16) unsigned char smem[];
+__device__ __forceinline__ void sil_small_block(unsigned char* smem, const RowCtx& ctx, const double2* __restrict__ cf,
+                                                const uint32_t* __restrict__ Wg, int ldw, const SmallSilArgs& a) {
     const int wsz = (M - 1) * ldw;
     uint32_t* Wl = (uint32_t*) smem;
     double* sql = (double*) (smem + ((wsz * 4 + 15) & ~15));
@@ -170,6 +169,22 @@ __global__ __launch_bounds__(kSmallSilBlock) void k_sil_small(RowCtx ctx, const 
     for (uint32_t i = threadIdx.x; i < n; i += T) C[i] = cscale(C[i], sc);
 }
 
+template <int M, int IDX, bool STAGE>
+__global__ __launch_bounds__(kSmallSilBlock) void k_sil_small(RowCtx ctx, const double2* __restrict__ cf,
+                                                              const uint32_t* __restrict__ Wg, int ldw,
+                                                              SmallSilArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    sil_small_block<M, IDX, STAGE>(smem, ctx, cf, Wg, ldw, a);
+}
+
+// Packed form: one workgroup per instance.
+template <int M, int IDX, bool STAGE>
+__global__ __launch_bounds__(kSmallSilBlock) void k_sil_small_pk(const PkSil* __restrict__ tab) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const PkSil& p = tab[blockIdx.x];
+    sil_small_block<M, IDX, STAGE>(smem, p.ctx, p.cf, p.W, p.ldw, p.a);
+}
+
 template <int M>
 int HcFused<M>::sil_small(const Space* S, const void* cf, const SmallSilArgs& a, int index, hipStream_t st) {
     if (!sil_small_fits(S, a.lm)) {
@@ -192,6 +207,19 @@ int HcFused<M>::sil_small(const Space* S, const void* cf, const SmallSilArgs& a,
         else
             k_sil_small<M, kIdxTables, false><<<1, kSmallSilBlock, smem, st>>>(ctx, (const double2*) cf, S->d_W,
                                                                               S->ldw, a);
+    }
+    MCTDHB_HIP_CHECK(hipGetLastError());
+    return kOk;
+}
+
+template <int M>
+int HcFused<M>::sil_small_pk(const PkSil* tab, int count, int index, bool stage, size_t smem, hipStream_t st) {
+    if (index == kIndexOnTheFly) {
+        if (stage) k_sil_small_pk<M, kIdxFly, true><<<count, kSmallSilBlock, smem, st>>>(tab);
+        else k_sil_small_pk<M, kIdxFly, false><<<count, kSmallSilBlock, smem, st>>>(tab);
+    } else {
+        if (stage) k_sil_small_pk<M, kIdxTables, true><<<count, kSmallSilBlock, smem, st>>>(tab);
+        else k_sil_small_pk<M, kIdxTables, false><<<count, kSmallSilBlock, smem, st>>>(tab);
     }
     MCTDHB_HIP_CHECK(hipGetLastError());
     return kOk;

@@ -128,6 +128,11 @@ def lib():
     L.mctdhb_step_graph_launch.argtypes = [_vp, _vp]
     L.mctdhb_step_graph_launch_many.argtypes = [_cpp, _i, _cpp, _i, _i]
     L.mctdhb_step_graph_destroy.argtypes = [_vp]
+    L.mctdhb_pack_eligible.argtypes = [_vp, _vp, _vp, _i, C.POINTER(_i)]
+    L.mctdhb_pack_create.argtypes = [_i, _cpp, _cpp, _cpp, _i, _cpp, _cpp, _cpp, _cpp, _cpp, _cpp, _cpp, _cpp]
+    L.mctdhb_pack_groups.argtypes = [_vp, C.POINTER(_i)]
+    L.mctdhb_pack_step.argtypes = [_vp, _cpp, _i, _i]
+    L.mctdhb_pack_destroy.argtypes = [_vp]
     # reference-named drop-in layer
     L.space_dimension.argtypes = [C.c_uint16, C.c_uint16]
     L.space_dimension.restype = C.c_uint32
@@ -406,6 +411,57 @@ def launch_graphs(graphs, streams, rounds: int = 1):
     ga = (_vp * n)(*[g._h for g in graphs])
     sa = (_vp * len(streams))(*[_stream(s) for s in streams])
     check(lib().mctdhb_step_graph_launch_many(ga, n, sa, len(streams), int(rounds)))
+
+
+def pack_eligible(space: DeviceSpace, sil: DeviceSil, orbital: DeviceOrbital, use_fft=True) -> bool:
+    """Whether this solve's step can run in a Pack (FFT linear step of a
+    power-of-two length, split step, 2 <= M <= 7, the one-workgroup FP64 SIL
+    step)."""
+    ok = _i(0)
+    check(lib().mctdhb_pack_eligible(space.handle, sil._h, orbital.handle, int(bool(use_fft)), C.byref(ok)))
+    return bool(ok.value)
+
+
+class Pack:
+    """The steps of many small solves as one kernel launch per stage
+    (mctdhb_pack_*).  `solves` holds, per solve, (space, sil, orbital, orb,
+    coef, hob, hint, rho1, rho2, rinv) as for StepGraph; each solve's
+    results are those of its own step graph, bit for bit."""
+
+    def __init__(self, solves, use_fft=True):
+        n = len(solves)
+        if n == 0:
+            raise ValueError("a pack needs at least one solve")
+        cols = list(zip(*solves))
+        sp = (_vp * n)(*[x.handle for x in cols[0]])
+        sl = (_vp * n)(*[x._h for x in cols[1]])
+        ob = (_vp * n)(*[x.handle for x in cols[2]])
+        bufs = [(_vp * n)(*[_ptr(t) for t in cols[k]]) for k in range(3, 10)]
+        h = _vp()
+        check(lib().mctdhb_pack_create(n, sp, sl, ob, int(bool(use_fft)), *bufs, C.byref(h)))
+        self._h = h
+        self._keep = solves
+
+    @property
+    def groups(self) -> int:
+        g = _i(0)
+        check(lib().mctdhb_pack_groups(self._h, C.byref(g)))
+        return g.value
+
+    def step(self, streams, rounds: int = 1):
+        """`rounds` steps of every solve, group g on streams[g % len(streams)]."""
+        if rounds <= 0:
+            return
+        sa = (_vp * len(streams))(*[_stream(s) for s in streams])
+        check(lib().mctdhb_pack_step(self._h, sa, len(streams), int(rounds)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mctdhb_pack_destroy(self._h)
+            self._h = None
+            self._keep = None
+
+    __del__ = close
 
 
 class Multiconf:

@@ -221,21 +221,50 @@ def errors_vs_fp64(got: tuple, ref: tuple) -> dict:
 
 class Batch:
     """Instances stepped together: step k of every instance is enqueued on
-    stream (instance index mod len(streams)) before step k + 1 of any."""
+    stream (instance index mod len(streams)) before step k + 1 of any.
 
-    def __init__(self, instances: list[Instance], nstreams: int = 4):
+    packed: the instances whose step qualifies (amd.pack_eligible: the
+    one-workgroup FP64 step of the small spaces) run as an amd.Pack, one
+    kernel launch per stage for each group of alike instances, on streams of
+    their own; the others replay their step graphs as before."""
+
+    def __init__(self, instances: list[Instance], nstreams: int = 4, packed: bool = False,
+                 pack_streams: int = 8):
         import torch
+        import mctdhb_amd as amd
 
         self.instances = instances
         self.streams = [torch.cuda.Stream() for _ in range(max(1, nstreams))]
+        self.pack = None
+        self.pack_streams = []
+        self.graphs = [inst.graph for inst in instances]
+        if packed:
+            inpack = [i for i in instances if amd.pack_eligible(i.space, i.sil, i.orbital)]
+            if inpack:
+                t = lambda i: (i.space, i.sil, i.orbital, i.t["orb"], i.t["coef"], i.t["hob"], i.t["hint"],
+                               i.t["r1"], i.t["r2"], i.t["rinv"])
+                self.pack = amd.Pack([t(i) for i in inpack])
+                self.pack_streams = [torch.cuda.Stream() for _ in range(max(1, min(pack_streams,
+                                                                                    self.pack.groups)))]
+                ids = {id(i) for i in inpack}
+                self.graphs = [i.graph for i in instances if id(i) not in ids]
+        self.packed_count = len(instances) - len(self.graphs)
 
     def run(self, steps: int):
-        """steps rounds; in each, instance i's step graph on stream i mod
-        len(streams) (one library call for all the launches)."""
+        """steps rounds: the pack's groups (one graph of stage launches per
+        group and round) and the other instances' step graphs, instance i's
+        on stream i mod len(streams); one library call each."""
         import mctdhb_amd as amd
 
-        amd.launch_graphs([inst.graph for inst in self.instances], self.streams, steps)
+        if self.pack is not None:
+            self.pack.step(self.pack_streams, steps)
+        amd.launch_graphs(self.graphs, self.streams, steps)
 
     def synchronize(self):
-        for s in self.streams:
+        for s in self.streams + self.pack_streams:
             s.synchronize()
+
+    def close(self):
+        if self.pack is not None:
+            self.pack.close()
+            self.pack = None

@@ -407,9 +407,12 @@ def batched_bench(args):
     {2..6} (NC <= 2e5), Ng in {256, 512, 1024} on [-L, L) with V(L) = 200,
     double-well traps, g log-uniform in [0.01, 10], dt = 0.005, m = 5
     (mctdhb_amd.batch).
-    Every instance's step is a HIP graph; the batch replays them round-robin
-    over a few streams.  For comparison the same instances are also stepped
-    one after another, and together, for min(K, 10) steps each."""
+    The instances of the one-workgroup FP64 step run packed (one launch per
+    stage for each group of alike instances, mctdhb_pack_*); the others
+    replay their own step graphs round-robin over a few streams.  For
+    comparison the same instances are also stepped one after another, and
+    together, for min(K, 10) steps each, and every instance by its own graph
+    for K steps."""
     import torch
     from mctdhb_amd import batch
 
@@ -418,7 +421,7 @@ def batched_bench(args):
     insts = [batch.Instance(d) for d in draws]
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
-    b = batch.Batch(insts, nstreams=args.batch_streams)
+    b = batch.Batch(insts, nstreams=args.batch_streams, packed=not args.batch_graphs)
     K = args.batch_steps
     b.run(1)
     b.synchronize()
@@ -463,12 +466,23 @@ def batched_bench(args):
     b.synchronize()
     together = time.perf_counter() - t0
     log(f"{K2} steps each: one after another {serial:.2f} s, together {together:.2f} s")
+    # every instance by its own step graph (no pack), K steps
+    gb = batch.Batch(insts, nstreams=args.batch_streams)
+    gb.run(1)
+    gb.synchronize()
+    t0 = time.perf_counter()
+    gb.run(K)
+    gb.synchronize()
+    graphs_wall = time.perf_counter() - t0
+    log(f"graphs only: {len(insts)} x {K} steps in {graphs_wall:.2f} s")
     e = [inst.energy_per_particle() for inst in insts]
     finite = [v for v in e if math.isfinite(v)]
     dims = np.array([d.dim for d in draws])
     out = {"workload": f"{len(insts)} uneven imaginary-time ground states x {K} steps (dt=0.005, m=5)",
            "instances": len(insts), "steps_each": K, "streams": args.batch_streams,
            "instance_steps_per_s": len(insts) * K / wall, "wall_s": wall, "setup_s": setup_s,
+           "packed_instances": b.packed_count, "pack_groups": b.pack.groups if b.pack is not None else 0,
+           "graphs_only_instance_steps_per_s": len(insts) * K / graphs_wall,
            "comparison_steps_each": K2, "one_after_another_s": serial, "together_s": together,
            "concurrency_gain": serial / together,
            "dim_min_median_max": [int(dims.min()), int(np.median(dims)), int(dims.max())],
@@ -481,6 +495,7 @@ def batched_bench(args):
                               "natural_occupations_rel_max": worst("natural_occupations_rel"),
                               "natural_occupations_abs_max": worst("natural_occupations_abs"),
                               "rho1_rel_fro_max": worst("rho1_rel_fro"), "worst_occupations": worst_occ}}
+    b.close()
     for inst in insts:
         inst.close()
     return out
@@ -585,6 +600,8 @@ def main():
     ap.add_argument("--batch-instances", type=int, default=128)
     ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--batch-streams", type=int, default=8)
+    ap.add_argument("--batch-graphs", action="store_true",
+                    help="batched workload: every instance by its own step graph (no packed steps)")
     ap.add_argument("--no-batched", action="store_true", help="skip the many-instance timing")
     ap.add_argument("--batched-only", action="store_true", help="only the many-instance timing (its JSON object)")
     ap.add_argument("--serve-rate", type=float, default=200.0)

@@ -419,6 +419,31 @@ int mctdhb_step_graph_launch_many(mctdhb_step_graph* const* graphs, int n, void*
                                   int rounds);
 int mctdhb_step_graph_destroy(mctdhb_step_graph* g);
 
+/* Packed steps: the step of mctdhb_propagate_step_dev (reference
+ * driver.c:159-202, mctdhb_propagate_step, once per solve) for many small
+ * solves at once, one kernel launch per stage over a table of the solves'
+ * arguments in device memory instead of one launch per stage and solve.
+ * A solve qualifies when its step takes the FFT linear part (grid_size - 1 a
+ * power of two), the split-step orbital integrator, 2 <= norb <= 7 and the
+ * one-workgroup FP64 SIL step (complex128 workspace, NC <= 8192); eligible
+ * receives 1 or 0 (then mctdhb_last_error names the reason).  create groups
+ * the n solves by the stages' kernel variants (norb, Runge-Kutta order,
+ * imaginary or real time, SIL index scheme and LDS staging) and records
+ * each group's step once as a graph; the buffers and the scalars of the
+ * moment of creation are kept as with mctdhb_step_graph_create.  step
+ * launches `rounds` steps of every group, group g on streams[g % nstreams];
+ * a solve's results are those of its own mctdhb_propagate_step_dev, bit for
+ * bit.  A solve must belong to one pack or step graph at a time. */
+typedef struct mctdhb_pack mctdhb_pack;
+int mctdhb_pack_eligible(const mctdhb_space* space, const mctdhb_sil* sil, const mctdhb_orbital* w, int use_fft,
+                         int* eligible);
+int mctdhb_pack_create(int n, mctdhb_space* const* spaces, mctdhb_sil* const* sils, mctdhb_orbital* const* w,
+                       int use_fft, void* const* orb, void* const* coef, void* const* hob, void* const* hint,
+                       void* const* rho1, void* const* rho2, void* const* rinv, mctdhb_pack** out);
+int mctdhb_pack_groups(const mctdhb_pack* p, int* groups);
+int mctdhb_pack_step(mctdhb_pack* p, void* const* streams, int nstreams, int rounds);
+int mctdhb_pack_destroy(mctdhb_pack* p);
+
 #ifdef __cplusplus
 }
 #endif
