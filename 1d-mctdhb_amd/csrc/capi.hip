@@ -576,9 +576,9 @@ struct mctdhb_step_graph {
 
 // Records what body enqueues on a private stream into g.  Everything body
 // allocates or tunes on first use must already be done: the capture takes
-// kernels and device copies only.
+// kernels and device copies only.  The capture executes nothing, so it waits
+// for no work in flight (other solves keep running while one is admitted).
 template <typename Body> static int capture_graph(Body body, mctdhb_step_graph* g) {
-    MCTDHB_HIP_CHECK(hipDeviceSynchronize());
     hipStream_t cs = nullptr;
     MCTDHB_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     hipGraph_t graph = nullptr;

@@ -381,7 +381,8 @@ int space_create(int npar, int norb, int flags, Space** out) {
         }
     }
 #undef ALLOC
-    MCTDHB_HIP_CHECK(hipDeviceSynchronize());
+    // the build ran on stream st alone: other streams' work is not waited for
+    MCTDHB_HIP_CHECK(hipStreamSynchronize(st));
     *out = S;
     return kOk;
 }

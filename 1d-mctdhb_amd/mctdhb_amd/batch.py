@@ -119,7 +119,9 @@ class Instance:
         self.orbital.orthonormalize(t["orb"])
         self.orbital.orbital_matrices(t["orb"], t["hob"], t["hint"])
         amd.density_sync(self.space, t["coef"], t["r1"], t["r2"], t["rinv"])
-        torch.cuda.synchronize()
+        # the initial state is formed on the current stream: wait for that
+        # stream alone, not for other solves' steps on their own streams
+        torch.cuda.current_stream().synchronize()
         self.graph = amd.StepGraph(self.space, self.sil, self.orbital, t["orb"], t["coef"], t["hob"], t["hint"],
                                    t["r1"], t["r2"], t["rinv"])
         self.orbital_real = self.graph_real = None
