@@ -122,6 +122,7 @@ class Instance:
         # the initial state is formed on the current stream: wait for that
         # stream alone, not for other solves' steps on their own streams
         torch.cuda.current_stream().synchronize()
+        self.energy0 = self.energy_per_particle()  # E/N of the initial state
         self.graph = amd.StepGraph(self.space, self.sil, self.orbital, t["orb"], t["coef"], t["hob"], t["hint"],
                                    t["r1"], t["r2"], t["rinv"])
         self.orbital_real = self.graph_real = None
@@ -155,15 +156,24 @@ class Instance:
 
     def health(self, tol: float = 1e-6) -> str:
         """Empty when the state is usable, else the first problem found:
-        rho1 not finite, its trace not N, or coefficients not of unit norm
-        (every step keeps both, in imaginary time by renormalising and in
-        real time by unitarity).  Call after a synchronisation."""
+        rho1 not finite, an energy above the grid's bound, an imaginary-time
+        solve (no quench) whose energy rose above its initial one (imaginary
+        time lowers the energy: a rise means the split step is unstable at
+        this dt, as when g (N - 1) dt is large, and the result is one of many
+        trajectories rounding picks between), rho1's trace not N, or
+        coefficients not of unit norm (every step keeps both, in imaginary
+        time by renormalising and in real time by unitarity).  Call after a
+        synchronisation."""
         M = self.draw.norb
         r1 = self.t["r1"].cpu().numpy().reshape(M, M)
         if not np.all(np.isfinite(r1)):
             return "rho1 not finite"
-        if abs(self.energy_per_particle()) > self.energy_bound():
+        e = self.energy_per_particle()
+        if abs(e) > self.energy_bound():
             return "energy above the grid's bound (diverged)"
+        e0 = getattr(self, "energy0", None)
+        if self.quench_g is None and e0 is not None and e > e0 + 1e-9 * abs(e0):
+            return "imaginary-time energy rose (unstable step)"
         if abs(np.trace(r1).real - self.draw.npar) > tol * self.draw.npar:
             return "trace of rho1 is not N"
         nrm = float(np.linalg.norm(self.t["coef"].cpu().numpy()))

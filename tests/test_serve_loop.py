@@ -154,3 +154,25 @@ def test_energy_bound_covers_the_grid_and_flags_divergence():
     inst.quench_g = -3.0  # an attractive quench, larger in magnitude
     assert inst.energy_bound() > b0
     assert 1.4e30 > inst.energy_bound()  # a diverged quench's E/N is far above it
+
+
+def test_health_flags_a_rising_imaginary_time_energy():
+    """An imaginary-time solve whose final energy is above its initial one
+    fails (the split step was unstable); quenches are not judged by it."""
+    import torch
+
+    d = batch.Draw(2, 2, 256, 1.0, 2.0, 0.5, 0)
+    inst = batch.Instance.__new__(batch.Instance)
+    inst.draw, inst.quench_g = d, None
+    e = 3.0
+    inst.t = {"r1": torch.tensor([2.0, 0, 0, 0], dtype=torch.complex128),
+              "hob": torch.tensor([e, 0, 0, e], dtype=torch.complex128),
+              "r2": torch.zeros(16, dtype=torch.complex128), "hint": torch.zeros(16, dtype=torch.complex128),
+              "coef": torch.tensor([1.0, 0, 0], dtype=torch.complex128)}
+    assert inst.energy_per_particle() == e  # tr(rho1 h) / N
+    inst.energy0 = e + 1.0
+    assert inst.health() == ""
+    inst.energy0 = e - 1.0
+    assert inst.health() == "imaginary-time energy rose (unstable step)"
+    inst.quench_g = 1.0
+    assert inst.health() == ""
