@@ -222,15 +222,18 @@ static int scan_strides(const uint32_t* d_cnt, uint32_t n, uint32_t m2, uint32_t
     return kOk;
 }
 
+// The temporaries are stream-ordered allocations on st: freeing them waits
+// for no other stream's work (hipFree would synchronise the device).
 static int build_pair_slots(Space* S, hipStream_t st) {
     const uint32_t dim = S->dim;
     uint32_t *pf = nullptr, *sf = nullptr, *pp = nullptr, *sp = nullptr;
     const size_t b = sizeof(uint32_t) * dim;
-    if (hipMalloc((void**) &pf, b) != hipSuccess || hipMalloc((void**) &sf, b) != hipSuccess ||
-        hipMalloc((void**) &pp, b) != hipSuccess || hipMalloc((void**) &sp, b) != hipSuccess) {
-        hipFree(pf);
-        hipFree(sf);
-        hipFree(pp);
+    if (hipMallocAsync((void**) &pf, b, st) != hipSuccess || hipMallocAsync((void**) &sf, b, st) != hipSuccess ||
+        hipMallocAsync((void**) &pp, b, st) != hipSuccess || hipMallocAsync((void**) &sp, b, st) != hipSuccess) {
+        if (pf) hipFreeAsync(pf, st);
+        if (sf) hipFreeAsync(sf, st);
+        if (pp) hipFreeAsync(pp, st);
+        hipStreamSynchronize(st);
         set_error("device allocation failed (pair slots)");
         return kOutOfMemory;
     }
@@ -256,11 +259,11 @@ static int build_pair_slots(Space* S, hipStream_t st) {
             rc = kDeviceError;
         }
     }
+    hipFreeAsync(pf, st);
+    hipFreeAsync(sf, st);
+    hipFreeAsync(pp, st);
+    hipFreeAsync(sp, st);
     hipStreamSynchronize(st);
-    hipFree(pf);
-    hipFree(sf);
-    hipFree(pp);
-    hipFree(sp);
     return rc;
 }
 

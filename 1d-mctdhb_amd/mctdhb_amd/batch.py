@@ -141,15 +141,24 @@ class Instance:
     def footprint_bytes(self) -> int:
         return footprint_bytes(self.draw.dim, self.draw.norb, self.draw.ngrid + 1)
 
-    def energy_per_particle(self) -> float:
-        """E / N from rho1, rho2, h and g (reference total_energy,
-        integrator_monitor.c:56-88); call after a synchronisation."""
+    def small_matrices(self) -> tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
+        """rho1, h (M x M), rho2, g (M, M, M, M; flat index k + l M + s M^2
+        + q M^3 on numpy axes (q, s, l, k)) on the host, in one
+        device-to-host copy; call after a synchronisation."""
+        import torch
+
         M = self.draw.norb
-        r1 = self.t["r1"].cpu().numpy().reshape(M, M)
-        h = self.t["hob"].cpu().numpy().reshape(M, M)
-        # flat k + l M + s M^2 + q M^3 -> numpy axes (q, s, l, k)
-        r2 = self.t["r2"].cpu().numpy().reshape(M, M, M, M)
-        g = self.t["hint"].cpu().numpy().reshape(M, M, M, M)
+        t = self.t
+        flat = torch.cat([t["r1"], t["hob"], t["r2"], t["hint"]]).cpu().numpy()
+        m2, m4 = M * M, M ** 4
+        return (flat[:m2].reshape(M, M), flat[m2:2 * m2].reshape(M, M),
+                flat[2 * m2:2 * m2 + m4].reshape(M, M, M, M), flat[2 * m2 + m4:].reshape(M, M, M, M))
+
+    def energy_per_particle(self, mats=None) -> float:
+        """E / N from rho1, rho2, h and g (reference total_energy,
+        integrator_monitor.c:56-88); mats: small_matrices() already read.
+        Call after a synchronisation."""
+        r1, h, r2, g = mats if mats is not None else self.small_matrices()
         # rho2[k,l,s,q] g[k,l,q,s]: swap the two leading axes of g
         e = np.sum(r1 * h) + 0.5 * np.sum(r2 * g.transpose(1, 0, 2, 3))
         return float(e.real) / self.draw.npar
@@ -164,11 +173,13 @@ class Instance:
         coefficients not of unit norm (every step keeps both, in imaginary
         time by renormalising and in real time by unitarity).  Call after a
         synchronisation."""
-        M = self.draw.norb
-        r1 = self.t["r1"].cpu().numpy().reshape(M, M)
+        import torch
+
+        mats = self.small_matrices()
+        r1 = mats[0]
         if not np.all(np.isfinite(r1)):
             return "rho1 not finite"
-        e = self.energy_per_particle()
+        e = self.energy_per_particle(mats)
         if abs(e) > self.energy_bound():
             return "energy above the grid's bound (diverged)"
         e0 = getattr(self, "energy0", None)
@@ -176,7 +187,7 @@ class Instance:
             return "imaginary-time energy rose (unstable step)"
         if abs(np.trace(r1).real - self.draw.npar) > tol * self.draw.npar:
             return "trace of rho1 is not N"
-        nrm = float(np.linalg.norm(self.t["coef"].cpu().numpy()))
+        nrm = float(torch.linalg.vector_norm(self.t["coef"]))  # on the device: one scalar comes back
         if not abs(nrm - 1.0) <= tol:
             return "coefficient norm is not 1"
         return ""
@@ -196,9 +207,9 @@ class Instance:
     def observables(self) -> tuple[float, np.ndarray, np.ndarray]:
         """(E/N, rho1, natural occupations in ascending order); call after a
         synchronisation."""
-        M = self.draw.norb
-        r1 = self.t["r1"].cpu().numpy().reshape(M, M)
-        return self.energy_per_particle(), r1, np.sort(np.linalg.eigvalsh(r1)) / self.draw.npar
+        mats = self.small_matrices()
+        r1 = mats[0]
+        return self.energy_per_particle(mats), r1, np.sort(np.linalg.eigvalsh(r1)) / self.draw.npar
 
     def occupations(self) -> np.ndarray:
         M = self.draw.norb

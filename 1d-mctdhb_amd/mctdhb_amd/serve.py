@@ -219,7 +219,8 @@ class Server:
                     rec = {"rid": a.req.rid, "kind": a.req.kind, "dim": a.req.draw.dim, "norb": a.req.draw.norb,
                            "npar": a.req.draw.npar, "steps": a.total, "arrival": a.req.arrival,
                            "admitted": a.admitted, "done": done, "time_to_solution": done - a.req.arrival,
-                           "energy_per_particle": a.inst.energy_per_particle(), "status": status,
+                           "energy_per_particle": obs[0] if obs is not None else a.inst.energy_per_particle(),
+                           "status": status,
                            "reason": reason}
                     rec["mixed"] = bool(getattr(a.inst, "prec", 0))
                     if obs is not None:
