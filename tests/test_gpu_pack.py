@@ -118,3 +118,68 @@ def test_pack_refuses_what_it_does_not_cover(amd):
             amd.Pack([(inst.space, inst.sil, inst.orbital, t["orb"], t["coef"], t["hob"], t["hint"], t["r1"],
                        t["r2"], t["rinv"])])
         inst.close()
+
+
+def _solve(amd, npar, norb, grid, bounds, rk_order, imagtime, seed):
+    """One solve built from the low-level API, its initial state formed."""
+    import math
+
+    import torch
+
+    from mctdhb_amd import inputs
+
+    x = np.linspace(-8.0, 8.0, grid)
+    pot = 0.5 * x ** 2
+    space = amd.DeviceSpace(npar, norb, tables=False)
+    sil = amd.DeviceSil(space, 5, amd.FP64, amd.INDEX_FLY)
+    orbital = amd.DeviceOrbital(norb, grid, x[0], x[-1], 0.005, 0.7, pot, imagtime=imagtime, bounds=bounds,
+                                rk_order=rk_order)
+    orb = inputs.hermite_gauss(norb, x, seed=seed, noise=1e-3)
+    if bounds == 1001:
+        orb[:, -1] = orb[:, 0]
+    dim = math.comb(npar + norb - 1, norb - 1)
+    t = {"orb": torch.from_numpy(np.ascontiguousarray(orb.ravel())).cuda(),
+         "coef": torch.from_numpy(inputs.coefficients(dim, seed)).cuda()}
+    for k, n in (("hob", norb ** 2), ("hint", norb ** 4), ("r1", norb ** 2), ("r2", norb ** 4),
+                 ("rinv", norb ** 2)):
+        t[k] = torch.zeros(n, dtype=torch.complex128, device="cuda")
+    orbital.orthonormalize(t["orb"])
+    orbital.orbital_matrices(t["orb"], t["hob"], t["hint"])
+    amd.density_sync(space, t["coef"], t["r1"], t["r2"], t["rinv"])
+    torch.cuda.synchronize()
+    return {"objs": (space, sil, orbital), "t": t}
+
+
+def test_pack_rk_orders_bounds_and_real_time(amd):
+    """Runge-Kutta orders 2 and 5 (2 and 6 stages) beside 4, zero and
+    periodic bounds, imaginary and real time, in one pack."""
+    import torch
+
+    cases = [(8, 3, 257, 1000, 2, True), (6, 4, 129, 1001, 5, True), (10, 2, 257, 1000, 4, False),
+             (5, 5, 513, 1001, 5, False), (7, 3, 129, 1001, 2, False)]
+    print("pack cases", cases, flush=True)
+
+    def tup(s):
+        t = s["t"]
+        return (*s["objs"], t["orb"], t["coef"], t["hob"], t["hint"], t["r1"], t["r2"], t["rinv"])
+
+    ref = [_solve(amd, *c, seed=40 + i) for i, c in enumerate(cases)]
+    graphs = [amd.StepGraph(*tup(s)) for s in ref]
+    for _ in range(4):
+        for g in graphs:
+            g.launch()
+    torch.cuda.synchronize()
+    packed = [_solve(amd, *c, seed=40 + i) for i, c in enumerate(cases)]
+    for s in packed:
+        assert amd.pack_eligible(*s["objs"])
+    pk = amd.Pack([tup(s) for s in packed])
+    assert pk.groups == len(cases)  # every case differs in M, stage count or time direction
+    st = torch.cuda.Stream()
+    pk.step([st], 4)
+    st.synchronize()
+    for c, a, b in zip(cases, ref, packed):
+        for k in KEYS:
+            assert np.array_equal(a["t"][k].cpu().numpy(), b["t"][k].cpu().numpy()), (c, k)
+    pk.close()
+    for g in graphs:
+        g.close()
