@@ -783,6 +783,17 @@ extern "C" int mctdhb_pack_create(int n, mctdhb_space* const* spaces, mctdhb_sil
             return kInvalidArgument;
         }
     }
+    // the solves of a group run every stage at once: each needs workspaces of its own
+    // (the space holds the density partial sums, the SIL workspace the Krylov vectors,
+    // the orbital workspace the Runge-Kutta tables)
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < i; j++)
+            if (spaces[i] == spaces[j] || sils[i] == sils[j] || orbitals[i] == orbitals[j] ||
+                coef[i] == coef[j] || orb[i] == orb[j]) {
+                set_error("instances " + std::to_string(j) + " and " + std::to_string(i) +
+                          " share a workspace or buffer; every solve in a pack needs its own");
+                return kInvalidArgument;
+            }
     // groups by the stages' template parameters, members in input order
     std::vector<std::vector<int>> members;
     std::vector<PackGroup> groups;

@@ -433,7 +433,9 @@ int mctdhb_step_graph_destroy(mctdhb_step_graph* g);
  * moment of creation are kept as with mctdhb_step_graph_create.  step
  * launches `rounds` steps of every group, group g on streams[g % nstreams];
  * a solve's results are those of its own mctdhb_propagate_step_dev, bit for
- * bit.  A solve must belong to one pack or step graph at a time. */
+ * bit.  The solves of a pack share no space, SIL or orbital workspace and no
+ * coefficient or orbital buffer (create refuses that), and a solve must not
+ * be stepped by a pack and a step graph at the same time. */
 typedef struct mctdhb_pack mctdhb_pack;
 int mctdhb_pack_eligible(const mctdhb_space* space, const mctdhb_sil* sil, const mctdhb_orbital* w, int use_fft,
                          int* eligible);

@@ -183,3 +183,22 @@ def test_pack_rk_orders_bounds_and_real_time(amd):
     pk.close()
     for g in graphs:
         g.close()
+
+
+def test_pack_refuses_shared_workspaces(amd):
+    """Two entries over one space (its density partial sums) or one
+    orbital workspace would race inside a packed launch."""
+    d = batch.draw_ground_states(1, seed=3, max_dim=2000)[0]
+    a, b = batch.Instance(d), batch.Instance(d)
+
+    def tup(i, space=None, orbital=None):
+        t = i.t
+        return (space or i.space, i.sil, orbital or i.orbital, t["orb"], t["coef"], t["hob"], t["hint"], t["r1"],
+                t["r2"], t["rinv"])
+
+    for bad in ([tup(a), tup(b, space=a.space)], [tup(a), tup(b, orbital=a.orbital)], [tup(a), tup(a)]):
+        with pytest.raises(amd.MctdhbError):
+            amd.Pack(bad)
+    amd.Pack([tup(a), tup(b)]).close()
+    a.close()
+    b.close()
