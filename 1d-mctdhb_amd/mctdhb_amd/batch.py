@@ -141,18 +141,35 @@ class Instance:
     def footprint_bytes(self) -> int:
         return footprint_bytes(self.draw.dim, self.draw.norb, self.draw.ngrid + 1)
 
+    def _split(self, flat: np.ndarray):
+        M = self.draw.norb
+        m2, m4 = M * M, M ** 4
+        return (flat[:m2].reshape(M, M), flat[m2:2 * m2].reshape(M, M),
+                flat[2 * m2:2 * m2 + m4].reshape(M, M, M, M), flat[2 * m2 + m4:].reshape(M, M, M, M))
+
     def small_matrices(self) -> tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
         """rho1, h (M x M), rho2, g (M, M, M, M; flat index k + l M + s M^2
         + q M^3 on numpy axes (q, s, l, k)) on the host, in one
         device-to-host copy; call after a synchronisation."""
         import torch
 
-        M = self.draw.norb
         t = self.t
-        flat = torch.cat([t["r1"], t["hob"], t["r2"], t["hint"]]).cpu().numpy()
-        m2, m4 = M * M, M ** 4
-        return (flat[:m2].reshape(M, M), flat[m2:2 * m2].reshape(M, M),
-                flat[2 * m2:2 * m2 + m4].reshape(M, M, M, M), flat[2 * m2 + m4:].reshape(M, M, M, M))
+        return self._split(torch.cat([t["r1"], t["hob"], t["r2"], t["hint"]]).cpu().numpy())
+
+    def snapshot(self, stream=None, tag: str = "imag"):
+        """Enqueue on `stream` a copy of rho1, h, rho2 and g into pinned host
+        memory, without waiting: read by health() once the stream has passed
+        it (after the request's completion event).  tag: "imag" (during the
+        relaxation), "relaxed" (before the first real-time step) or
+        "quenched" (after it)."""
+        import torch
+
+        t = self.t
+        with torch.cuda.stream(stream):
+            src = torch.cat([t["r1"], t["hob"], t["r2"], t["hint"]])
+            buf = torch.empty(src.shape, dtype=src.dtype, pin_memory=True)
+            buf.copy_(src, non_blocking=True)
+        self.snaps = getattr(self, "snaps", []) + [(tag, buf)]
 
     def energy_per_particle(self, mats=None) -> float:
         """E / N from rho1, rho2, h and g (reference total_energy,
@@ -163,16 +180,31 @@ class Instance:
         e = np.sum(r1 * h) + 0.5 * np.sum(r2 * g.transpose(1, 0, 2, 3))
         return float(e.real) / self.draw.npar
 
+    # A real-time run whose energy moved by more than this fraction of the
+    # energy right after the quench has not resolved the dynamics at its dt
+    # (stable quenches of the served mix drift by 0.05-2 %).
+    REAL_TIME_DRIFT = 0.1
+    # A relaxation whose energy rose by more than this fraction between two
+    # snapshots is unstable.  The split step's imaginary-time energy is not
+    # strictly monotone (with weakly occupied orbitals it rises by 0.1 % and
+    # more over 8 steps at dt = 0.005), while an unstable one rises by tens
+    # of percent (g (N - 1) = 1185: 105 -> 120 -> 50 -> 97 over 10-step spans).
+    RELAX_RISE = 0.1
+
     def health(self, tol: float = 1e-6) -> str:
         """Empty when the state is usable, else the first problem found:
         rho1 not finite, an energy above the grid's bound, an imaginary-time
-        solve (no quench) whose energy rose above its initial one (imaginary
-        time lowers the energy: a rise means the split step is unstable at
-        this dt, as when g (N - 1) dt is large, and the result is one of many
-        trajectories rounding picks between), rho1's trace not N, or
-        coefficients not of unit norm (every step keeps both, in imaginary
-        time by renormalising and in real time by unitarity).  Call after a
-        synchronisation."""
+        relaxation whose energy rose (by more than RELAX_RISE, relative) from
+        one snapshot to the next (imaginary time lowers the energy: a rise means
+        the split step is unstable at this dt, as when g (N - 1) dt is
+        large, and the result is one of many trajectories rounding picks
+        between), a quench whose real-time energy was not conserved to
+        REAL_TIME_DRIFT, rho1's trace not N, or coefficients not of unit norm
+        (every step keeps both, in imaginary time by renormalising and in
+        real time by unitarity).  The energies come from the initial state,
+        the snapshots (serve takes one every 32 relaxation steps and two
+        around a quench's first real-time step) and, for a relaxation alone,
+        the final state.  Call after a synchronisation."""
         import torch
 
         mats = self.small_matrices()
@@ -183,8 +215,19 @@ class Instance:
         if abs(e) > self.energy_bound():
             return "energy above the grid's bound (diverged)"
         e0 = getattr(self, "energy0", None)
-        if self.quench_g is None and e0 is not None and e > e0 + 1e-9 * abs(e0):
-            return "imaginary-time energy rose (unstable step)"
+        snaps = getattr(self, "snaps", [])
+        energy = lambda buf: self.energy_per_particle(self._split(buf.numpy()))
+        relax = ([e0] if e0 is not None else []) + [energy(b) for tag, b in snaps if tag in ("imag", "relaxed")]
+        if self.quench_g is None:
+            relax.append(e)
+        for a, b in zip(relax, relax[1:]):
+            if b > a + self.RELAX_RISE * abs(a):
+                return "imaginary-time energy rose (unstable step)"
+        quenched = [b for tag, b in snaps if tag == "quenched"]
+        if self.quench_g is not None and quenched:
+            eq = energy(quenched[0])
+            if not abs(e - eq) <= self.REAL_TIME_DRIFT * abs(eq):
+                return "real-time energy not conserved (unstable step)"
         if abs(np.trace(r1).real - self.draw.npar) > tol * self.draw.npar:
             return "trace of rho1 is not N"
         nrm = float(torch.linalg.vector_norm(self.t["coef"]))  # on the device: one scalar comes back

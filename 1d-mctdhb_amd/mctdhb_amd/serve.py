@@ -123,13 +123,29 @@ class _Active:
         self.total = req.imag_steps + req.real_steps
         self.event = make_event()
         self.enqueued_all = False
+        self.next_snap = self.SNAP_EVERY
+
+    # relaxation steps between two snapshots
+    SNAP_EVERY = 32
 
     def advance(self, k: int):
+        # snapshots for health(), copied on this request's stream without
+        # waiting: every SNAP_EVERY relaxation steps, and the relaxed state
+        # and the state one real-time step after a quench
+        snap = getattr(self.inst, "snapshot", None)
         for _ in range(min(k, self.total - self.done_steps)):
             if self.done_steps < self.req.imag_steps:
                 self.inst.step(self.stream)
+                if snap is not None and self.done_steps + 1 == self.next_snap < self.req.imag_steps:
+                    snap(self.stream, "imag")
+                    self.next_snap += self.SNAP_EVERY
             else:
+                first = self.done_steps == self.req.imag_steps
+                if first and snap is not None:
+                    snap(self.stream, "relaxed")
                 self.inst.step_real(self.stream)
+                if first and snap is not None:
+                    snap(self.stream, "quenched")
             self.done_steps += 1
         if self.done_steps == self.total and not self.enqueued_all:
             self.event.record(self.stream)

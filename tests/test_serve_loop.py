@@ -174,5 +174,26 @@ def test_health_flags_a_rising_imaginary_time_energy():
     assert inst.health() == ""
     inst.energy0 = e - 1.0
     assert inst.health() == "imaginary-time energy rose (unstable step)"
-    inst.quench_g = 1.0
+    inst.energy0 = e * (1 - 1e-3)  # a rise well below RELAX_RISE: not unstable
     assert inst.health() == ""
+    inst.quench_g = 1.0
+    assert inst.health() == ""  # a quench without snapshots: the final state only
+
+    def snap(energy):  # rho1, h, rho2, g as serve's snapshots hold them, E/N = energy
+        t = inst.t
+        return torch.cat([t["r1"], torch.tensor([energy, 0, 0, energy], dtype=torch.complex128), t["r2"],
+                          t["hint"]])
+
+    inst.energy0 = e + 1.0
+    # relaxed below E0 through the chunks; final within 10 % of the quench
+    inst.snaps = [("imag", snap(e + 0.8)), ("relaxed", snap(e + 0.5)), ("quenched", snap(e * 1.05))]
+    assert inst.health() == ""
+    inst.snaps = [("imag", snap(e - 0.5)), ("relaxed", snap(e)), ("quenched", snap(e))]  # rose between chunks
+    assert inst.health() == "imaginary-time energy rose (unstable step)"
+    inst.snaps = [("relaxed", snap(e)), ("quenched", snap(2.0 * e))]  # final e against 2e after the quench
+    assert inst.health() == "real-time energy not conserved (unstable step)"
+    inst.quench_g = None  # a relaxation: its chunks, then the final state
+    inst.snaps = [("imag", snap(e + 0.5))]
+    assert inst.health() == ""
+    inst.snaps = [("imag", snap(e - 0.5))]  # the final e is above the last chunk's
+    assert inst.health() == "imaginary-time energy rose (unstable step)"
