@@ -626,7 +626,7 @@ def main():
                          "for hardware-counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU sharding is a design note, not an implementation)", file=sys.stderr)
         sys.exit(2)
 
