@@ -778,6 +778,35 @@ def main():
         "after_steps": n_mixed,
     }
 
+    # ---- the same mixed step with C handed over in host memory, as the
+    # reference-named host entry points take it: host -> device copy, step,
+    # device -> host copy, host clock around each step.  Reported beside the
+    # value, never as it.
+    def host_boundary(nsteps=5):
+        sil = amd.DeviceSil(space, lm, amd.MIXED, amd.INDEX_FLY)
+        Cd = C0.clone()
+        pinned = torch.empty(C0.shape, dtype=C0.dtype, pin_memory=True)
+        pinned.copy_(C0)
+        res = {"what": f"one mixed SIL step with C ({C0.numel() * 16 / 1e6:.1f} MB) copied from host memory "
+                       f"and back per step, {nsteps} steps, host clock"}
+        for name, hbuf in (("pinned", pinned), ("pageable", C0.cpu())):
+            sil.step(Cd, H, G, dt, False, stream=stream)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(nsteps):
+                Cd.copy_(hbuf, non_blocking=True)
+                sil.step(Cd, H, G, dt, False, stream=stream)
+                hbuf.copy_(Cd, non_blocking=True)
+                stream.synchronize()
+            el = (time.perf_counter() - t0) / nsteps
+            res[name] = {"ms_per_step": el * 1e3, "steps_per_s": 1.0 / el}
+        sil.close()
+        log(f"SIL step across the host boundary: pinned {res['pinned']['ms_per_step']:.3f} ms, "
+            f"pageable {res['pageable']['ms_per_step']:.3f} ms")
+        return res
+
+    host_bound = host_boundary()
+
     steps_mixed = args.steps / (t_mixed / 1e3)
     steps_fly = args.steps / (t_fly / 1e3)
     steps_tab = args.steps / (t_tab / 1e3)
@@ -854,6 +883,7 @@ def main():
         "mixed_fly_vs_fp64_tables": steps_mixed / steps_tab,
         "lanczos_iter_done": done_mixed,
         "errors_mixed_vs_fp64": errors,
+        "host_boundary": host_bound,
         "hc_ms": hc_ms,
         "hc_per_s_mixed": 1e3 / hc_ms["mixed_fly"],
         "hc_effective_gbs": {"fp64_tables": ab_f64["hc"] / hc_ms["fp64_tables"] / 1e6,
