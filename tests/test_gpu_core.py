@@ -156,7 +156,9 @@ def test_sil_step_matches_oracle(amd, orc, npar, norb, lm, imag):
 
 
 @pytest.mark.parametrize("npar,norb,lm,index", [(10, 4, 5, 1), (10, 4, 5, 0), (2, 2, 5, 1), (30, 2, 15, 1),
-                                                (6, 7, 8, 1), (12, 5, 6, 0), (3, 3, 12, 1)])
+                                                (6, 7, 8, 1), (12, 5, 6, 0), (3, 3, 12, 1),
+                                                # staged in LDS above 64 KiB: NC 4845, 3003, 5456
+                                                (16, 5, 5, 1), (8, 7, 5, 1), (30, 4, 6, 0)])
 @pytest.mark.parametrize("imag", [False, True])
 def test_one_workgroup_sil_step_matches_multikernel_and_oracle(amd, orc, npar, norb, lm, index, imag):
     """Small FP64 spaces take the one-workgroup step (mctdhb_sil_set_small);
