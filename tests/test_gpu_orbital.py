@@ -98,7 +98,8 @@ def test_nonlinear_rhs(orc, amd, imagtime):
     assert gu.rel_err(gu.host(der), ref) < 1e-12
 
 
-@pytest.mark.parametrize("grid", [257, 258, 2049])
+# 5 orbitals: staged in LDS up to 160 KiB (1025, 2000 points), not at 2049
+@pytest.mark.parametrize("grid", [257, 258, 1025, 2000, 2049])
 def test_orthonormalize(orc, amd, grid):
     norb = 5
     x, eq, w, orb = _setup(orc, amd, norb, grid, True)
