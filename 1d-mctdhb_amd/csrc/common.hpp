@@ -12,6 +12,9 @@
 
 namespace mctdhb {
 
+// LDS one workgroup may allocate on gfx950 (MI355X): 160 KiB.
+constexpr size_t kLdsPerWorkgroup = 160 * 1024;
+
 // ------------------------------------------------------------ error model
 // Every C-ABI entry point returns an int status (0 = ok) and leaves a
 // message in a thread-local buffer (mctdhb_last_error()).

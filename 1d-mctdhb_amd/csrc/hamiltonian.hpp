@@ -159,7 +159,6 @@ inline size_t sil_small_smem(const Space* S) {
 // The staged form keeps the current vector and the occupation rows in LDS
 // when everything fits in the 160 KiB one workgroup may allocate on gfx950
 // (NC up to about 6000).
-constexpr size_t kLdsPerWorkgroup = 160 * 1024;
 inline size_t sil_small_stage_bytes(const Space* S) { return (size_t) S->dim * (sizeof(double2) + 8); }
 inline bool sil_small_stage(const Space* S) {
     return sil_small_smem(S) + sil_small_stage_bytes(S) <= kLdsPerWorkgroup;

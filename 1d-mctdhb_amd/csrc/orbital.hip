@@ -1143,7 +1143,7 @@ __device__ __forceinline__ void gram_schmidt_block(double2* __restrict__ F, int 
 // LDS, not the cache hierarchy).  With tail set, the last grid point of
 // every orbital is then set to the first (the periodic copy the whole step
 // applies after the orbitals).
-constexpr size_t kGsStageBytes = 160 * 1024;  // what one workgroup may allocate on gfx950
+constexpr size_t kGsStageBytes = kLdsPerWorkgroup;
 __device__ __forceinline__ void gram_schmidt_whole(double2* Fs, double2* __restrict__ Fg, int M, int n, double dx,
                                                    int tail, int staged) {
     const int tot = M * n;
